@@ -1,0 +1,174 @@
+"""Benchmark of the batched FFT/STFT path on one MI355X.
+
+Default workload (one "step" = one pass of the hot path over one batch): batched R2C STFT,
+N = 1024, Hann window 1024, hop 256, half spectrum + magnitude, over 2^24 samples of
+float32 PCM (noise + 8 sinusoids) -> 65533 frames x 513 bins, written as the three
+output planes of fft_execute_batched. Inputs and outputs are resident in HBM before the
+timed region; the step is one fft_execute_batched_device call (one kernel launch).
+
+`value` is frames/s over the timed steps; `roofline` prices the kernel against HBM
+(algorithmic bytes = 4*hop + 12*(N/2+1) per frame: each new input sample read once, each
+of the three output planes written once); `cpu_baseline` times the oracle's restatement
+of the reference CPU path on the same workload on this host's cores.
+
+Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="stft", choices=["stft", "c2c1024"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--variant", type=int, default=-1)
+    return ap.parse_args()
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def timed(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / 1000.0, wall
+
+
+def cpu_baseline_stft(pcm, n, W, H, seconds):
+    from oracle import fp64, ref
+    taps = fp64.build_window(1, W)
+    threads = ref.default_threads()
+    runs = 0
+    t0 = time.perf_counter()
+    frames = 0
+    while True:
+        got, _, _, _ = ref.execute_batched(pcm, n, W, H, ctx_pad=0, pad_mode=0, kernel=2, transform=1,
+                                           reduce_magnitude=True, half_spectrum=True, window_taps=taps,
+                                           threads=threads)
+        frames += got
+        runs += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{runs} full runs of the 2^24-sample workload (65533 frames each), Stockham code 2, "
+                      f"restated reference CPU path, {threads} threads, {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU sharding is a design note, "
+                                   "not a measured configuration", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from fftfree_amd import CffiFft
+    from fftfree_amd.workloads import complex_normal, noise_plus_sines
+
+    torch.cuda.set_device(0)
+    ff = CffiFft()
+    if args.workload == "stft":
+        n, W, H, L = 1024, 1024, 256, 1 << 24
+        p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,
+                         half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=0)
+        if args.variant >= 0:
+            assert ff.set_variant(p, args.variant)
+        pcm = noise_plus_sines(L, seed=2)
+        frames = ff.frame_count(p, L, 0)
+        bins = n // 2 + 1
+        d_pcm = torch.from_numpy(pcm).cuda()
+        out = torch.empty((3, frames, bins), device="cuda")
+
+        def step():
+            ff.stft_device(p, d_pcm, out[0], out[1], out[2], pad_mode=0)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per_launch = gpu_s / args.steps
+        alg_bytes = frames * (4 * H + 12 * bins)
+        achieved = alg_bytes / per_launch / 1e9
+        traffic = read_traffic("stft_r2c_n1024")
+        line = {
+            "metric": "STFT frames/s & %HBM peak at N=1024; ragged-batch and fwd+bwd vs plain, 1 GPU",
+            "value": frames * args.steps / gpu_s,
+            "unit": "frames/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": per_launch * 1e3,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic: 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
+            "config": {"workload": "batched R2C STFT, Hann 1024, hop 256, half spectrum + magnitude, 2^24 samples "
+                                   "-> 65533 frames x 513 bins, three planes",
+                       "n": n, "window": W, "hop": H, "samples": L, "frames": frames, "variant": ff.variants(p)[
+                           max(args.variant, 0)]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                         "algorithmic_bytes_per_launch": alg_bytes},
+            "wall_s": wall,
+        }
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_baseline_stft(pcm, n, W, H, args.cpu_seconds)
+    else:
+        n, B = 1024, 64
+        p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
+        x = torch.from_numpy(complex_normal(B, n, seed=1)).cuda()
+        y = torch.empty_like(x)
+
+        def step():
+            ff.c2c_device(p, x, y)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per_launch = gpu_s / args.steps
+        alg_bytes = 2 * 8 * n * B
+        achieved = alg_bytes / per_launch / 1e9
+        line = {"metric": "C2C transforms/s, N=1024 batch 64", "value": B * args.steps / gpu_s,
+                "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": per_launch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
+                "config": {"workload": "C2C forward N=1024 x 64, Stockham"},
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+    print(json.dumps(line), flush=True)
+    ff.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+"""Builds libfft_cffi.so (gfx950) in-tree with hipcc.
+
+Compile flags: -O3, gfx950 only, and packed FP32 instructions disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops) for every HIP translation unit.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libfft_cffi.so"
+SOURCES = ["fft_kernels.hip", "plan.cpp", "fft_cffi.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+HIP_ONLY = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _compile(src: Path, obj: Path, verbose: bool) -> None:
+    cmd = [HIPCC, *COMMON]
+    if src.suffix == ".hip":
+        cmd += HIP_ONLY
+    cmd += ["-c", str(src), "-o", str(obj)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    # hipcc's host pass warns that it does not know the packed-fp32 feature; that is harmless
+    msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src.name}:\n{msg}")
+    if msg.strip() and verbose:
+        print(msg)
+
+
+def build(verbose: bool = False) -> Path:
+    out = PKG / "_build"
+    out.mkdir(exist_ok=True)
+    objs = []
+    for name in SOURCES:
+        src = CSRC / name
+        obj = out / (name + ".o")
+        _compile(src, obj, verbose)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    tmp.replace(LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

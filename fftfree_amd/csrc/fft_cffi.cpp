@@ -1,0 +1,886 @@
+// fft_cffi.cpp -- the C ABI (include/fft_cffi.h) over the gfx950 kernels.
+//
+// FftContext mirrors the reference context (fft_cffi.cpp:40-82): the init arguments,
+// windows built to N taps, a plan cache. It adds a HIP stream, device copies of the
+// windows and reusable device staging buffers. Host-pointer entry points stage through
+// device memory; the *_device entry points launch straight on caller buffers.
+//
+// Errors never cross the C boundary (reference: catch(...) at fft_cffi.cpp:1031-1033,
+// 1955-1973): they are reported on stderr and turned into NULL / 0.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/fft_cffi.h"
+#include "fft_kernels.hpp"
+#include "plan.hpp"
+
+namespace fftfree {
+namespace {
+
+// Reusable device allocation that only grows.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int device = 0;
+  void reserve(size_t want) {
+    if (want <= bytes) return;
+    release();
+    if (want == 0) return;
+    hip_check(hipMalloc(&ptr, want), "hipMalloc(staging)");
+    bytes = want;
+    (void)hipGetDevice(&device);
+  }
+  void release() {
+    if (ptr) {
+      (void)hipSetDevice(device);
+      (void)hipFree(ptr);
+    }
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+  ~DeviceBuffer() { release(); }
+};
+
+struct FftContext {
+  int N = 0;
+  bool inverse = false;
+  int kernel = 0;  // C-level code: 2 = Stockham, anything else = Cooley-Tukey
+  int radix = 0;
+  int pad_mode = 0;
+  std::vector<int> radix_pattern;
+  int window = 0;
+  int hop = 0;
+  int transform = 0;
+  int reduce_magnitude = 0;
+  int store_polar = 0;
+  int half_spectrum = 0;
+  int allow_outer_parallel = 1;
+  int allow_inner_parallel = 0;
+  int inner_threads = 0;
+  bool has_pool = false;  // the reference creates a WorkerPool here (fft_cffi.cpp:1928-1931)
+  int analysis_win_kind = 0;
+  int synthesis_win_kind = 0;
+  float win_param1 = 0.0f, win_param2 = 0.0f;
+  int win_norm_policy = 0;
+  int cola_mode = 0;
+  std::vector<float> analysis_win;
+  std::vector<float> synthesis_win;
+  bool windows_enabled = false;
+  int apply_ola = 0;
+  PlanRuntimeConfig cfg;
+  PlanCache cache;
+  std::shared_ptr<Plan> plan;
+
+  // device side (created on first use)
+  bool gpu_ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  StreamDispatcher dispatcher;
+  DeviceBuffer d_awin, d_swin, d_pcm, d_planes, d_cplx;
+  bool windows_dirty = true;
+
+  ~FftContext() {
+    d_awin.release();
+    d_swin.release();
+    d_pcm.release();
+    d_planes.release();
+    d_cplx.release();
+    plan.reset();
+    cache.clear();
+    if (stream) {
+      (void)hipSetDevice(device);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+};
+
+bool verbose() {
+  static const bool v = [] {
+    const char* e = std::getenv("FFTFREE_VERBOSE");
+    return e && *e && *e != '0';
+  }();
+  return v;
+}
+
+void report(const char* where, const char* what) { std::fprintf(stderr, "[fftfree-amd] %s: %s\n", where, what); }
+
+bool is_pow2(size_t v) { return v != 0 && (v & (v - 1)) == 0; }
+size_t next_pow2(size_t v) {
+  size_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// build_window / normalize_window (fft_cffi.cpp:163-208), same double formulas.
+std::vector<float> build_window(int kind, int W, float p1, float /*p2*/) {
+  const double PI = 3.141592653589793238462643383279502884;
+  std::vector<float> w(static_cast<size_t>(std::max(W, 0)), 1.0f);
+  switch (kind) {
+    case FFT_WINDOW_HANN:
+    case FFT_WINDOW_KAISER:  // the reference uses Hann as its Kaiser placeholder
+      for (int n = 0; n < W; ++n) w[n] = float(0.5 * (1.0 - std::cos(2.0 * PI * n / (W - 1))));
+      break;
+    case FFT_WINDOW_HAMMING:
+      for (int n = 0; n < W; ++n) w[n] = float(0.54 - 0.46 * std::cos(2.0 * PI * n / (W - 1)));
+      break;
+    case FFT_WINDOW_BLACKMAN:
+      for (int n = 0; n < W; ++n)
+        w[n] = float(0.42 - 0.5 * std::cos(2.0 * PI * n / (W - 1)) + 0.08 * std::cos(4.0 * PI * n / (W - 1)));
+      break;
+    case FFT_WINDOW_TUKEY: {
+      const double alpha = (p1 == 0.0f ? 0.5 : p1);
+      const int L = std::max(1, W - 1);
+      for (int n = 0; n < W; ++n) {
+        const double x = double(n) / double(L);
+        if (x < alpha / 2) w[n] = float(0.5 * (1 + std::cos(PI * (2 * x / alpha - 1))));
+        else if (x <= 1 - alpha / 2) w[n] = 1.0f;
+        else w[n] = float(0.5 * (1 + std::cos(PI * (2 * x / alpha - 2 / alpha + 1))));
+      }
+      break;
+    }
+    default:
+      break;
+  }
+  return w;
+}
+
+void normalize_window(std::vector<float>& w, int policy) {
+  if (w.empty()) return;
+  if (policy == FFT_WINDOW_NORM_L2) {
+    double s2 = 0.0;
+    for (float v : w) s2 += double(v) * double(v);
+    s2 = std::sqrt(std::max(1e-30, s2));
+    for (auto& v : w) v = float(v / s2);
+  } else if (policy == FFT_WINDOW_NORM_AREA) {
+    double s = 0.0;
+    for (float v : w) s += double(v);
+    if (std::abs(s) > 1e-30)
+      for (auto& v : w) v = float(v / s);
+  }
+}
+
+int parse_kernel_token(std::string t) {
+  auto b = t.begin();
+  while (b != t.end() && std::isspace((unsigned char)*b)) ++b;
+  auto e = t.end();
+  while (e != b && std::isspace((unsigned char)*(e - 1))) --e;
+  t = std::string(b, e);
+  for (auto& c : t) c = (char)std::tolower((unsigned char)c);
+  if (t.empty() || t == "0" || t == "auto" || t == "default") return 0;
+  if (t == "cooleytukey" || t == "cooley" || t == "cooley-tukey" || t == "cooley_tukey" || t == "ct")
+    return FFT_KERNEL_COOLEYTUKEY;
+  if (t == "stockham" || t == "stock" || t == "sh") return FFT_KERNEL_STOCKHAM;
+  if (t == "external" || t == "ext" || t == "custom") return FFT_KERNEL_EXTERNAL;
+  char* end = nullptr;
+  const long v = std::strtol(t.c_str(), &end, 10);
+  if (end && *end == '\0' && (v == 0 || v == 1 || v == 2)) return (int)v;
+  return 0;
+}
+
+// FFTFREE_DEFAULT_KERNEL (fft_cffi.cpp:127-159), consulted by fft_init_ex only.
+int env_default_kernel() {
+  static const int k = [] {
+    const char* e = std::getenv("FFTFREE_DEFAULT_KERNEL");
+    return e ? parse_kernel_token(e) : 0;
+  }();
+  return k;
+}
+
+bool use_stockham(const FftContext* c) { return c->kernel == 2; }
+
+void bind_plan(FftContext* c) {
+  c->plan = c->cache.get_plan(c->N, c->inverse, c->cfg);
+  c->plan->use_kernel(use_stockham(c) ? KernelKind::Stockham : KernelKind::CooleyTukey);
+}
+
+void ensure_gpu(FftContext* c) {
+  if (!c->gpu_ready) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw HipFailure("no HIP device available");
+    hip_check(hipGetDevice(&c->device), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->dispatcher.set_stream(c->stream);
+    c->gpu_ready = true;
+  }
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  c->plan->ensure_device();
+  c->plan->set_dispatcher(&c->dispatcher);
+  if (c->windows_dirty) {
+    c->d_awin.reserve(std::max<size_t>(1, c->analysis_win.size()) * sizeof(float));
+    c->d_swin.reserve(std::max<size_t>(1, c->synthesis_win.size()) * sizeof(float));
+    if (!c->analysis_win.empty())
+      hip_check(hipMemcpy(c->d_awin.ptr, c->analysis_win.data(), c->analysis_win.size() * sizeof(float),
+                          hipMemcpyHostToDevice),
+                "upload analysis window");
+    if (!c->synthesis_win.empty())
+      hip_check(hipMemcpy(c->d_swin.ptr, c->synthesis_win.data(), c->synthesis_win.size() * sizeof(float),
+                          hipMemcpyHostToDevice),
+                "upload synthesis window");
+    c->windows_dirty = false;
+  }
+}
+
+// Frame count of fft_execute_batched (fft_cffi.cpp:713-750); 0 = refused.
+size_t frame_count(const FftContext* c, size_t pcm_len, int pad_mode, size_t max_frames) {
+  const int N = c->N;
+  if (N <= 0 || pcm_len == 0) return 0;
+  const int W = c->window ? c->window : N;
+  const int H = c->hop ? c->hop : W;
+  if (W <= 0 || H <= 0 || W > N) return 0;
+  const int pad = (pad_mode == 0) ? c->pad_mode : pad_mode;
+  size_t frames;
+  if (pcm_len < (size_t)W) {
+    if (pad != 1) return 0;
+    frames = 1;
+  } else {
+    frames = 1 + (pcm_len - (size_t)W) / (size_t)H;
+    const size_t last = (frames - 1) * (size_t)H;
+    const size_t remaining = pcm_len > last ? pcm_len - last : 0;
+    if (remaining < (size_t)W) {
+      if (pad == 1) {
+      } else if (pad == 2) {
+        frames -= 1;
+      } else {
+        return 0;
+      }
+    }
+  }
+  if (frames == 0) return 0;
+  if (max_frames != 0 && frames > max_frames) frames = max_frames;
+  return frames;
+}
+
+size_t bins_of(const FftContext* c) { return c->half_spectrum ? (size_t)(c->N / 2 + 1) : (size_t)c->N; }
+
+// The fused kernel assumes real input. Only the Cooley-Tukey C2R half-spectrum hook
+// (Hermitian rebuild of the framed input, eigen_fft.hpp:2501-2527) breaks that.
+bool fused_ok(const FftContext* c) {
+  if (c->N > stft_max_lds_n()) return false;
+  return !(!use_stockham(c) && c->transform == FFT_TRANSFORM_C2R && c->half_spectrum);
+}
+
+StftArgs stft_args(FftContext* c, const float* d_pcm, size_t pcm_len) {
+  StftArgs a;
+  a.pcm = d_pcm;
+  a.pcm_len = pcm_len;
+  a.window = c->window ? c->window : c->N;
+  a.hop = c->hop ? c->hop : a.window;
+  a.win = (c->windows_enabled && !c->analysis_win.empty()) ? c->d_awin.as<float>() : nullptr;
+  a.tw = c->plan->W;
+  a.n = c->N;
+  a.bins = (int)bins_of(c);
+  a.out_mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : (c->transform == FFT_TRANSFORM_C2R ? 1 : 0);
+  a.scale = c->inverse ? 1.0f / (float)c->N : 1.0f;
+  return a;
+}
+
+// Frames [f0, f0+nf) of the STFT into planes (already offset to frame f0).
+void run_stft_frames(FftContext* c, const StftArgs& base, size_t f0, size_t nf, float* re, float* im, float* mag,
+                     hipStream_t s) {
+  StftArgs a = base;
+  a.frames = nf;
+  a.frame_offset = f0;
+  a.o_re = re;
+  a.o_im = im;
+  a.o_mag = mag;
+  if (fused_ok(c)) {
+    hip_check(launch_stft(a, c->inverse ? 1 : 0, c->plan->stft_variant, s), "launch_stft");
+    return;
+  }
+  // generic path: frame -> plan kernel (hooks) -> planes, in chunks of <= 256 MiB
+  const size_t per_frame = (size_t)c->N * sizeof(float2);
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(nf, (size_t(256) << 20) / per_frame));
+  c->d_cplx.reserve(chunk * per_frame);
+  for (size_t b = 0; b < nf; b += chunk) {
+    const size_t cnt = std::min(chunk, nf - b);
+    StftArgs fa = a;
+    fa.frames = cnt;
+    fa.frame_offset = f0 + b;
+    hip_check(launch_frame_complex(fa, c->d_cplx.as<float2>(), s), "launch_frame_complex");
+    AxisLayout l;
+    l.base = c->d_cplx.as<float2>();
+    l.axis_size = c->N;
+    l.batch_size = (int64_t)cnt;
+    l.axis_stride = 1;
+    l.batch_stride = c->N;
+    c->dispatcher.set_stream(s);
+    fft_apply_axis(*c->plan, l);
+    c->dispatcher.set_stream(c->stream);
+    const size_t bins = bins_of(c);
+    hip_check(launch_planes(c->d_cplx.as<float2>(), cnt, c->N, (int)bins, c->store_polar, re ? re + b * bins : nullptr,
+                            im ? im + b * bins : nullptr, mag ? mag + b * bins : nullptr, s),
+              "launch_planes");
+  }
+}
+
+FftContext* make_context(size_t n, int threads, int lanes, int inverse, int kernel, int radix,
+                         const int* radix_pattern, size_t radix_pattern_len, int pad_mode, int window, int hop) {
+  auto c = std::make_unique<FftContext>();
+  c->N = (int)n;
+  c->inverse = inverse != 0;
+  c->kernel = kernel;
+  c->radix = radix;
+  c->pad_mode = pad_mode;
+  c->window = window;
+  c->hop = hop;
+  if (radix_pattern && radix_pattern_len > 0) c->radix_pattern.assign(radix_pattern, radix_pattern + radix_pattern_len);
+  c->cfg.threads = threads > 0 ? threads : 1;
+  c->cfg.lanes = lanes > 0 ? lanes : 1;
+  c->cfg.inverse = c->inverse;
+  c->cfg.radix = radix;
+  c->cfg.radix_pattern = c->radix_pattern;
+  return c.release();
+}
+
+const char* variant_desc(const FftContext* c, int idx, char* buf, size_t len) {
+  const VariantInfo* v = fused_ok(c) ? stft_variant(c->N, idx) : c2c_variant(c->N, idx);
+  if (!v) return nullptr;
+  std::snprintf(buf, len, "%s fpw=%d", v->radices, v->fpw);
+  return buf;
+}
+
+}  // namespace
+}  // namespace fftfree
+
+using fftfree::FftContext;
+using fftfree::HipFailure;
+
+extern "C" {
+
+void fft_pcm_to_channels(const float* in_pcm, float* out_real, float* out_imag, float* out_mag, size_t n,
+                         int threads) {
+  if (!in_pcm || !out_real || !out_imag || !out_mag || n == 0) return;
+  void* h = nullptr;
+  try {
+    const size_t plan_n = fftfree::next_pow2(n);
+    h = fft_init_full(plan_n, threads, 1, 0, 0, 0, nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1);
+    if (!h) throw std::runtime_error("context");
+    std::vector<float> re(plan_n), im(plan_n), mag(plan_n);
+    if (!fft_execute(h, in_pcm, re.data(), im.data(), mag.data(), n)) throw std::runtime_error("execute");
+    std::memcpy(out_real, re.data(), n * sizeof(float));
+    std::memcpy(out_imag, im.data(), n * sizeof(float));
+    std::memcpy(out_mag, mag.data(), n * sizeof(float));
+  } catch (...) {
+    // reference behaviour: zero the outputs (fft_cffi.cpp:489-510)
+    std::memset(out_real, 0, n * sizeof(float));
+    std::memset(out_imag, 0, n * sizeof(float));
+    std::memset(out_mag, 0, n * sizeof(float));
+  }
+  fft_free(h);
+}
+
+void* fft_init(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+               size_t radix_pattern_len, int pad_mode) {
+  return fft_init_ex(n, threads, lanes, inverse, kernel, radix, radix_pattern, radix_pattern_len, pad_mode, 0, 0);
+}
+
+void* fft_init_ex(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+                  size_t radix_pattern_len, int pad_mode, int window, int hop) {
+  // fft_cffi.cpp:540-658: N must be a power of two unless pad_mode == 1 (pad up).
+  if (n == 0) return nullptr;
+  FftContext* c = nullptr;
+  try {
+    if (kernel == 0) kernel = fftfree::env_default_kernel();
+    size_t plan_n = n;
+    if (!fftfree::is_pow2(plan_n)) {
+      if (pad_mode != 1) {
+        std::fprintf(stderr,
+                     "fft_init: refusing to auto-pad N=%zu -> %zu because pad_mode!=ALWAYS; set pad_mode=1 to "
+                     "permit padding or supply power-of-two N\n",
+                     n, fftfree::next_pow2(n));
+        return nullptr;
+      }
+      plan_n = fftfree::next_pow2(plan_n);
+    }
+    if (plan_n > (size_t)(1u << 30)) return nullptr;
+    c = fftfree::make_context(plan_n, threads, lanes, inverse, kernel, radix, radix_pattern, radix_pattern_len,
+                              pad_mode, window, hop);
+    c->cfg.threads = threads > 0 ? threads : 4;  // Plan default runtime threads
+    c->cfg.lanes = lanes > 0 ? lanes : 0;
+    fftfree::bind_plan(c);
+    return c;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_init", e.what());
+  } catch (...) {
+    fftfree::report("fft_init", "unknown exception");
+  }
+  delete c;
+  return nullptr;
+}
+
+void* fft_init_full(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+                    size_t radix_pattern_len, int pad_mode, int window, int hop, int transform,
+                    int reduce_magnitude, int store_polar, int half_spectrum, int allow_outer_parallel,
+                    int allow_inner_parallel, int inner_threads, int save_crash_logs, int silent_crash_reports) {
+  return fft_init_full_v2(n, threads, lanes, inverse, kernel, radix, radix_pattern, radix_pattern_len, pad_mode,
+                          window, hop, transform, reduce_magnitude, store_polar, half_spectrum, allow_outer_parallel,
+                          allow_inner_parallel, inner_threads, save_crash_logs, silent_crash_reports, 0, 0, 0, 0.0f,
+                          0.0f, 0, 0.0f, 0.0f, 0, 0);
+}
+
+void* fft_init_full_v2(size_t n, int threads, int lanes, int inverse, int kernel, int radix,
+                       const int* radix_pattern, size_t radix_pattern_len, int pad_mode, int window, int hop,
+                       int transform, int reduce_magnitude, int store_polar, int half_spectrum,
+                       int allow_outer_parallel, int allow_inner_parallel, int inner_threads, int save_crash_logs,
+                       int silent_crash_reports, int apply_windows, int apply_ola, int analysis_window_kind,
+                       float analysis_param1, float analysis_param2, int synthesis_window_kind,
+                       float synthesis_param1, float synthesis_param2, int window_norm_policy, int cola_mode) {
+  (void)save_crash_logs;
+  (void)silent_crash_reports;
+  FftContext* c = nullptr;
+  try {
+    // The reference's Plan constructor loops forever on N == 0; refuse it instead.
+    if (n == 0 || n > (size_t)(1u << 30)) return nullptr;
+    c = fftfree::make_context(n, threads, lanes, inverse, kernel, radix, radix_pattern, radix_pattern_len, pad_mode,
+                              window, hop);
+    c->transform = transform;
+    c->reduce_magnitude = reduce_magnitude;
+    c->store_polar = store_polar;
+    c->half_spectrum = half_spectrum;
+    c->allow_outer_parallel = allow_outer_parallel;
+    c->allow_inner_parallel = allow_inner_parallel;
+    c->inner_threads = inner_threads;
+    c->analysis_win_kind = analysis_window_kind;
+    c->synthesis_win_kind = synthesis_window_kind;
+    c->win_param1 = analysis_param1;
+    c->win_param2 = analysis_param2;
+    c->win_norm_policy = window_norm_policy;
+    c->cola_mode = cola_mode;
+    c->windows_enabled = apply_windows != 0;
+    c->apply_ola = apply_ola != 0;
+    // Windows are N taps: the first W come from the builder, the rest are 1.0
+    // (fft_cffi.cpp:1894-1906).
+    const int useW = window > 0 ? window : c->N;
+    {
+      auto aw = fftfree::build_window(analysis_window_kind, useW, analysis_param1, analysis_param2);
+      fftfree::normalize_window(aw, window_norm_policy);
+      c->analysis_win.assign((size_t)c->N, 1.0f);
+      for (int i = 0; i < useW && i < c->N; ++i) c->analysis_win[i] = aw[i];
+    }
+    {
+      auto sw = fftfree::build_window(synthesis_window_kind, useW, synthesis_param1, synthesis_param2);
+      fftfree::normalize_window(sw, window_norm_policy);
+      c->synthesis_win.assign((size_t)c->N, 1.0f);
+      for (int i = 0; i < useW && i < c->N; ++i) c->synthesis_win[i] = sw[i];
+    }
+    c->cfg.transform = transform;
+    c->cfg.reduce_magnitude = reduce_magnitude != 0;
+    c->cfg.store_polar = store_polar != 0;
+    c->cfg.half_spectrum = half_spectrum != 0;
+    c->cfg.allow_outer_parallel = allow_outer_parallel != 0;
+    c->cfg.allow_inner_parallel = allow_inner_parallel != 0;
+    c->cfg.inner_threads = inner_threads;
+    c->has_pool = allow_outer_parallel != 0;
+    if (fftfree::verbose()) {
+      std::fprintf(stderr, "[diag] fft_init_full: N=%d threads=%d lanes=%d allow_outer=%d allow_inner=%d\n", c->N,
+                   c->cfg.threads, c->cfg.lanes, allow_outer_parallel ? 1 : 0, allow_inner_parallel ? 1 : 0);
+    }
+    fftfree::bind_plan(c);  // throws for a non-power-of-two N, as the Plan ctor does
+    return c;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_init_full", e.what());
+  } catch (...) {
+    fftfree::report("fft_init_full", "unknown exception");
+  }
+  delete c;
+  return nullptr;
+}
+
+int fft_execute(void* handle, const float* in_pcm, float* out_real, float* out_imag, float* out_mag, size_t n) {
+  if (!handle || !in_pcm || !out_real || !out_imag || !out_mag || n == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (c->pad_mode == 2 && (int)n != c->N) return 0;
+  if (n > (size_t)c->N) return 0;
+  try {
+    fftfree::ensure_gpu(c);
+    const size_t N = (size_t)c->N;
+    std::vector<std::complex<float>> col(N);
+    for (size_t i = 0; i < n; ++i) col[i] = std::complex<float>(in_pcm[i], 0.0f);
+    c->d_cplx.reserve(N * sizeof(float2));
+    fftfree::hip_check(hipMemcpyAsync(c->d_cplx.ptr, col.data(), N * sizeof(float2), hipMemcpyHostToDevice, c->stream),
+                       "H2D");
+    fftfree::AxisLayout l;
+    l.base = c->d_cplx.as<float2>();
+    l.axis_size = c->N;
+    l.batch_size = 1;
+    l.batch_stride = c->N;
+    fftfree::fft_apply_axis(*c->plan, l);
+    fftfree::hip_check(hipMemcpyAsync(col.data(), c->d_cplx.ptr, N * sizeof(float2), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
+    // fft_cffi.cpp:690-695: re, im, |v| of the post-transformed value
+    for (size_t j = 0; j < N; ++j) {
+      out_real[j] = col[j].real();
+      out_imag[j] = col[j].imag();
+      out_mag[j] = std::abs(col[j]);
+    }
+    return 1;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_frame_count(void* handle, size_t pcm_len, int pad_mode, size_t max_frames) {
+  if (!handle) return 0;
+  return fftfree::frame_count(static_cast<FftContext*>(handle), pcm_len, pad_mode, max_frames);
+}
+
+size_t fft_execute_batched(void* handle, const float* pcm, size_t pcm_len, float* out_real, float* out_imag,
+                           float* out_mag, int pad_mode, int enable_backup, size_t max_frames) {
+  (void)enable_backup;
+  if (!handle || !pcm || pcm_len == 0 || !out_real || !out_imag || !out_mag) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, max_frames);
+  if (frames == 0) return 0;
+  size_t produced = 0;
+  try {
+    fftfree::ensure_gpu(c);
+    const size_t bins = fftfree::bins_of(c);
+    c->d_pcm.reserve(pcm_len * sizeof(float));
+    fftfree::hip_check(hipMemcpyAsync(c->d_pcm.ptr, pcm, pcm_len * sizeof(float), hipMemcpyHostToDevice, c->stream),
+                       "H2D pcm");
+    const fftfree::StftArgs base = fftfree::stft_args(c, c->d_pcm.as<float>(), pcm_len);
+    // device planes in chunks of <= 768 MiB (three planes)
+    const size_t per_frame = 3 * bins * sizeof(float);
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(frames, (size_t(768) << 20) / per_frame));
+    c->d_planes.reserve(chunk * per_frame);
+    float* d_re = c->d_planes.as<float>();
+    float* d_im = d_re + chunk * bins;
+    float* d_mag = d_im + chunk * bins;
+    for (size_t f0 = 0; f0 < frames; f0 += chunk) {
+      const size_t nf = std::min(chunk, frames - f0);
+      fftfree::run_stft_frames(c, base, f0, nf, d_re, d_im, d_mag, c->stream);
+      const size_t bytes = nf * bins * sizeof(float);
+      fftfree::hip_check(hipMemcpyAsync(out_real + f0 * bins, d_re, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+      fftfree::hip_check(hipMemcpyAsync(out_imag + f0 * bins, d_im, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+      fftfree::hip_check(hipMemcpyAsync(out_mag + f0 * bins, d_mag, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+      fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
+      produced = f0 + nf;
+    }
+    return produced;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute_batched", e.what());
+    return produced;
+  } catch (...) {
+    return produced;
+  }
+}
+
+size_t fft_stft_execute(void* handle, const float* pcm, size_t pcm_len, float* out_real, float* out_imag,
+                        float* out_mag, size_t max_frames) {
+  return fft_execute_batched(handle, pcm, pcm_len, out_real, out_imag, out_mag, 1, 0, max_frames);
+}
+
+size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_len, float* d_out_real,
+                                  float* d_out_imag, float* d_out_mag, int pad_mode, size_t max_frames,
+                                  void* stream) {
+  if (!handle || !d_pcm || pcm_len == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, max_frames);
+  if (frames == 0) return 0;
+  try {
+    fftfree::ensure_gpu(c);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const fftfree::StftArgs base = fftfree::stft_args(c, d_pcm, pcm_len);
+    fftfree::run_stft_frames(c, base, 0, frames, d_out_real, d_out_imag, d_out_mag, s);
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute_batched_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream) {
+  if (!handle || !d_in || !d_out || batch == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  try {
+    fftfree::ensure_gpu(c);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t N = (size_t)c->N;
+    if (d_in != d_out)
+      fftfree::hip_check(hipMemcpyAsync(d_out, d_in, batch * N * sizeof(float2), hipMemcpyDeviceToDevice, s), "D2D");
+    fftfree::AxisLayout l;
+    l.base = reinterpret_cast<float2*>(d_out);
+    l.axis_size = c->N;
+    l.batch_size = (int64_t)batch;
+    l.batch_stride = c->N;
+    c->dispatcher.set_stream(s);
+    fftfree::fft_apply_axis(*c->plan, l);
+    c->dispatcher.set_stream(c->stream);
+    return batch;
+  } catch (const std::exception& e) {
+    c->dispatcher.set_stream(c->stream);
+    fftfree::report("fft_execute_c2c_batched_device", e.what());
+    return 0;
+  } catch (...) {
+    c->dispatcher.set_stream(c->stream);
+    return 0;
+  }
+}
+
+size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t batch) {
+  if (!handle || !in || !out || batch == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  try {
+    fftfree::ensure_gpu(c);
+    const size_t bytes = batch * (size_t)c->N * sizeof(float2);
+    c->d_cplx.reserve(bytes);
+    fftfree::hip_check(hipMemcpyAsync(c->d_cplx.ptr, in, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    const size_t got = fft_execute_c2c_batched_device(handle, c->d_cplx.as<float>(), c->d_cplx.as<float>(), batch,
+                                                      c->stream);
+    if (got != batch) return 0;
+    fftfree::hip_check(hipMemcpyAsync(out, c->d_cplx.ptr, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+    fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return batch;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute_c2c_batched", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_execute_complex_batched(void* handle, const float* in_real, const float* in_imag, size_t frames,
+                                   float* out_pcm, int pad_mode, int enable_backup, size_t max_frames) {
+  (void)pad_mode;
+  (void)enable_backup;
+  if (!handle || !in_real || !in_imag || !out_pcm || frames == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (max_frames != 0 && frames > max_frames) frames = max_frames;
+  try {
+    fftfree::ensure_gpu(c);
+    const size_t N = (size_t)c->N;
+    const size_t bins = fftfree::bins_of(c);
+    // Column rebuild exactly as fft_cffi.cpp:1062-1098 (Hermitian mirror for a C2R half
+    // spectrum, zero fill otherwise), then the plan kernel with its hooks.
+    std::vector<std::complex<float>> buf(frames * N);
+    for (size_t f = 0; f < frames; ++f) {
+      std::complex<float>* col = buf.data() + f * N;
+      for (size_t i = 0; i < bins; ++i) col[i] = std::complex<float>(in_real[f * bins + i], in_imag[f * bins + i]);
+      if (c->half_spectrum && c->transform == FFT_TRANSFORM_C2R) {
+        col[0].imag(0.0f);
+        const size_t ny = N / 2;
+        if (ny < bins) col[ny].imag(0.0f);
+        const size_t maxk = (bins > ny + 1) ? ny : (bins - 1);
+        for (size_t k = 1; k < maxk; ++k) col[N - k] = std::conj(col[k]);
+      }
+    }
+    const size_t bytes = frames * N * sizeof(float2);
+    c->d_cplx.reserve(bytes);
+    fftfree::hip_check(hipMemcpyAsync(c->d_cplx.ptr, buf.data(), bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    fftfree::AxisLayout l;
+    l.base = c->d_cplx.as<float2>();
+    l.axis_size = c->N;
+    l.batch_size = (int64_t)frames;
+    l.batch_stride = c->N;
+    fftfree::fft_apply_axis(*c->plan, l);
+    fftfree::hip_check(hipMemcpyAsync(buf.data(), c->d_cplx.ptr, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+    fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
+    const bool win = c->windows_enabled && !c->synthesis_win.empty();
+    if (c->apply_ola) {
+      // fft_cffi.cpp:1119-1145
+      const int H = c->hop > 0 ? c->hop : (c->window > 0 ? c->window : c->N);
+      const size_t total = frames * N;
+      std::fill(out_pcm, out_pcm + total, 0.0f);
+      std::vector<float> norm;
+      if (c->cola_mode == FFT_COLA_NORMALIZE) norm.assign(total, 0.0f);
+      for (size_t f = 0; f < frames; ++f) {
+        const size_t start = f * (size_t)H;
+        for (size_t i = 0; i < N; ++i) {
+          float s = buf[f * N + i].real();
+          if (win) s *= c->synthesis_win[i];
+          const size_t t = start + i;
+          if (t < total) {
+            out_pcm[t] += s;
+            if (!norm.empty()) {
+              float a = 1.0f, b = 1.0f;
+              if (c->windows_enabled) {
+                if (i < c->analysis_win.size()) a = c->analysis_win[i];
+                if (i < c->synthesis_win.size()) b = c->synthesis_win[i];
+              }
+              norm[t] += a * b;
+            }
+          }
+        }
+      }
+      if (!norm.empty())
+        for (size_t t = 0; t < total; ++t)
+          if (norm[t] > 1e-12f) out_pcm[t] /= norm[t];
+    } else {
+      for (size_t f = 0; f < frames; ++f)
+        for (size_t i = 0; i < N; ++i) {
+          float s = buf[f * N + i].real();
+          if (win) s *= c->synthesis_win[i];
+          out_pcm[f * N + i] = s;
+        }
+    }
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute_complex_batched", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_ctx_size(void* handle) { return handle ? (size_t) static_cast<FftContext*>(handle)->N : 0; }
+
+size_t fft_ctx_worker_threads(void* handle) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  int t = c->cfg.threads > 0 ? c->cfg.threads : 1;
+  if (c->has_pool) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    if (hw > 0) t = std::min(t, hw);
+  }
+  return (size_t)t;
+}
+
+size_t fft_ctx_effective_threads(void* handle) {
+  // eigen_fft.hpp:2332-2342: min(requested, hardware, 16)
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  int lim = c->cfg.threads > 0 ? std::min(c->cfg.threads, 16) : 4;
+  const int hw = (int)std::thread::hardware_concurrency();
+  if (hw > 0) lim = std::min(lim, hw);
+  return (size_t)std::max(1, lim);
+}
+
+void fft_free(void* handle) {
+  if (!handle) return;
+  delete static_cast<FftContext*>(handle);
+}
+
+int fft_config_window(void* handle, int analysis_kind, int synthesis_kind, float param1, float param2,
+                      int norm_policy, int cola_mode) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  const int W = c->window > 0 ? c->window : c->N;
+  if (W <= 0) return 0;
+  try {
+    c->analysis_win_kind = analysis_kind;
+    c->synthesis_win_kind = synthesis_kind;
+    c->win_param1 = param1;
+    c->win_param2 = param2;
+    c->win_norm_policy = norm_policy;
+    c->cola_mode = cola_mode;
+    c->analysis_win = fftfree::build_window(analysis_kind, W, param1, param2);
+    c->synthesis_win = fftfree::build_window(synthesis_kind, W, param1, param2);
+    fftfree::normalize_window(c->analysis_win, norm_policy);
+    fftfree::normalize_window(c->synthesis_win, norm_policy);
+    c->windows_enabled = false;  // fft_cffi.cpp:2005-2006
+    c->windows_dirty = true;
+    return 1;
+  } catch (...) {
+    return 0;
+  }
+}
+
+int fft_config_window_custom(void* handle, const float* analysis_window, size_t analysis_len,
+                             const float* synthesis_window, size_t synthesis_len, int norm_policy, int cola_mode) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  const int W = c->window > 0 ? c->window : c->N;
+  if (W <= 0) return 0;
+  try {
+    if (analysis_window && analysis_len == (size_t)W) c->analysis_win.assign(analysis_window, analysis_window + W);
+    if (synthesis_window && synthesis_len == (size_t)W) c->synthesis_win.assign(synthesis_window, synthesis_window + W);
+    c->win_norm_policy = norm_policy;
+    c->cola_mode = cola_mode;
+    fftfree::normalize_window(c->analysis_win, norm_policy);
+    fftfree::normalize_window(c->synthesis_win, norm_policy);
+    c->windows_enabled = false;
+    c->windows_dirty = true;
+    return 1;
+  } catch (...) {
+    return 0;
+  }
+}
+
+void fft_set_workerpool_debug_dropout(int) {}
+void fft_set_workerpool_dropout_pattern(int, int, int) {}
+void fft_set_workerpool_dropout_single(long long) {}
+void fft_set_workerpool_dropout_persistent(int) {}
+void fft_clear_workerpool_dropout_history(void) {}
+void fft_register_restore_observer(fft_restore_observer_t) {}
+
+size_t fft_plan_metadata(void* handle, int kind, float* out, size_t capacity) {
+  if (!handle || !out) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  try {
+    fftfree::ensure_gpu(c);
+    const bool r2c_half = c->transform == FFT_TRANSFORM_R2C && c->half_spectrum;
+    // capacity bound for the device scratch: N*lgN entries at most
+    size_t need = 0;
+    fftfree::hip_check(fftfree::launch_metadata(c->N, fftfree::use_stockham(c) ? 2 : 0, kind, r2c_half ? 1 : 0,
+                                                c->plan->W, c->inverse ? 1 : 0, nullptr, 0, &need, c->stream),
+                       "metadata size");
+    if (need == 0 || need > capacity) return 0;
+    c->d_cplx.reserve(need * sizeof(float2));
+    size_t got = 0;
+    fftfree::hip_check(fftfree::launch_metadata(c->N, fftfree::use_stockham(c) ? 2 : 0, kind, r2c_half ? 1 : 0,
+                                                c->plan->W, c->inverse ? 1 : 0, c->d_cplx.as<float2>(), need, &got,
+                                                c->stream),
+                       "launch_metadata");
+    fftfree::hip_check(hipMemcpyAsync(out, c->d_cplx.ptr, got * sizeof(float2), hipMemcpyDeviceToHost, c->stream),
+                       "D2H");
+    fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return got;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_plan_metadata", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+int fft_set_variant(void* handle, int variant) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (variant < 0) {
+    c->plan->stft_variant = -1;
+    c->plan->c2c_variant = -1;
+    return 1;
+  }
+  if (fftfree::fused_ok(c)) {
+    if (variant >= fftfree::stft_variant_count(c->N)) return 0;
+    c->plan->stft_variant = variant;
+  } else {
+    if (variant >= fftfree::c2c_variant_count(c->N)) return 0;
+    c->plan->c2c_variant = variant;
+  }
+  return 1;
+}
+
+int fft_variant_count(void* handle) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  return fftfree::fused_ok(c) ? fftfree::stft_variant_count(c->N) : fftfree::c2c_variant_count(c->N);
+}
+
+int fft_variant_name(void* handle, int variant, char* buf, size_t buflen) {
+  if (!handle || !buf || buflen == 0) return 0;
+  return fftfree::variant_desc(static_cast<FftContext*>(handle), variant, buf, buflen) ? 1 : 0;
+}
+
+}  // extern "C"

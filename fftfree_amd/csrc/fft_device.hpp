@@ -1,0 +1,254 @@
+// fft_device.hpp -- gfx950 device building blocks of the batched FFT/STFT engine.
+//
+// Complex values are float2 (x = real, y = imag). A transform of complex length L is
+// held by T = L/E threads, each owning E values in registers. Every pass is a Stockham
+// autosort radix-R step (natural-order output, no bit reversal): butterfly b reads
+// src[b + r*L/R], multiplies by exp(DIR*2*pi*i*(b mod Ns)*r/(Ns*R)), runs an R-point DFT
+// in registers and writes dst[(b - b mod Ns)*R + (b mod Ns) + r*Ns]. Data moves between
+// passes through LDS ping-pong buffers, one barrier per pass. DIR = -1 is the forward
+// transform, +1 the inverse (unscaled here; callers apply 1/N).
+//
+// The reference computes the same DFT with lgN radix-2 stages (eigen_fft.hpp:2902-3057
+// for Cooley-Tukey, 3472-3532 for Stockham); radix-8/16 passes change only the order of
+// float operations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fftfree {
+namespace dev {
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
+// a * b with fused multiply-adds: (ax*bx - ay*by, ax*by + ay*bx)
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(__builtin_fmaf(a.x, b.x, -a.y * b.y), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
+// a * (DIR * i)
+template <int DIR>
+__device__ __forceinline__ float2 c_mul_i(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// cos(2*pi*k/16) and sin(2*pi*k/16), k = 0..15, correctly rounded to float.
+__device__ constexpr float kCos16[16] = {
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+__device__ constexpr float kSin16[16] = {
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f};
+
+// x * exp(DIR * 2*pi*i * K / R) for a compile-time K, R <= 16 (R a power of two).
+template <int R, int K, int DIR>
+__device__ __forceinline__ float2 c_mul_root(float2 x) {
+  constexpr int k16 = (K * (16 / R)) & 15;
+  if constexpr (k16 == 0) {
+    return x;
+  } else if constexpr (k16 == 8) {
+    return make_float2(-x.x, -x.y);
+  } else if constexpr (k16 == 4) {
+    return c_mul_i<DIR>(x);
+  } else if constexpr (k16 == 12) {
+    return c_mul_i<-DIR>(x);
+  } else {
+    constexpr float c = kCos16[k16];
+    constexpr float s = (DIR < 0 ? -1.0f : 1.0f) * kSin16[k16];
+    return make_float2(__builtin_fmaf(x.x, c, -x.y * s), __builtin_fmaf(x.x, s, x.y * c));
+  }
+}
+
+// In-register R-point DFT, natural order in and out (recursive radix-2 split on
+// compile-time indices so every register index and twiddle constant folds).
+template <int R, int DIR, int STRIDE = 1, int OFF = 0>
+struct RegDFT;
+
+template <int DIR, int STRIDE, int OFF>
+struct RegDFT<1, DIR, STRIDE, OFF> {
+  __device__ static __forceinline__ void run(const float2* in, float2* out) { out[0] = in[OFF]; }
+};
+
+template <int R, int DIR, int STRIDE, int OFF>
+struct RegDFT {
+  template <int K>
+  __device__ static __forceinline__ void combine(const float2* e, const float2* o, float2* out) {
+    if constexpr (K < R / 2) {
+      const float2 t = c_mul_root<R, K, DIR>(o[K]);
+      out[K] = c_add(e[K], t);
+      out[K + R / 2] = c_sub(e[K], t);
+      combine<K + 1>(e, o, out);
+    }
+  }
+  __device__ static __forceinline__ void run(const float2* in, float2* out) {
+    float2 e[R / 2], o[R / 2];
+    RegDFT<R / 2, DIR, STRIDE * 2, OFF>::run(in, e);
+    RegDFT<R / 2, DIR, STRIDE * 2, OFF + STRIDE>::run(in, o);
+    combine<0>(e, o, out);
+  }
+};
+
+template <int R, int DIR>
+__device__ __forceinline__ void reg_dft(float2* v) {
+  float2 t[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) t[i] = v[i];
+  RegDFT<R, DIR>::run(t, v);
+}
+
+// Twiddle exp(DIR*2*pi*i*m/Lt) from a forward table tw[m'] = exp(-2*pi*i*m'/N) where
+// Lt divides N: m' = m * (N/Lt) = m * stride. The inverse direction conjugates.
+template <int DIR>
+__device__ __forceinline__ float2 load_tw(const float2* __restrict__ tw, int idx) {
+  const float2 w = tw[idx];
+  return DIR < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// Radix-list helpers.
+template <int R0, int... Rs>
+struct FirstOf {
+  static constexpr int value = R0;
+};
+
+template <int R0, int... Rs>
+struct LastOf {
+  static constexpr int value = LastOf<Rs...>::value;
+};
+template <int R0>
+struct LastOf<R0> {
+  static constexpr int value = R0;
+};
+
+template <int... Rs>
+constexpr int product() {
+  return (1 * ... * Rs);
+}
+
+// Twiddle + R-point DFT of the E/R butterflies this thread owns in one Stockham radix-R
+// pass over a transform of length L held by T = L/E threads. `v` holds the inputs in
+// butterfly order: v[q*R + r] = src[b_q + r*L/R] with b_q = t + q*T. `tw` is a forward
+// table of exp(-2*pi*i*m/(L*tw_stride)).
+template <int L, int E, int R, int DIR>
+__device__ __forceinline__ void pass_compute(float2* v, int t, int Ns,
+                                             const float2* __restrict__ tw, int tw_stride) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int q = 0; q < E / R; ++q) {
+    const int b = t + q * T;
+    const int k = b & (Ns - 1);
+    float2* u = v + q * R;
+    if (Ns > 1) {
+      // element r turns by k*r*(L/(Ns*R)) units of 2*pi/L
+      const int base = k * (L / R / Ns) * tw_stride;
+#pragma unroll
+      for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], load_tw<DIR>(tw, base * r));
+    }
+    reg_dft<R, DIR>(u);
+  }
+}
+
+// Autosort scatter of a computed pass: dst[(b - b mod Ns)*R + b mod Ns + r*Ns].
+template <int L, int E, int R>
+__device__ __forceinline__ void pass_store(const float2* v, int t, int Ns, float2* dst) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int q = 0; q < E / R; ++q) {
+    const int b = t + q * T;
+    const int k = b & (Ns - 1);
+    const int o = (b - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[o + r * Ns] = v[q * R + r];
+  }
+}
+
+template <int L, int E, int R>
+__device__ __forceinline__ void pass_load(float2* v, int t, const float2* src) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int q = 0; q < E / R; ++q) {
+    const int b = t + q * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q * R + r] = src[b + r * (L / R)];
+  }
+}
+
+// Runs the passes Rs on registers `v` (the first pass's inputs are already loaded in
+// butterfly order). Between passes data goes through the LDS ping-pong pair (a, b), one
+// barrier each. On return v holds the result in last-pass order:
+//   v[q*Rl + r] = X[b_q + r*L/Rl],  b_q = t + q*T,  Rl = last radix,
+// i.e. consecutive threads own consecutive outputs. `free_buf` names the LDS buffer
+// that nobody reads any more (safe to write before the next barrier).
+template <int L, int E, int DIR, int Ns, int R, int... Rest>
+__device__ __forceinline__ void run_passes(float2* v, int t, float2* a, float2* b,
+                                           const float2* __restrict__ tw, int tw_stride,
+                                           float2*& free_buf) {
+  pass_compute<L, E, R, DIR>(v, t, Ns, tw, tw_stride);
+  if constexpr (sizeof...(Rest) == 0) {
+    free_buf = a;
+  } else {
+    pass_store<L, E, R>(v, t, Ns, a);
+    __syncthreads();
+    constexpr int Rn = FirstOf<Rest...>::value;
+    pass_load<L, E, Rn>(v, t, a);
+    run_passes<L, E, DIR, Ns * R, Rest...>(v, t, b, a, tw, tw_stride, free_buf);
+  }
+}
+
+// Stores last-pass-ordered registers to LDS in natural order (buffer of length L).
+template <int L, int E, int Rl>
+__device__ __forceinline__ void store_natural(const float2* v, int t, float2* dst) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q) {
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) dst[t + q * T + r * (L / Rl)] = v[q * Rl + r];
+  }
+}
+
+// XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so remap the
+// linear workgroup id so that each XCD walks a contiguous run of tiles (neighbouring
+// frames share input samples and land in the same L2). The map is a bijection on
+// [0, G) for any G, so correctness does not depend on the dispatch order.
+__device__ __forceinline__ int xcd_tile(int i, int G) {
+  const int q = G >> 3, rem = G & 7;
+  const int x = i & 7, s = i >> 3;
+  return x * q + (x < rem ? x : rem) + s;
+}
+
+// Output modes of the post-transform hook (eigen_fft.hpp:2535-2596) + output copy
+// (fft_cffi.cpp:811-825): which value lands in each of the three planes.
+enum OutMode : int {
+  kOutComplex = 0,    // (re, im, |z|)
+  kOutC2RClear = 1,   // (re, 0, |re|)
+  kOutMagnitude = 2,  // (|z|, 0, |z|)
+  kOutPolar = 3,      // (|z|, arg z, |z|)
+};
+
+__device__ __forceinline__ void store_planes(float2 z, int mode, size_t idx, float* __restrict__ o_re,
+                                             float* __restrict__ o_im, float* __restrict__ o_mag) {
+  float a, b, c;
+  if (mode == kOutComplex) {
+    a = z.x;
+    b = z.y;
+    c = __builtin_sqrtf(__builtin_fmaf(z.x, z.x, z.y * z.y));
+  } else if (mode == kOutC2RClear) {
+    a = z.x;
+    b = 0.0f;
+    c = __builtin_fabsf(z.x);
+  } else {
+    a = __builtin_sqrtf(__builtin_fmaf(z.x, z.x, z.y * z.y));
+    b = (mode == kOutPolar) ? atan2f(z.y, z.x) : 0.0f;
+    c = a;
+  }
+  if (o_re) o_re[idx] = a;
+  if (o_im) o_im[idx] = b;
+  if (o_mag) o_mag[idx] = c;
+}
+
+}  // namespace dev
+}  // namespace fftfree

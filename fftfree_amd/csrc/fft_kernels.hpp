@@ -1,0 +1,88 @@
+// fft_kernels.hpp -- host-side view of the HIP kernels: argument blocks and launchers.
+// Everything here takes plain device pointers and a hipStream_t; no torch types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fftfree {
+
+// Real-input batched STFT (fft_execute_batched semantics, fft_cffi.cpp:701-825):
+// frame f covers pcm[f*hop, f*hop + window), times the analysis window, zero-padded to
+// N; output planes are frame-major with `bins` floats per frame.
+struct StftArgs {
+  const float* pcm = nullptr;
+  size_t pcm_len = 0;
+  const float* win = nullptr;  // `window` taps, or nullptr when windows are off
+  int window = 0;              // W <= N
+  int hop = 0;
+  size_t frames = 0;
+  const float2* tw = nullptr;  // forward table exp(-2 pi i m / N), m < N
+  int n = 0;                   // N (power of two)
+  int bins = 0;                // N/2+1 or N
+  int out_mode = 0;            // dev::OutMode
+  float scale = 1.0f;          // 1 or 1/N
+  float* o_re = nullptr;       // any plane may be nullptr (device entry only)
+  float* o_im = nullptr;
+  float* o_mag = nullptr;
+  size_t frame_offset = 0;     // first frame index (ragged / chunked callers)
+};
+
+// Complex batched transform (fft_inplace_batched semantics, eigen_fft.hpp:3152-3175):
+// `batch` transforms of N complex values, transform-major, pre hook on load and post
+// hook on store (eigen_fft.hpp:2484-2596). in may equal out.
+struct C2CArgs {
+  const float2* in = nullptr;
+  float2* out = nullptr;
+  size_t batch = 0;
+  const float2* tw = nullptr;  // forward table exp(-2 pi i m / N), m < N
+  int n = 0;
+  int pre_mode = 0;   // 0 none, 1 clear imag (R2C), 2 Hermitian rebuild (C2R half)
+  int pre_polar = 0;  // with pre_mode 2: bins hold (mag, phase)
+  int post_half = 0;  // R2C half: DC/Nyquist imag = 0, bins > N/2 zero
+  int post_mode = 0;  // 0 none, 1 C2R imag clear, 2 magnitude, 3 polar
+  int post_upper = 0; // bins the magnitude/polar reduction covers
+  float scale = 1.0f;
+};
+
+// Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
+enum MetaKind : int {
+  kMetaTwiddle = 0,
+  kMetaLayoutMap = 1,
+  kMetaStageParams = 3,
+  kMetaButterflyPairs = 4,
+  kMetaTwiddleIndexMap = 6,
+  kMetaOmega = 7,
+};
+
+// Launchers; return hipSuccess or the launch error. `variant` < 0 picks the default
+// configuration for N; otherwise an index into the variant table (autotuner).
+hipError_t launch_stft(const StftArgs& a, int inverse, int variant, hipStream_t s);
+hipError_t launch_c2c(const C2CArgs& a, int inverse, int variant, hipStream_t s);
+// Generic real-input path: frames as complex columns (N per frame), and the output
+// copy of transformed columns into planes (mag plane = re when polar).
+hipError_t launch_frame_complex(const StftArgs& a, float2* out, hipStream_t s);
+hipError_t launch_planes(const float2* in, size_t frames, int n, int bins, int polar, float* o_re, float* o_im,
+                         float* o_mag, hipStream_t s);
+// Index metadata of one plan (kernel: 2 Stockham, else Cooley-Tukey), written as
+// complex<float> pairs; returns the entry count in *count (0 if not recorded).
+hipError_t launch_metadata(int n, int kernel, int kind, int r2c_half, const float2* tw, int inverse,
+                           float2* out, size_t capacity, size_t* count, hipStream_t s);
+
+// Variant tables for the autotuner.
+struct VariantInfo {
+  int l;            // complex length the kernel transforms
+  int e;            // values per thread
+  int fpw;          // transforms per workgroup
+  const char* radices;
+};
+int stft_variant_count(int n);
+const VariantInfo* stft_variant(int n, int idx);
+int c2c_variant_count(int n);
+const VariantInfo* c2c_variant(int n, int idx);
+// Largest N the single-launch LDS kernels cover.
+int stft_max_lds_n();
+int c2c_max_lds_n();
+
+}  // namespace fftfree

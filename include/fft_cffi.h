@@ -1,0 +1,158 @@
+/* fft_cffi.h -- C ABI of the MI355X (gfx950) batched FFT/STFT library, libfft_cffi.so.
+ *
+ * Drop-in for the reference library's C API (fft_cffi.hpp): the entry points marked
+ * "reference" keep the reference's names, signatures, argument meaning and error
+ * behaviour; each cites the declaration it replaces. The entry points marked "additive"
+ * have no reference counterpart; they take device pointers (for roofline timing and
+ * torch callers) or expose data the reference only offers through its C++ API.
+ *
+ * Pointers in reference entry points are HOST pointers, exactly as in the reference, and
+ * those calls return when the results are in the caller's buffers. "_device" entry points
+ * take device pointers and a hipStream_t passed as void* (NULL = the default stream); they
+ * only enqueue work on that stream and return without waiting for it.
+ *
+ * All functions are safe to call with a NULL handle (they return 0 / do nothing).
+ * A context is not re-entrant: concurrent calls on one context race, separate
+ * contexts are independent (reference: fft_cffi.hpp:99-176, eigen_fft.hpp:1418).
+ */
+#ifndef FFTFREE_AMD_FFT_CFFI_H
+#define FFTFREE_AMD_FFT_CFFI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kernel, radix and transform constants (fft_cffi.hpp:62-81; values follow the
+ * eigfft::KernelKind / ButterflyRadix enum order, eigen_fft.hpp:1892,
+ * butterfly_kernel.hpp:16-21). At the C level the kernel argument is read as
+ * 2 = Stockham and anything else = Cooley-Tukey (fft_cffi.cpp:799-800). */
+enum { FFT_KERNEL_COOLEYTUKEY = 0, FFT_KERNEL_STOCKHAM = 1, FFT_KERNEL_EXTERNAL = 2 };
+enum { FFT_BUTTERFLY_RADIX_2 = 0, FFT_BUTTERFLY_RADIX_4 = 1, FFT_BUTTERFLY_RADIX_8 = 2, FFT_BUTTERFLY_RADIX_16 = 3 };
+enum { FFT_TRANSFORM_C2C = 0, FFT_TRANSFORM_R2C = 1, FFT_TRANSFORM_C2R = 2, FFT_TRANSFORM_R2R = 3 };
+/* Window kinds, normalisation and COLA modes (fft_cffi.hpp:292-313). */
+enum { FFT_WINDOW_RECT = 0, FFT_WINDOW_HANN = 1, FFT_WINDOW_HAMMING = 2, FFT_WINDOW_BLACKMAN = 3,
+       FFT_WINDOW_TUKEY = 4, FFT_WINDOW_KAISER = 5 };
+enum { FFT_WINDOW_NORM_NONE = 0, FFT_WINDOW_NORM_L2 = 1, FFT_WINDOW_NORM_AREA = 2 };
+enum { FFT_COLA_OFF = 0, FFT_COLA_NORMALIZE = 1 };
+enum { FFT_STREAM_FLUSH_NONE = 0, FFT_STREAM_FLUSH_PARTIAL = 1, FFT_STREAM_FLUSH_FINAL = 2 };
+/* Metadata kinds of fft_plan_metadata (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491). */
+enum { FFT_META_TWIDDLE = 0, FFT_META_LAYOUT_MAP = 1, FFT_META_STAGE_PARAMS = 3, FFT_META_BUTTERFLY_PAIRS = 4,
+       FFT_META_TWIDDLE_INDEX_MAP = 6, FFT_META_OMEGA = 7 };
+
+/* ---- reference entry points -------------------------------------------------------- */
+
+/* reference: fft_cffi.hpp:31 (fft_cffi.cpp:431-514). One forward FFT of n samples padded
+ * to the next power of two; writes the first n bins. */
+void fft_pcm_to_channels(const float* in_pcm, float* out_real, float* out_imag, float* out_mag, size_t n,
+                         int threads);
+
+/* reference: fft_cffi.hpp:99 (fft_cffi.cpp:535-538). */
+void* fft_init(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+               size_t radix_pattern_len, int pad_mode);
+
+/* reference: fft_cffi.hpp:105 (fft_cffi.cpp:540-658). Pads N to a power of two only when
+ * pad_mode == 1; honours FFTFREE_DEFAULT_KERNEL when kernel == 0. */
+void* fft_init_ex(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+                  size_t radix_pattern_len, int pad_mode, int window, int hop);
+
+/* reference: fft_cffi.hpp:108-127 (fft_cffi.cpp:1792-1822). Never windows. */
+void* fft_init_full(size_t n, int threads, int lanes, int inverse, int kernel, int radix, const int* radix_pattern,
+                    size_t radix_pattern_len, int pad_mode, int window, int hop, int transform,
+                    int reduce_magnitude, int store_polar, int half_spectrum, int allow_outer_parallel,
+                    int allow_inner_parallel, int inner_threads, int save_crash_logs, int silent_crash_reports);
+
+/* reference: fft_cffi.hpp:130-150 (fft_cffi.cpp:1824-1978). NULL unless n is a power of
+ * two. Crash-handler flags are accepted and ignored. */
+void* fft_init_full_v2(size_t n, int threads, int lanes, int inverse, int kernel, int radix,
+                       const int* radix_pattern, size_t radix_pattern_len, int pad_mode, int window, int hop,
+                       int transform, int reduce_magnitude, int store_polar, int half_spectrum,
+                       int allow_outer_parallel, int allow_inner_parallel, int inner_threads, int save_crash_logs,
+                       int silent_crash_reports, int apply_windows, int apply_ola, int analysis_window_kind,
+                       float analysis_param1, float analysis_param2, int synthesis_window_kind,
+                       float synthesis_param1, float synthesis_param2, int window_norm_policy, int cola_mode);
+
+/* reference: fft_cffi.hpp:153-158 (fft_cffi.cpp:660-699). One frame of n <= N samples;
+ * returns 1 on success, 0 on error. */
+int fft_execute(void* handle, const float* in_pcm, float* out_real, float* out_imag, float* out_mag, size_t n);
+
+/* reference: fft_cffi.hpp:168-176 (fft_cffi.cpp:701-1034). Frames pcm with the context's
+ * window/hop, pads per pad_mode (0 = context setting), writes max_frames*bins floats per
+ * plane at most; returns frames produced, 0 on error. enable_backup is accepted and has
+ * no effect on results (the reference's recovery path computes the same values). */
+size_t fft_execute_batched(void* handle, const float* pcm, size_t pcm_len, float* out_real, float* out_imag,
+                           float* out_mag, int pad_mode, int enable_backup, size_t max_frames);
+
+/* reference: fft_cffi.hpp:181-187 (fft_cffi.cpp:1458-1466) = batched with pad_mode 1. */
+size_t fft_stft_execute(void* handle, const float* pcm, size_t pcm_len, float* out_real, float* out_imag,
+                        float* out_mag, size_t max_frames);
+
+/* reference: fft_cffi.hpp:228-235 (fft_cffi.cpp:1042-1340). Complex frame-major bins in,
+ * N real samples per frame out (times the synthesis window when windows are enabled),
+ * optional internal overlap-add when the context was created with apply_ola. */
+size_t fft_execute_complex_batched(void* handle, const float* in_real, const float* in_imag, size_t frames,
+                                   float* out_pcm, int pad_mode, int enable_backup, size_t max_frames);
+
+/* reference: fft_cffi.hpp:237-241 (fft_cffi.cpp:1744-1773). */
+size_t fft_ctx_size(void* handle);
+size_t fft_ctx_worker_threads(void* handle);
+size_t fft_ctx_effective_threads(void* handle);
+
+/* reference: fft_cffi.hpp:280 (fft_cffi.cpp:1775-1779). fft_free(NULL) is a no-op. */
+void fft_free(void* handle);
+
+/* reference: fft_cffi.hpp:329-345 (fft_cffi.cpp:1983-2032). Rebuild (or replace) the
+ * analysis/synthesis windows; like the reference this leaves internal windowing off. */
+int fft_config_window(void* handle, int analysis_kind, int synthesis_kind, float param1, float param2,
+                      int norm_policy, int cola_mode);
+int fft_config_window_custom(void* handle, const float* analysis_window, size_t analysis_len,
+                             const float* synthesis_window, size_t synthesis_len, int norm_policy, int cola_mode);
+
+/* reference: fft_cffi.hpp:246-266, 286-287. The reference's CPU worker-pool fault
+ * injection hooks; there is no worker pool here, so these record nothing and the
+ * observer is never called. Present so existing callers link. */
+void fft_set_workerpool_debug_dropout(int pct);
+void fft_set_workerpool_dropout_pattern(int period, int on_len, int phase);
+void fft_set_workerpool_dropout_single(long long index);
+void fft_set_workerpool_dropout_persistent(int on);
+void fft_clear_workerpool_dropout_history(void);
+typedef void (*fft_restore_observer_t)(size_t start, size_t end);
+void fft_register_restore_observer(fft_restore_observer_t cb);
+
+/* ---- additive entry points --------------------------------------------------------- */
+
+/* additive. Frames fft_execute_batched would produce for pcm_len samples (the
+ * reference's rule, fft_cffi.cpp:715-750); 0 = refused. Host-only, no GPU work. */
+size_t fft_frame_count(void* handle, size_t pcm_len, int pad_mode, size_t max_frames);
+
+/* additive. fft_execute_batched on device-resident buffers: d_pcm holds pcm_len floats,
+ * the planes hold frames*bins floats each; any plane may be NULL to skip writing it. */
+size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_len, float* d_out_real,
+                                  float* d_out_imag, float* d_out_mag, int pad_mode, size_t max_frames,
+                                  void* stream);
+
+/* additive. Batched complex transform of `batch` columns of N interleaved complex64
+ * values (the C++ fft_inplace_batched path, eigen_fft.hpp:3152-3175, including the
+ * plan's pre/post hooks). in may equal out. Host pointers / device pointers. */
+size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t batch);
+size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream);
+
+/* additive. Index metadata of the context's plan (MetadataRequest kinds of the C++ API,
+ * eigen_fft.hpp:1496-1501), written by a GPU index kernel as complex<float> pairs into
+ * the host buffer `out` (capacity in complex entries). Returns entries written; 0 if the
+ * kind is not recorded by the plan's kernel or capacity is too small. */
+size_t fft_plan_metadata(void* handle, int kind, float* out, size_t capacity);
+
+/* additive. Selects the kernel variant (radix pattern / frames per workgroup) for the
+ * context's size; -1 restores the default. Returns 1 if the variant exists. The count
+ * and a description ("8x8x8 fpw=4") are queryable for autotuners. */
+int fft_set_variant(void* handle, int variant);
+int fft_variant_count(void* handle);
+int fft_variant_name(void* handle, int variant, char* buf, size_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFTFREE_AMD_FFT_CFFI_H */

@@ -1,0 +1,34 @@
+"""Build recipe of the oracle's C++ restatement (libcpu_ref.so). TEST INFRASTRUCTURE ONLY.
+
+The reference's own compile flags are -O3 -ffast-math -fno-math-errno -march=native
+(CMakeLists.txt:16, 225, 237-239). -march=native is replaced by x86-64-v3 (AVX2/FMA)
+because this library is built once here and then run on other hosts' CPUs.
+
+The reference library itself is not built: it needs Eigen 3.4.0 fetched by CMake
+FetchContent (CMakeLists.txt:105-111), which is not available offline, so oracle/_ref
+stays empty.
+"""
+from __future__ import annotations
+
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+SRC = HERE / "cpu_ref.cpp"
+OUT = HERE / "libcpu_ref.so"
+FLAGS = ["-O3", "-ffast-math", "-fno-math-errno", "-march=x86-64-v3", "-std=c++17", "-fPIC", "-shared", "-pthread"]
+
+
+def build(verbose: bool = False) -> Path:
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = ["g++", *FLAGS, str(SRC), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    tmp.replace(OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

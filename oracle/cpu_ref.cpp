@@ -1,0 +1,506 @@
+// cpu_ref.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// A CPU restatement of the reference library's batched FFT/STFT path
+// (sangderenard/fftfree), used for two things only:
+//   * as the parity checker of the HIP product path (tests, smoke());
+//   * as the CPU baseline that bench.py times beside the GPU.
+// Nothing in the product (fftfree_amd/, include/) links, loads or calls this file.
+//
+// The reference itself cannot be compiled here: it needs Eigen 3.4.0 through a CMake
+// FetchContent download (CMakeLists.txt:105-111) and no Eigen exists offline. This file
+// restates, in plain C++ over std::complex<float>, what these reference regions compute:
+//   twiddles + bit reversal           eigen_fft.hpp:2234-2253
+//   pre/post transform hooks          eigen_fft.hpp:2484-2596
+//   Cooley-Tukey permute + stages     eigen_fft.hpp:2777-2880, 2902-3085
+//   Stockham autosort stages          eigen_fft.hpp:3439-3593
+//   radix-2 DIT scatter/in-place      butterfly_kernel.hpp:77-98, 128-233
+//   window builder + normalisation    fft_cffi.cpp:163-208
+//   frame count + framing + output    fft_cffi.cpp:701-825
+//   complex-input batched inverse     fft_cffi.cpp:1042-1157
+//   metadata writers (index maps)     eigen_fft.hpp:1529-1888 as called at 2672-2682,
+//                                     2981-3011, 3239-3244, 3478-3505
+// The reference's WorkerPool (eigen_fft.hpp:818-1419) is restated as a plain
+// std::thread fan-out over contiguous frame ranges; every frame is transformed
+// independently, exactly as the reference's lanes=1 column chunks are.
+//
+// Float arithmetic follows the reference's order of operations; bit-level equality with
+// the reference binary is not claimed (its butterflies go through Eigen packet math
+// compiled with -ffast-math, which no reference test pins bit-for-bit).
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using cf = std::complex<float>;
+
+struct RefPlan {
+  int N = 0;
+  int lgN = 0;
+  bool inverse = false;
+  std::vector<cf> W;      // N/2 twiddles, float angle (eigen_fft.hpp:2234-2242)
+  std::vector<int> bitrev;  // N entries (eigen_fft.hpp:2244-2253)
+};
+
+bool make_plan(int N, bool inverse, RefPlan& P) {
+  if (N <= 0) return false;
+  int lg = 0, t = N;
+  while ((t & 1) == 0) { ++lg; t >>= 1; }
+  if ((1 << lg) != N) return false;  // eigen_fft.hpp:2228-2232: power of two only
+  P.N = N;
+  P.lgN = lg;
+  P.inverse = inverse;
+  // eigen_fft.hpp:2234-2242 -- the angle is formed in float, then cos/sin in float.
+  const float sgn = inverse ? 1.0f : -1.0f;
+  const float tau = sgn * float(2 * std::acos(float(-1))) / float(N);
+  P.W.assign(std::max(1, N / 2), cf(1.0f, 0.0f));
+  for (int k = 0; k < N / 2; ++k) {
+    const float ang = tau * float(k);
+    P.W[k] = cf(float(std::cos(ang)), float(std::sin(ang)));
+  }
+  P.bitrev.assign(N, 0);
+  for (int i = 0; i < N; ++i) {  // eigen_fft.hpp:2244-2253
+    unsigned x = unsigned(i), r = 0;
+    for (int b = 0; b < lg; ++b) { r = (r << 1) | (x & 1u); x >>= 1; }
+    P.bitrev[i] = int(r);
+  }
+  return true;
+}
+
+struct Flags {
+  int transform = 0;  // 0 C2C, 1 R2C, 2 C2R, 3 R2R
+  bool reduce_magnitude = false;
+  bool store_polar = false;
+  bool half_spectrum = false;
+};
+
+// eigen_fft.hpp:2484-2533 (Cooley-Tukey only; the Stockham path never calls it).
+void pre_transform(const RefPlan& P, const Flags& f, cf* col) {
+  const int N = P.N;
+  if (f.transform == 1) {
+    for (int i = 0; i < N; ++i) if (col[i].imag() != 0.0f) col[i].imag(0.0f);
+    return;
+  }
+  if (f.transform == 2 && f.half_spectrum) {
+    if (f.store_polar) {
+      for (int k = 0; k <= N / 2; ++k) {
+        const float mag = col[k].real(), ph = col[k].imag();
+        col[k] = cf(mag * std::cos(ph), mag * std::sin(ph));
+      }
+    }
+    col[0].imag(0.0f);
+    col[N / 2].imag(0.0f);
+    for (int k = 1; k < N / 2; ++k) col[N - k] = std::conj(col[k]);
+  }
+}
+
+// eigen_fft.hpp:2535-2596.
+void post_transform(const RefPlan& P, const Flags& f, cf* col) {
+  const int N = P.N;
+  const bool r2c_half = (f.transform == 1 && f.half_spectrum);
+  if (r2c_half) {
+    col[0].imag(0.0f);
+    col[N / 2].imag(0.0f);
+    for (int k = N / 2 + 1; k < N; ++k) col[k] = cf(0.0f, 0.0f);
+  }
+  const int upper = r2c_half ? (N / 2 + 1) : N;
+  if (f.store_polar) {
+    for (int i = 0; i < upper; ++i) {
+      const float mag = std::abs(col[i]);
+      const float ph = std::arg(col[i]);
+      col[i] = cf(mag, ph);
+    }
+    return;
+  }
+  if (f.reduce_magnitude) {
+    for (int i = 0; i < upper; ++i) col[i] = cf(std::abs(col[i]), 0.0f);
+    return;
+  }
+  if (f.transform == 2) {
+    for (int i = 0; i < N; ++i) if (col[i].imag() != 0.0f) col[i].imag(0.0f);
+  }
+}
+
+// Index-map capture for one column (frame 0), matching the reference metadata kinds.
+// Buffers hold complex<float> entries like the reference's MetadataRequest buffers.
+struct Capture {
+  cf* layout_map = nullptr;     // N
+  cf* twiddle_index = nullptr;  // N * lgN
+  cf* pairs = nullptr;          // N * lgN
+  cf* stage_params = nullptr;   // lgN
+  cf* twiddle = nullptr;        // 2^lgN - 1
+  cf* omega = nullptr;          // N (Cooley-Tukey only)
+};
+
+// Cooley-Tukey: bit-reverse swap then lgN in-place radix-2 DIT stages
+// (eigen_fft.hpp:2805-2865 permute, 2902-3016 stages, 3076-3085 scaling).
+void ct_column(const RefPlan& P, const Flags& f, cf* col, const Capture* cap) {
+  const int N = P.N;
+  pre_transform(P, f, col);
+  for (int i = 0; i < N; ++i) {
+    const int src = P.bitrev[i];
+    if (src <= i) continue;
+    std::swap(col[i], col[src]);
+  }
+  for (int len = 2, s = 0; len <= N; len <<= 1, ++s) {
+    const int half = len >> 1;
+    const int step = N / len;
+    const int blocks = N / len;
+    if (cap && cap->stage_params) cap->stage_params[s] = cf(float(len), float(step));
+    for (int k = 0; k < half; ++k) {
+      const cf w = P.W[k * step];
+      for (int block = 0; block < blocks; ++block) {
+        const int a = block * len + k;
+        const int b = a + half;
+        const cf ai = col[a];
+        const cf bt = col[b] * w;  // butterfly_kernel.hpp:93-97 (DIT)
+        col[a] = ai + bt;
+        col[b] = ai - bt;
+        if (cap && cap->twiddle_index) {
+          cap->twiddle_index[size_t(s) * N + a] = cf(float(k * step), 0.0f);
+          cap->twiddle_index[size_t(s) * N + b] = cf(float(k * step), 0.0f);
+        }
+        if (cap && cap->pairs) {
+          cap->pairs[size_t(s) * N + a] = cf(float(a), float(b));
+          cap->pairs[size_t(s) * N + b] = cf(float(a), float(b));
+        }
+      }
+    }
+  }
+  if (P.inverse) {
+    const float scale = 1.0f / float(N);
+    for (int i = 0; i < N; ++i) col[i] *= scale;
+  }
+  post_transform(P, f, col);
+}
+
+// Stockham autosort: ping-pong radix-2 DIT scatter stages with natural-order output
+// (eigen_fft.hpp:3439-3593); no pre-transform hook, post hook after the batch (3617).
+void stockham_column(const RefPlan& P, const Flags& f, cf* col, std::vector<cf>& ping,
+                     std::vector<cf>& pong, const Capture* cap) {
+  const int N = P.N;
+  ping.assign(col, col + N);
+  pong.resize(N);
+  cf* in = ping.data();
+  cf* out = pong.data();
+  const int halfN = N >> 1;
+  for (int s = 0; s < P.lgN; ++s) {
+    const int m = 1 << s;
+    const int seg = N >> (s + 1);
+    if (cap && cap->stage_params) cap->stage_params[s] = cf(float(m << 1), float(seg));
+    for (int j = 0; j < m; ++j) {
+      const int tw = j * seg;
+      const cf w = P.W[tw];
+      for (int g = 0; g < seg; ++g) {
+        const int i0 = 2 * j * seg + g;
+        const int i1 = i0 + seg;
+        const int o0 = j * seg + g;
+        const int o1 = o0 + halfN;
+        const cf a = in[i0];
+        const cf bt = in[i1] * w;  // butterfly_kernel.hpp:171-176 (DIT scatter)
+        out[o0] = a + bt;
+        out[o1] = a - bt;
+        if (cap && cap->pairs) {
+          cap->pairs[size_t(s) * N + o0] = cf(float(i0), float(i1));
+          cap->pairs[size_t(s) * N + o1] = cf(float(i0), float(i1));
+        }
+        if (cap && cap->twiddle_index) {
+          cap->twiddle_index[size_t(s) * N + o0] = cf(float(tw), 0.0f);
+          cap->twiddle_index[size_t(s) * N + o1] = cf(float(tw), 0.0f);
+        }
+      }
+    }
+    std::swap(in, out);
+  }
+  const float scale = P.inverse ? 1.0f / float(N) : 1.0f;
+  for (int i = 0; i < N; ++i) col[i] = P.inverse ? in[i] * scale : in[i];
+  post_transform(P, f, col);
+}
+
+// Kernel integer semantics at the C level (fft_cffi.cpp:799-800, eigen_fft.hpp:2296):
+// 2 selects Stockham, everything else runs the Cooley-Tukey default.
+bool is_stockham(int kernel) { return kernel == 2; }
+
+template <class Fn>
+void fan_out(size_t total, int threads, Fn fn) {
+  if (threads <= 1 || total < 2) { fn(size_t(0), total); return; }
+  const size_t nt = std::min<size_t>(size_t(threads), total);
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (size_t t = 0; t < nt; ++t) {
+    const size_t b = total * t / nt, e = total * (t + 1) / nt;
+    pool.emplace_back([=]() { fn(b, e); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+void transform_batch(const RefPlan& P, const Flags& f, int kernel, cf* data, size_t B,
+                     int threads) {
+  fan_out(B, threads, [&](size_t b, size_t e) {
+    std::vector<cf> ping, pong;
+    for (size_t c = b; c < e; ++c) {
+      cf* col = data + c * size_t(P.N);
+      if (is_stockham(kernel)) stockham_column(P, f, col, ping, pong, nullptr);
+      else ct_column(P, f, col, nullptr);
+    }
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+// Plan twiddles W[k], k < N/2, as interleaved floats (eigen_fft.hpp:2234-2242).
+int ref_twiddles(int N, int inverse, float* out) {
+  RefPlan P;
+  if (!make_plan(N, inverse != 0, P)) return 0;
+  std::memcpy(out, P.W.data(), sizeof(cf) * size_t(N / 2));
+  return 1;
+}
+
+// lgN-bit reversal table (eigen_fft.hpp:2244-2253).
+int ref_bitrev(int N, int* out) {
+  RefPlan P;
+  if (!make_plan(N, false, P)) return 0;
+  std::memcpy(out, P.bitrev.data(), sizeof(int) * size_t(N));
+  return 1;
+}
+
+// build_window + normalize_window (fft_cffi.cpp:163-208); writes W floats. Evaluated
+// in the source's order of operations (no fast-math reassociation) so the taps are a
+// deterministic function of the formulas: with reassociation, e.g. the Blackman end taps
+// come out as 0 or -1.4e-17 depending on the compiler.
+#pragma GCC push_options
+#pragma GCC optimize("no-fast-math")
+void ref_build_window(int kind, int W, float p1, float p2, int norm, float* out) {
+  (void)p2;
+  const double PI = 3.141592653589793238462643383279502884;
+  std::vector<float> w(size_t(std::max(W, 0)), 1.0f);
+  switch (kind) {
+    case 1:  // HANN
+    case 5:  // KAISER placeholder = Hann (fft_cffi.cpp:200-203)
+      for (int n = 0; n < W; ++n) w[n] = float(0.5 * (1.0 - std::cos(2.0 * PI * n / (W - 1))));
+      break;
+    case 2:
+      for (int n = 0; n < W; ++n) w[n] = float(0.54 - 0.46 * std::cos(2.0 * PI * n / (W - 1)));
+      break;
+    case 3:
+      for (int n = 0; n < W; ++n)
+        w[n] = float(0.42 - 0.5 * std::cos(2.0 * PI * n / (W - 1)) +
+                     0.08 * std::cos(4.0 * PI * n / (W - 1)));
+      break;
+    case 4: {
+      const double alpha = (p1 == 0.0f ? 0.5 : p1);
+      const int L = std::max(1, W - 1);
+      for (int n = 0; n < W; ++n) {
+        const double x = double(n) / double(L);
+        if (x < alpha / 2) w[n] = float(0.5 * (1 + std::cos(PI * (2 * x / alpha - 1))));
+        else if (x <= 1 - alpha / 2) w[n] = 1.0f;
+        else w[n] = float(0.5 * (1 + std::cos(PI * (2 * x / alpha - 2 / alpha + 1))));
+      }
+      break;
+    }
+    default:
+      break;
+  }
+  if (!w.empty() && norm == 1) {
+    double s2 = 0.0;
+    for (float v : w) s2 += double(v) * double(v);
+    s2 = std::sqrt(std::max(1e-30, s2));
+    for (auto& v : w) v = float(v / s2);
+  } else if (!w.empty() && norm == 2) {
+    double s = 0.0;
+    for (float v : w) s += double(v);
+    if (std::abs(s) > 1e-30) for (auto& v : w) v = float(v / s);
+  }
+  if (W > 0) std::memcpy(out, w.data(), sizeof(float) * size_t(W));
+}
+#pragma GCC pop_options
+
+// Frame count of fft_execute_batched (fft_cffi.cpp:715-750); 0 means "refused".
+size_t ref_frame_count(size_t pcm_len, int N, int window, int hop, int ctx_pad, int pad_mode,
+                       size_t max_frames) {
+  const int W = window ? window : N;
+  const int H = hop ? hop : W;
+  if (W <= 0 || H <= 0 || W > N || pcm_len == 0) return 0;
+  const int pad = (pad_mode == 0) ? ctx_pad : pad_mode;
+  size_t frames = 0;
+  if (pcm_len < size_t(W)) {
+    if (pad == 1) frames = 1; else return 0;
+  } else {
+    frames = 1 + (pcm_len - size_t(W)) / size_t(H);
+    const size_t last_start = (frames - 1) * size_t(H);
+    const size_t remaining = (pcm_len > last_start) ? (pcm_len - last_start) : 0;
+    if (remaining < size_t(W)) {
+      if (pad == 1) {
+      } else if (pad == 2) {
+        if (frames > 0) frames -= 1;
+      } else {
+        return 0;
+      }
+    }
+  }
+  if (frames == 0) return 0;
+  if (max_frames != 0 && frames > max_frames) frames = max_frames;
+  return frames;
+}
+
+// In-place batched transform over a column-major N x B complex buffer, i.e. the
+// reference's fft_inplace_batched (eigen_fft.hpp:3152-3175) with the plan's
+// transform/reduce/polar/half flags and the C-level kernel code.
+int ref_fft_batched(int N, int inverse, int kernel, int transform, int reduce_magnitude,
+                    int store_polar, int half_spectrum, float* data, size_t B, int threads) {
+  RefPlan P;
+  if (!make_plan(N, inverse != 0, P)) return 0;
+  Flags f{transform, reduce_magnitude != 0, store_polar != 0, half_spectrum != 0};
+  transform_batch(P, f, kernel, reinterpret_cast<cf*>(data), B, threads);
+  return 1;
+}
+
+// fft_execute_batched (fft_cffi.cpp:701-825, shared-plan fast path): frames the PCM,
+// applies the analysis window (first W taps, only when windows are enabled), zero-pads to
+// N, transforms, then writes three frame-major planes of `bins` floats each.
+// `window_taps` holds N floats (taps >= W are 1.0) or is NULL when windows are off.
+size_t ref_execute_batched(int N, int inverse, int kernel, int window, int hop, int ctx_pad,
+                           int transform, int reduce_magnitude, int store_polar,
+                           int half_spectrum, const float* window_taps, const float* pcm,
+                           size_t pcm_len, float* out_real, float* out_imag, float* out_mag,
+                           int pad_mode, size_t max_frames, int threads) {
+  if (!pcm || pcm_len == 0 || !out_real || !out_imag || !out_mag) return 0;
+  RefPlan P;
+  if (!make_plan(N, inverse != 0, P)) return 0;
+  const int W = window ? window : N;
+  const int H = hop ? hop : W;
+  const size_t frames =
+      ref_frame_count(pcm_len, N, window, hop, ctx_pad, pad_mode, max_frames);
+  if (frames == 0) return 0;
+  Flags f{transform, reduce_magnitude != 0, store_polar != 0, half_spectrum != 0};
+  const size_t bins = half_spectrum ? size_t(N / 2 + 1) : size_t(N);
+  fan_out(frames, threads, [&](size_t b, size_t e) {
+    std::vector<cf> col{}, ping, pong;
+    col.resize(size_t(N));
+    for (size_t fr = b; fr < e; ++fr) {
+      const size_t start = fr * size_t(H);
+      size_t copy = 0;
+      if (start < pcm_len) copy = std::min(size_t(W), pcm_len - start);
+      for (size_t i = 0; i < copy; ++i)
+        col[i] = cf(window_taps ? pcm[start + i] * window_taps[i] : pcm[start + i], 0.0f);
+      for (size_t i = copy; i < size_t(N); ++i) col[i] = cf(0.0f, 0.0f);
+      if (is_stockham(kernel)) stockham_column(P, f, col.data(), ping, pong, nullptr);
+      else ct_column(P, f, col.data(), nullptr);
+      const size_t ob = fr * bins;
+      for (size_t i = 0; i < bins; ++i) {
+        out_real[ob + i] = col[i].real();
+        out_imag[ob + i] = col[i].imag();
+        out_mag[ob + i] = store_polar ? col[i].real() : std::abs(col[i]);
+      }
+    }
+  });
+  return frames;
+}
+
+// fft_execute_complex_batched without OLA (fft_cffi.cpp:1042-1157): frame-major complex
+// bins in, Hermitian rebuild for half-spectrum C2R, transform, real part times the
+// synthesis window (NULL = off) out, N floats per frame.
+size_t ref_execute_complex_batched(int N, int inverse, int kernel, int transform,
+                                   int reduce_magnitude, int store_polar, int half_spectrum,
+                                   const float* synth_taps, const float* in_real,
+                                   const float* in_imag, size_t frames, float* out_pcm,
+                                   size_t max_frames, int threads) {
+  if (!in_real || !in_imag || !out_pcm || frames == 0) return 0;
+  RefPlan P;
+  if (!make_plan(N, inverse != 0, P)) return 0;
+  if (max_frames != 0 && frames > max_frames) frames = max_frames;
+  Flags f{transform, reduce_magnitude != 0, store_polar != 0, half_spectrum != 0};
+  const size_t bins = half_spectrum ? size_t(N / 2 + 1) : size_t(N);
+  fan_out(frames, threads, [&](size_t b, size_t e) {
+    std::vector<cf> col{}, ping, pong;
+    col.resize(size_t(N));
+    for (size_t fr = b; fr < e; ++fr) {
+      for (size_t i = 0; i < bins; ++i) col[i] = cf(in_real[fr * bins + i], in_imag[fr * bins + i]);
+      if (half_spectrum && transform == 2) {
+        col[0].imag(0.0f);
+        const size_t ny = size_t(N) / 2;
+        if (ny < bins) col[ny].imag(0.0f);
+        const size_t maxk = (bins > ny + 1) ? ny : (bins - 1);
+        for (size_t k = 1; k < maxk; ++k) col[size_t(N) - k] = std::conj(col[k]);
+      } else {
+        for (size_t i = bins; i < size_t(N); ++i) col[i] = cf(0.0f, 0.0f);
+      }
+      if (is_stockham(kernel)) stockham_column(P, f, col.data(), ping, pong, nullptr);
+      else ct_column(P, f, col.data(), nullptr);
+      for (size_t i = 0; i < size_t(N); ++i) {
+        float s = col[i].real();
+        if (synth_taps) s *= synth_taps[i];
+        out_pcm[fr * size_t(N) + i] = s;
+      }
+    }
+  });
+  return frames;
+}
+
+// Metadata of one column as the reference writers record it. kind follows
+// eigfft::MetadataKind order (eigen_fft.hpp:1477-1491): 0 Twiddle, 1 LayoutMap,
+// 3 StageParams, 4 ButterflyPairs, 6 TwiddleIndexMap, 7 Omega. Returns the entry count
+// written (complex<float> entries), 0 if the kind is not recorded by that kernel.
+size_t ref_metadata(int N, int inverse, int kernel, int transform, int half_spectrum, int kind,
+                    float* out, size_t capacity) {
+  RefPlan P;
+  if (!make_plan(N, inverse != 0, P)) return 0;
+  const int S = P.lgN;
+  std::vector<cf> buf;
+  Capture cap;
+  size_t count = 0;
+  switch (kind) {
+    case 0: {  // Twiddle: record_plan_twiddles (eigen_fft.hpp:1566-1577)
+      count = (size_t(1) << S) - 1;
+      buf.assign(count, cf(0, 0));
+      for (int s = 0; s < S; ++s) {
+        const int m = 1 << s, step = N / (m << 1);
+        for (int j = 0; j < m; ++j) buf[size_t(m - 1 + j)] = P.W[j * step];
+      }
+      break;
+    }
+    case 1:  // LayoutMap: identity for both kernels (eigen_fft.hpp:2672-2682, 3239-3244)
+      count = size_t(N);
+      buf.resize(count);
+      for (int p = 0; p < N; ++p) buf[p] = cf(float(p), 0.0f);
+      break;
+    case 7: {  // Omega: Cooley-Tukey only (eigen_fft.hpp:1752-1762, 2665-2667)
+      if (is_stockham(kernel)) return 0;
+      const size_t upper = (transform == 1 && half_spectrum) ? size_t(N / 2 + 1) : size_t(N);
+      count = upper;
+      buf.assign(count, cf(0, 0));
+      const float two_pi = float(2 * std::acos(float(-1)));
+      for (size_t k = 0; k < upper; ++k) buf[k] = cf(two_pi * float(k) / float(N), 0.0f);
+      break;
+    }
+    case 3:
+    case 4:
+    case 6: {
+      count = (kind == 3) ? size_t(S) : size_t(N) * size_t(S);
+      buf.assign(count, cf(0, 0));
+      if (kind == 3) cap.stage_params = buf.data();
+      if (kind == 4) cap.pairs = buf.data();
+      if (kind == 6) cap.twiddle_index = buf.data();
+      std::vector<cf> col(size_t(N), cf(0, 0)), ping, pong;
+      Flags f{};
+      if (is_stockham(kernel)) stockham_column(P, f, col.data(), ping, pong, &cap);
+      else ct_column(P, f, col.data(), &cap);
+      break;
+    }
+    default:
+      return 0;
+  }
+  if (count > capacity) return 0;
+  std::memcpy(out, buf.data(), sizeof(cf) * count);
+  return count;
+}
+
+}  // extern "C"

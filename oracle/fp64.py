@@ -1,0 +1,205 @@
+"""FP64 oracle for the fftfree batched FFT/STFT path. TEST INFRASTRUCTURE ONLY.
+
+The reference computes in float32 (std::complex<float> plans). This module restates what
+its float path is meant to compute, in float64, from the *same float32 inputs* the
+reference feeds its butterflies:
+
+* framed samples are the float32 products ``pcm[s+i] * win[i]`` (fft_cffi.cpp:772-775);
+* windows are the float32 casts of the double-precision formulas (fft_cffi.cpp:163-208),
+  built here with ``math.cos`` (libm double, as ``std::cos``) so they are bit-exact;
+* the DFT itself is numpy's float64 FFT (checked against a direct O(N^2) sum in tests);
+* inverse plans scale by 1/N (eigen_fft.hpp:3076-3085, 3540-3556);
+* the post-transform precedence polar > magnitude > C2R-imag-clear and the R2C
+  half-spectrum DC/Nyquist rule follow eigen_fft.hpp:2535-2596, and the output planes
+  follow fft_cffi.cpp:811-825.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+WINDOW_RECT, WINDOW_HANN, WINDOW_HAMMING, WINDOW_BLACKMAN, WINDOW_TUKEY, WINDOW_KAISER = range(6)
+NORM_NONE, NORM_L2, NORM_AREA = range(3)
+C2C, R2C, C2R, R2R = range(4)
+
+
+def dft_direct(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """Direct O(N^2) DFT over the last axis in complex128 (no library FFT)."""
+    x = np.asarray(x, dtype=np.complex128)
+    n = x.shape[-1]
+    k = np.arange(n)
+    sgn = 1.0 if inverse else -1.0
+    # exact integer phase index (k*m mod n) keeps the twiddle argument small
+    idx = np.outer(k, k) % n
+    mat = np.exp(sgn * 2j * np.pi * idx / n)
+    y = x @ mat.T
+    return y / n if inverse else y
+
+
+def fft64(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """FP64 DFT over the last axis: X[k] = sum_n x[n] e^{-+2 pi i kn/N}, inverse scaled 1/N."""
+    x = np.asarray(x, dtype=np.complex128)
+    return np.fft.ifft(x, axis=-1) if inverse else np.fft.fft(x, axis=-1)
+
+
+def build_window(kind: int, W: int, p1: float = 0.0, p2: float = 0.0, norm: int = NORM_NONE) -> np.ndarray:
+    """float32 window taps, bit-exact restatement of build_window + normalize_window
+    (fft_cffi.cpp:163-208). KAISER is the reference's Hann placeholder (200-203)."""
+    f32 = np.float32
+    w = np.ones(max(W, 0), dtype=np.float32)
+    pi = 3.141592653589793238462643383279502884
+    if kind in (WINDOW_HANN, WINDOW_KAISER):
+        for n in range(W):
+            w[n] = f32(0.5 * (1.0 - math.cos(2.0 * pi * n / (W - 1))))
+    elif kind == WINDOW_HAMMING:
+        for n in range(W):
+            w[n] = f32(0.54 - 0.46 * math.cos(2.0 * pi * n / (W - 1)))
+    elif kind == WINDOW_BLACKMAN:
+        for n in range(W):
+            w[n] = f32(0.42 - 0.5 * math.cos(2.0 * pi * n / (W - 1)) + 0.08 * math.cos(4.0 * pi * n / (W - 1)))
+    elif kind == WINDOW_TUKEY:
+        alpha = 0.5 if float(np.float32(p1)) == 0.0 else float(np.float32(p1))
+        L = max(1, W - 1)
+        for n in range(W):
+            x = n / L
+            if x < alpha / 2:
+                w[n] = f32(0.5 * (1 + math.cos(pi * (2 * x / alpha - 1))))
+            elif x <= 1 - alpha / 2:
+                w[n] = f32(1.0)
+            else:
+                w[n] = f32(0.5 * (1 + math.cos(pi * (2 * x / alpha - 2 / alpha + 1))))
+    if W > 0 and norm == NORM_L2:
+        s2 = 0.0
+        for v in w:
+            s2 += float(v) * float(v)
+        s2 = math.sqrt(max(1e-30, s2))
+        w = np.array([f32(float(v) / s2) for v in w], dtype=np.float32)
+    elif W > 0 and norm == NORM_AREA:
+        s = 0.0
+        for v in w:
+            s += float(v)
+        if abs(s) > 1e-30:
+            w = np.array([f32(float(v) / s) for v in w], dtype=np.float32)
+    return w
+
+
+def frame_count(pcm_len: int, N: int, window: int, hop: int, ctx_pad: int, pad_mode: int, max_frames: int = 0) -> int:
+    """Frames fft_execute_batched produces (fft_cffi.cpp:715-750); 0 = refused."""
+    W = window if window else N
+    H = hop if hop else W
+    if W <= 0 or H <= 0 or W > N or pcm_len == 0:
+        return 0
+    pad = ctx_pad if pad_mode == 0 else pad_mode
+    if pcm_len < W:
+        if pad != 1:
+            return 0
+        frames = 1
+    else:
+        frames = 1 + (pcm_len - W) // H
+        remaining = pcm_len - (frames - 1) * H
+        if remaining < W:
+            if pad == 1:
+                pass
+            elif pad == 2:
+                frames -= 1
+            else:
+                return 0
+    if frames == 0:
+        return 0
+    if max_frames and frames > max_frames:
+        frames = max_frames
+    return frames
+
+
+def frame_signal(pcm: np.ndarray, N: int, W: int, H: int, frames: int, win: np.ndarray | None) -> np.ndarray:
+    """frames x N float32 matrix of windowed, zero-padded frames (fft_cffi.cpp:766-791).
+    The product pcm*win is rounded to float32, as the reference's float multiply is."""
+    pcm = np.asarray(pcm, dtype=np.float32)
+    L = pcm.shape[0]
+    out = np.zeros((frames, N), dtype=np.float32)
+    for f in range(frames):
+        s = f * H
+        c = min(W, L - s) if s < L else 0
+        if c > 0:
+            seg = pcm[s:s + c]
+            out[f, :c] = seg * win[:c] if win is not None else seg
+    return out
+
+
+def post_planes(X: np.ndarray, N: int, transform: int, reduce_magnitude: bool, store_polar: bool,
+                half_spectrum: bool):
+    """(out_real, out_imag, out_mag) in float64 from spectra X (frames x N complex128),
+    per eigen_fft.hpp:2535-2596 and fft_cffi.cpp:811-825."""
+    X = np.array(X, dtype=np.complex128, copy=True)
+    r2c_half = transform == R2C and half_spectrum
+    if r2c_half:
+        X[:, 0] = X[:, 0].real
+        X[:, N // 2] = X[:, N // 2].real
+        X[:, N // 2 + 1:] = 0
+    bins = N // 2 + 1 if half_spectrum else N
+    X = X[:, :bins]
+    if store_polar:
+        mag = np.abs(X)
+        return mag, np.angle(X), mag
+    if reduce_magnitude:
+        mag = np.abs(X)
+        return mag, np.zeros_like(mag), mag
+    if transform == C2R:
+        re = X.real
+        return re, np.zeros_like(re), np.abs(re)
+    return X.real, X.imag, np.abs(X)
+
+
+def stft_planes(pcm, N, W, H, frames, win, inverse, transform, reduce_magnitude, store_polar, half_spectrum):
+    """FP64 oracle of fft_execute_batched: framed float32 input, float64 DFT, planes."""
+    x = frame_signal(pcm, N, W, H, frames, win)
+    X = fft64(x.astype(np.float64), inverse)
+    return post_planes(X, N, transform, reduce_magnitude, store_polar, half_spectrum)
+
+
+def rel_l2(a, b) -> float:
+    """||a - b||_2 / ||b||_2 over all entries (b is the reference)."""
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def tolerance(N: int) -> float:
+    """Relative-L2 bound against this oracle: 2e-7 * log2(N)."""
+    return 2e-7 * max(1.0, math.log2(N))
+
+
+def stft_magnitude_grad(pcm, y_mag, N, W, H, frames, win, grad_phase=None):
+    """FP64 gradient of L = sum_{f,k} (|X_fk| - Y_fk)^2 with respect to the float32 PCM,
+    for the half spectrum k = 0..N/2 of the windowed frames (hand derivation:
+    dL/dx[fH+n] = w[n] * Re(sum_k U_fk e^{+2 pi i kn/N}), U = 2(|X|-Y) X/|X|).
+    ``grad_phase`` (frames x bins) adds dL/d(arg X) terms: U += i * Gp * X/|X|^2."""
+    pcm = np.asarray(pcm, dtype=np.float64)
+    L = pcm.shape[0]
+    win64 = np.asarray(win, dtype=np.float64) if win is not None else np.ones(W)
+    x = np.zeros((frames, N))
+    for f in range(frames):
+        s = f * H
+        c = min(W, L - s) if s < L else 0
+        if c > 0:
+            x[f, :c] = pcm[s:s + c] * win64[:c]
+    X = np.fft.fft(x, axis=-1)[:, :N // 2 + 1]
+    mag = np.abs(X)
+    safe = np.where(mag > 0, mag, 1.0)
+    U = 2.0 * (mag - y_mag) * X / safe
+    if grad_phase is not None:
+        U = U + 1j * grad_phase * X / (safe * safe)
+    U = np.where(mag > 0, U, 0)
+    n = np.arange(N)
+    k = np.arange(N // 2 + 1)
+    E = np.exp(2j * np.pi * np.outer(k, n) / N)  # bins x N
+    g_frames = (U @ E).real  # frames x N
+    g = np.zeros(L)
+    for f in range(frames):
+        s = f * H
+        c = min(W, L - s) if s < L else 0
+        if c > 0:
+            g[s:s + c] += g_frames[f, :c] * win64[:c]
+    return g

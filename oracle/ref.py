@@ -1,0 +1,131 @@
+"""ctypes binding of oracle/cpu_ref.cpp (libcpu_ref.so). TEST INFRASTRUCTURE ONLY.
+
+cpu_ref.cpp restates the reference's float CPU path (Cooley-Tukey and Stockham radix-2,
+framing, window, output planes); see the header of that file for the reference regions.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libcpu_ref.so"
+_lib = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int)
+_sz = ctypes.c_size_t
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_f32p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing: run oracle/build.py (or __graft_entry__.build())")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.ref_twiddles.argtypes = [ctypes.c_int, ctypes.c_int, _f32p]
+        L.ref_bitrev.argtypes = [ctypes.c_int, _i32p]
+        L.ref_build_window.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, _f32p]
+        L.ref_build_window.restype = None
+        L.ref_frame_count.argtypes = [_sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz]
+        L.ref_frame_count.restype = _sz
+        L.ref_fft_batched.argtypes = [ctypes.c_int] * 7 + [_f32p, _sz, ctypes.c_int]
+        L.ref_execute_batched.argtypes = ([ctypes.c_int] * 10 + [_f32p, _f32p, _sz, _f32p, _f32p, _f32p,
+                                                                 ctypes.c_int, _sz, ctypes.c_int])
+        L.ref_execute_batched.restype = _sz
+        L.ref_execute_complex_batched.argtypes = ([ctypes.c_int] * 7 + [_f32p, _f32p, _f32p, _sz, _f32p, _sz,
+                                                                        ctypes.c_int])
+        L.ref_execute_complex_batched.restype = _sz
+        L.ref_metadata.argtypes = [ctypes.c_int] * 6 + [_f32p, _sz]
+        L.ref_metadata.restype = _sz
+        _lib = L
+    return _lib
+
+
+def default_threads() -> int:
+    """min(16, cores): the reference clamps plan threads to 16 (eigen_fft.hpp:1966)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def twiddles(N: int, inverse: bool = False) -> np.ndarray:
+    out = np.zeros(max(1, N // 2), dtype=np.complex64)
+    if not lib().ref_twiddles(N, int(inverse), out.view(np.float32).ctypes.data_as(_f32p)):
+        raise ValueError("N must be a power of two")
+    return out[: N // 2]
+
+
+def bitrev(N: int) -> np.ndarray:
+    out = np.zeros(N, dtype=np.int32)
+    if not lib().ref_bitrev(N, out.ctypes.data_as(_i32p)):
+        raise ValueError("N must be a power of two")
+    return out
+
+
+def build_window(kind: int, W: int, p1: float = 0.0, p2: float = 0.0, norm: int = 0) -> np.ndarray:
+    out = np.zeros(W, dtype=np.float32)
+    lib().ref_build_window(kind, W, p1, p2, norm, _p(out))
+    return out
+
+
+def frame_count(pcm_len, N, window, hop, ctx_pad, pad_mode, max_frames=0) -> int:
+    return int(lib().ref_frame_count(pcm_len, N, window, hop, ctx_pad, pad_mode, max_frames))
+
+
+def fft_batched(x: np.ndarray, inverse=False, kernel=0, transform=0, reduce_magnitude=False,
+                store_polar=False, half_spectrum=False, threads=None) -> np.ndarray:
+    """x: (B, N) complex64; returns the transformed copy (reference fft_inplace_batched)."""
+    x = np.ascontiguousarray(x, dtype=np.complex64).copy()
+    B, N = x.shape
+    ok = lib().ref_fft_batched(N, int(inverse), kernel, transform, int(reduce_magnitude), int(store_polar),
+                               int(half_spectrum), x.view(np.float32).ctypes.data_as(_f32p), B,
+                               threads or default_threads())
+    if not ok:
+        raise ValueError("N must be a power of two")
+    return x
+
+
+def execute_batched(pcm: np.ndarray, N, window=0, hop=0, ctx_pad=0, pad_mode=0, max_frames=0, inverse=False,
+                    kernel=0, transform=0, reduce_magnitude=False, store_polar=False, half_spectrum=False,
+                    window_taps: np.ndarray | None = None, threads=None):
+    """Reference fft_execute_batched; returns (frames, real, imag, mag) as (frames, bins)."""
+    pcm = np.ascontiguousarray(pcm, dtype=np.float32)
+    bins = N // 2 + 1 if half_spectrum else N
+    cap = frame_count(pcm.size, N, window, hop, ctx_pad, pad_mode, max_frames)
+    re = np.zeros((max(cap, 1), bins), np.float32)
+    im = np.zeros_like(re)
+    mg = np.zeros_like(re)
+    taps = None if window_taps is None else np.ascontiguousarray(window_taps, dtype=np.float32)
+    got = lib().ref_execute_batched(N, int(inverse), kernel, window, hop, ctx_pad, transform, int(reduce_magnitude),
+                                    int(store_polar), int(half_spectrum), _p(taps), _p(pcm), pcm.size,
+                                    _p(re), _p(im), _p(mg), pad_mode, max_frames, threads or default_threads())
+    return int(got), re[:got], im[:got], mg[:got]
+
+
+def execute_complex_batched(in_real, in_imag, N, inverse=True, kernel=0, transform=2, reduce_magnitude=False,
+                            store_polar=False, half_spectrum=True, synth_taps=None, max_frames=0, threads=None):
+    in_real = np.ascontiguousarray(in_real, dtype=np.float32)
+    in_imag = np.ascontiguousarray(in_imag, dtype=np.float32)
+    frames = in_real.shape[0]
+    out = np.zeros((frames, N), np.float32)
+    taps = None if synth_taps is None else np.ascontiguousarray(synth_taps, dtype=np.float32)
+    got = lib().ref_execute_complex_batched(N, int(inverse), kernel, transform, int(reduce_magnitude),
+                                            int(store_polar), int(half_spectrum), _p(taps), _p(in_real),
+                                            _p(in_imag), frames, _p(out), max_frames, threads or default_threads())
+    return int(got), out[:got]
+
+
+def metadata(N: int, kind: int, kernel: int = 0, inverse=False, transform=0, half_spectrum=False) -> np.ndarray:
+    cap = max(4 * N * max(1, N.bit_length()), 16)
+    out = np.zeros(cap, dtype=np.complex64)
+    n = lib().ref_metadata(N, int(inverse), kernel, transform, int(half_spectrum), kind,
+                           out.view(np.float32).ctypes.data_as(_f32p), cap)
+    return out[:n]

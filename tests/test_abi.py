@@ -1,0 +1,132 @@
+"""The C ABI library on the CPU: it loads, exports every symbol include/fft_cffi.h
+declares, and its host-side logic (argument checks, padding rule, frame count) matches
+the reference. Contexts are created lazily on the device, so none of this needs a GPU
+and no compute entry point is called here."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fftfree_amd import _lib
+from oracle import fp64
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "fft_cffi.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    names = re.findall(r"\b(?:void\s*\*|void|int|size_t)\s+(fft_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _lib.load()
+
+
+def test_header_parses_to_the_full_entry_list():
+    names = declared_functions()
+    assert len(names) >= 25
+    for must in ("fft_init_full", "fft_init_full_v2", "fft_execute_batched", "fft_execute_batched_device",
+                 "fft_execute_c2c_batched_device", "fft_plan_metadata"):
+        assert must in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert set(declared_functions()) == set(_lib.SIGNATURES)
+
+
+def init_v2(lib, n, **kw):
+    a = dict(threads=1, lanes=1, inverse=0, kernel=0, radix=0, pad_mode=0, window=0, hop=0, transform=1,
+             reduce_magnitude=0, store_polar=0, half_spectrum=1, apply_windows=0, analysis=0)
+    a.update(kw)
+    return lib.fft_init_full_v2(n, a["threads"], a["lanes"], a["inverse"], a["kernel"], a["radix"], None, 0,
+                                a["pad_mode"], a["window"], a["hop"], a["transform"], a["reduce_magnitude"],
+                                a["store_polar"], a["half_spectrum"], 0, 0, 0, 0, 1, a["apply_windows"], 0,
+                                a["analysis"], 0.0, 0.0, 0, 0.0, 0.0, 0, 0)
+
+
+def test_init_refuses_non_power_of_two_and_zero(lib):
+    # Plan ctor throws for non-power-of-two N -> NULL (fft_cffi.cpp:1955-1973)
+    for n in (0, 3, 1000, 1025):
+        assert not init_v2(lib, n)
+    for n in (1, 2, 1024, 1 << 22):
+        h = init_v2(lib, n)
+        assert h
+        assert lib.fft_ctx_size(h) == n
+        lib.fft_free(h)
+
+
+def test_init_ex_pads_only_when_asked(lib):
+    # fft_cffi.cpp:609-632
+    assert not lib.fft_init_ex(1000, 1, 1, 0, 0, 0, None, 0, 0, 0, 0)
+    h = lib.fft_init_ex(1000, 1, 1, 0, 0, 0, None, 0, 1, 0, 0)
+    assert h and lib.fft_ctx_size(h) == 1024
+    lib.fft_free(h)
+    h = lib.fft_init(512, 2, 1, 0, 2, 4, None, 0, 0)
+    assert h and lib.fft_ctx_size(h) == 512
+    lib.fft_free(h)
+
+
+def test_null_handles_are_harmless(lib):
+    lib.fft_free(None)
+    assert lib.fft_ctx_size(None) == 0
+    assert lib.fft_frame_count(None, 100, 1, 0) == 0
+    buf = np.zeros(16, np.float32)
+    p = buf.ctypes.data_as(_lib.f32p)
+    assert lib.fft_execute_batched(None, p, 16, p, p, p, 1, 0, 0) == 0
+    assert lib.fft_execute(None, p, p, p, p, 16) == 0
+    assert lib.fft_config_window(None, 1, 1, 0.0, 0.0, 0, 0) == 0
+    # reference test-only hooks exist and do nothing
+    lib.fft_set_workerpool_debug_dropout(50)
+    lib.fft_set_workerpool_dropout_pattern(4, 1, 0)
+    lib.fft_set_workerpool_dropout_single(3)
+    lib.fft_set_workerpool_dropout_persistent(1)
+    lib.fft_clear_workerpool_dropout_history()
+    lib.fft_register_restore_observer(None)
+
+
+def test_frame_count_matches_reference_rule(lib):
+    for n, W, H in [(1024, 1024, 256), (1024, 0, 0), (512, 256, 100), (256, 512, 64), (2048, 2048, 512)]:
+        for ctx_pad in (0, 1, 2):
+            h = init_v2(lib, n, window=W, hop=H, pad_mode=ctx_pad)
+            for L in (0, 1, 255, 256, 511, 1024, 1025, 5000, 1 << 24):
+                for pad in (0, 1, 2):
+                    for mx in (0, 2):
+                        want = fp64.frame_count(L, n, W, H, ctx_pad, pad, mx)
+                        assert lib.fft_frame_count(h, L, pad, mx) == want
+            lib.fft_free(h)
+
+
+def test_thread_queries(lib):
+    h = init_v2(lib, 1024, threads=64)
+    assert 1 <= lib.fft_ctx_effective_threads(h) <= 16
+    assert lib.fft_ctx_worker_threads(h) >= 1
+    lib.fft_free(h)
+
+
+def test_window_config_arguments(lib):
+    h = init_v2(lib, 256, window=128, hop=32)
+    assert lib.fft_config_window(h, 1, 2, 0.0, 0.0, 1, 1) == 1
+    w = np.ones(128, np.float32)
+    assert lib.fft_config_window_custom(h, w.ctypes.data_as(_lib.f32p), 128, None, 0, 0, 0) == 1
+    lib.fft_free(h)
+
+
+def test_variant_queries(lib):
+    h = init_v2(lib, 1024)
+    n = lib.fft_variant_count(h)
+    assert n >= 1
+    buf = ctypes.create_string_buffer(64)
+    assert lib.fft_variant_name(h, 0, buf, 64) == 1 and b"fpw" in buf.value
+    assert lib.fft_set_variant(h, n) == 0
+    assert lib.fft_set_variant(h, 0) == 1
+    assert lib.fft_set_variant(h, -1) == 1
+    lib.fft_free(h)

@@ -1,0 +1,264 @@
+"""Parity of the HIP path (through the C ABI) against the oracle.
+
+Tolerance for floating-point outputs: relative L2 <= 2e-7 * log2(N) against the FP64
+oracle (fp64.tolerance); integer/index outputs and frame counts: bit-exact.
+"""
+import numpy as np
+import pytest
+
+from fftfree_amd import CffiFft
+from fftfree_amd.workloads import complex_normal, noise_plus_sines
+from oracle import fp64, index_maps as M, ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ff():
+    f = CffiFft()
+    yield f
+    f.close()
+
+
+def spectra_close(re, im, ore, oim, polar, tol):
+    if polar:
+        return fp64.rel_l2(re * np.exp(1j * im), ore * np.exp(1j * oim)) <= tol
+    return fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= tol
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("inverse", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_c2c_all_lds_sizes(ff, n, inverse, kernel):
+    print(f"c2c n={n} inverse={inverse} kernel={kernel}", flush=True)
+    p = ff.make_plan(n, inverse=inverse, kernel=kernel, transform=0, half_spectrum=False)
+    x = complex_normal(37, n, seed=n + 7 * inverse)
+    y = ff.c2c(p, x)
+    assert fp64.rel_l2(y, fp64.fft64(x, bool(inverse))) <= fp64.tolerance(n)
+
+
+def test_config1_c2c_1024_stockham_and_metadata(ff):
+    """Plain C2C forward, Stockham (kernel code 2), N=1024, radix 4, batch 64, complex64
+    N(0,1): FP64-oracle error within 2e-7*10; LayoutMap and TwiddleIndexMap bit-exact."""
+    n = 1024
+    p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
+    x = complex_normal(64, n, seed=1)
+    y = ff.c2c(p, x)
+    err = fp64.rel_l2(y, fp64.fft64(x))
+    print(f"config1 rel_l2={err:.3e} tol={fp64.tolerance(n):.1e}")
+    assert err <= fp64.tolerance(n)
+    S = 10
+    lm = ff.metadata(p, 1)
+    assert np.array_equal(lm.real.astype(np.int64), M.layout_map(n)) and not lm.imag.any()
+    ti = ff.metadata(p, 6)
+    assert np.array_equal(ti.real.astype(np.int64).reshape(S, n), M.twiddle_index_map(n, 2)) and not ti.imag.any()
+    bp = ff.metadata(p, 4)
+    assert np.array_equal(np.stack([bp.real, bp.imag], -1).astype(np.int64).reshape(S, n, 2), M.butterfly_pairs(n, 2))
+    sp = ff.metadata(p, 3)
+    assert np.array_equal(np.stack([sp.real, sp.imag], -1).astype(np.int64), M.stage_params(n))
+    assert ff.metadata(p, 7).size == 0  # Omega is recorded by Cooley-Tukey only
+
+
+@pytest.mark.parametrize("n", [2, 8, 64, 1024, 4096])
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_metadata_all_kinds_bit_exact(ff, n, kernel, inverse):
+    p = ff.make_plan(n, kernel=kernel, inverse=inverse, transform=1, half_spectrum=True)
+    S = M.log2_exact(n)
+    ti = ff.metadata(p, 6)
+    assert np.array_equal(ti.real.astype(np.int64).reshape(S, n), M.twiddle_index_map(n, kernel))
+    bp = ff.metadata(p, 4)
+    assert np.array_equal(np.stack([bp.real, bp.imag], -1).astype(np.int64).reshape(S, n, 2),
+                          M.butterfly_pairs(n, kernel))
+    om = ff.metadata(p, 7)
+    if kernel == 2:
+        assert om.size == 0
+    else:
+        assert np.array_equal(om.real.view(np.uint32), M.omega(n, 1, True).view(np.uint32))
+    # Twiddle metadata holds the plan's FP64-built twiddles: exact to float rounding
+    tw = ff.metadata(p, 0)
+    idx = M.twiddle_metadata_index(n)
+    sgn = 1.0 if inverse else -1.0
+    assert np.max(np.abs(tw - np.exp(sgn * 2j * np.pi * idx / n))) <= 6e-8
+
+
+def stft_case(ff, pcm, n, W, H, *, kernel=2, inverse=0, transform=1, reduce_mag=0, polar=0, half=1, windows=1,
+              win_kind=1, pad_mode=1, ctx_pad=0, max_frames=0):
+    p = ff.make_plan(n, window=W, hop=H, kernel=kernel, inverse=inverse, transform=transform,
+                     reduce_magnitude=bool(reduce_mag), store_polar=bool(polar), half_spectrum=bool(half),
+                     apply_windows=bool(windows), analysis_window=win_kind, pad_mode=ctx_pad)
+    got, re, im, mg = ff.stft_frames(p, pcm, pad_mode=pad_mode, max_frames=max_frames)
+    frames = fp64.frame_count(pcm.size, n, W, H, ctx_pad, pad_mode, max_frames)
+    assert got == frames
+    win = fp64.build_window(win_kind, W or n) if windows else None
+    return p, got, (re, im, mg), win
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192])
+def test_stft_r2c_half_magnitude_all_sizes(ff, n):
+    print(f"stft n={n}", flush=True)
+    pcm = noise_plus_sines(20 * n + 3, seed=n)
+    W, H = n, max(1, n // 4)
+    # the symmetric Hann formula divides by W-1, so windows start at W=4
+    p, got, (re, im, mg), win = stft_case(ff, pcm, n, W, H, reduce_mag=1, windows=int(n >= 4))
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, win, False, 1, True, False, True)
+    assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
+    assert fp64.rel_l2(re, ore) <= fp64.tolerance(n)
+    assert not im.any()
+
+
+FLAGS = [(t, r, pz, h) for t in (0, 1, 2, 3) for (r, pz, h) in [(0, 0, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 0)]]
+
+
+@pytest.mark.parametrize("transform,reduce_mag,polar,half", FLAGS)
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_stft_flag_matrix_vs_oracle_and_reference(ff, transform, reduce_mag, polar, half, kernel, inverse):
+    n, W, H = 256, 200, 70
+    pcm = noise_plus_sines(3000, seed=transform * 10 + kernel)
+    p, got, (re, im, mg), win = stft_case(ff, pcm, n, W, H, kernel=kernel, inverse=inverse, transform=transform,
+                                          reduce_mag=reduce_mag, polar=polar, half=half)
+    taps = np.ones(n, np.float32)
+    taps[:W] = win
+    rgot, rre, rim, rmg = ref.execute_batched(pcm, n, W, H, ctx_pad=0, pad_mode=1, inverse=bool(inverse),
+                                              kernel=kernel, transform=transform, reduce_magnitude=reduce_mag,
+                                              store_polar=polar, half_spectrum=half, window_taps=taps)
+    assert rgot == got
+    tol = fp64.tolerance(n)
+    if transform == 2 and half and kernel != 2:
+        # Cooley-Tukey's C2R half hook rebuilds a Hermitian input from the framed samples:
+        # the restated reference path is the oracle here
+        assert spectra_close(re, im, rre, rim, polar, 4 * tol)
+        assert fp64.rel_l2(mg, rmg) <= 4 * tol
+        return
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, win, bool(inverse), transform, reduce_mag, polar, half)
+    assert spectra_close(re, im, ore, oim, polar, tol)
+    assert fp64.rel_l2(mg, omg) <= tol
+    assert spectra_close(re, im, rre, rim, polar, 4 * tol)
+
+
+def test_reference_batched_stft_vs_eigen_case(ff):
+    """cffi_stft_compare_test.cpp:406-475: N=W=512, H=128, 8 frames, Hann through v2
+    apply_windows=1, pad_mode=1, R2C full spectrum; max abs error <= 1e-4."""
+    n, W, H = 512, 512, 128
+    pcm = np.random.default_rng(42).standard_normal(W + 7 * H).astype(np.float32)
+    p = ff.make_plan(n, window=W, hop=H, transform=1, half_spectrum=False, apply_windows=True, analysis_window=1,
+                     synthesis_window=1, pad_mode=1)
+    got, re, im, mg = ff.stft_frames(p, pcm, pad_mode=1)
+    assert got == 8
+    ore, oim, _ = fp64.stft_planes(pcm, n, W, H, 8, fp64.build_window(1, W), False, 1, False, False, False)
+    assert np.max(np.abs((re + 1j * im) - (ore + 1j * oim))) <= 1e-4
+
+
+def test_reference_plain_fft_vs_eigen_case(ff):
+    """cffi_stft_compare_test.cpp:344-404: fft_execute, N=1024 normal PCM, R2C, 1e-4."""
+    n = 1024
+    pcm = np.random.default_rng(1337).standard_normal(n).astype(np.float32)
+    p = ff.make_plan(n, transform=1, half_spectrum=False, pad_mode=1)
+    re = np.zeros(n, np.float32)
+    im = np.zeros(n, np.float32)
+    mg = np.zeros(n, np.float32)
+    fp = lambda a: a.ctypes.data_as(ff.lib.fft_execute.argtypes[1])
+    assert ff.lib.fft_execute(p.handle, fp(pcm), fp(re), fp(im), fp(mg), n) == 1
+    X = fp64.fft64(pcm.astype(np.float64))
+    assert np.max(np.abs((re + 1j * im) - X)) <= 1e-4 * max(1.0, np.max(np.abs(X))) or \
+        fp64.rel_l2(re + 1j * im, X) <= fp64.tolerance(n)
+    assert fp64.rel_l2(mg, np.abs(X)) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("case", ["short_pad", "short_refuse", "max_frames", "w_lt_n", "odd_hop", "no_window",
+                                  "unaligned"])
+def test_stft_edge_cases(ff, case):
+    n = 1024
+    if case == "short_pad":
+        pcm = noise_plus_sines(300, seed=1)
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 1024, 256, pad_mode=1)
+        assert got == 1
+    elif case == "short_refuse":
+        pcm = noise_plus_sines(300, seed=1)
+        p = ff.make_plan(n, window=1024, hop=256)
+        assert ff.stft_frames(p, pcm, pad_mode=2)[0] == 0
+        return
+    elif case == "max_frames":
+        pcm = noise_plus_sines(50000, seed=2)
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 1024, 256, max_frames=17)
+        assert got == 17
+    elif case == "w_lt_n":
+        pcm = noise_plus_sines(20000, seed=3)
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 600, 150)
+    elif case == "odd_hop":
+        pcm = noise_plus_sines(20000, seed=4)
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 1024, 257)
+    elif case == "no_window":
+        pcm = noise_plus_sines(20000, seed=5)
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 1024, 256, windows=0)
+    else:  # unaligned: the PCM view starts one float past an 8-byte boundary
+        big = noise_plus_sines(20001, seed=6)
+        pcm = big[1:]
+        p, got, (re, im, mg), win = stft_case(ff, pcm, n, 1024, 256)
+    W = p.window
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, p.hop, got, win, False, 1, False, False, True)
+    assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= fp64.tolerance(n)
+    assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
+
+
+def test_config2_full_size_sampled_frames(ff):
+    """Batched R2C STFT, Hann 1024, hop 256, half + magnitude, 2^24 samples: frame count
+    exact, and 256 random frames checked against the FP64 oracle."""
+    import torch
+    n, W, H = 1024, 1024, 256
+    pcm = noise_plus_sines(1 << 24, seed=2)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=True, half_spectrum=True,
+                     apply_windows=True, analysis_window=1, pad_mode=0)
+    frames = ff.frame_count(p, pcm.size, 0)
+    assert frames == 65533
+    d_pcm = torch.from_numpy(pcm).cuda()
+    out = torch.empty((3, frames, n // 2 + 1), device="cuda", dtype=torch.float32)
+    assert ff.stft_device(p, d_pcm, out[0], out[1], out[2], pad_mode=0) == frames
+    torch.cuda.synchronize()
+    re, im, mg = (out[i].cpu().numpy() for i in range(3))
+    rng = np.random.default_rng(0)
+    pick = np.sort(rng.choice(frames, 256, replace=False))
+    win = fp64.build_window(1, W)
+    x = np.stack([pcm[f * H:f * H + W] * win for f in pick]).astype(np.float64)
+    want = np.abs(np.fft.rfft(x, axis=-1))
+    assert fp64.rel_l2(mg[pick], want) <= fp64.tolerance(n)
+    assert np.array_equal(re, mg) and not im.any()
+    # the device entry and the host entry produce the same bits
+    got, hre, him, hmg = ff.stft_frames(p, pcm[: 1 << 20], pad_mode=0)
+    assert np.array_equal(hmg, mg[:got])
+
+
+def test_device_entry_matches_host_entry_bitwise(ff):
+    import torch
+    n = 2048
+    pcm = noise_plus_sines(100000, seed=9)
+    p = ff.make_plan(n, window=2048, hop=512, transform=1, store_polar=True, half_spectrum=True,
+                     apply_windows=True, analysis_window=1)
+    got, re, im, mg = ff.stft_frames(p, pcm, pad_mode=1)
+    d = torch.from_numpy(pcm).cuda()
+    out = torch.zeros((3, got, n // 2 + 1), device="cuda")
+    assert ff.stft_device(p, d, out[0], out[1], None, pad_mode=1) == got
+    torch.cuda.synchronize()
+    assert np.array_equal(out[0].cpu().numpy(), re)
+    assert np.array_equal(out[1].cpu().numpy(), im)
+    assert not out[2].any()  # the skipped plane was not written
+
+
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_complex_batched_inverse_roundtrip(ff, kernel):
+    """fft_execute_complex_batched on an inverse C2R half-spectrum plan reproduces the
+    framed signal (Hermitian rebuild, fft_cffi.cpp:1062-1098) and matches the restated
+    reference path."""
+    n, H = 512, 128
+    pcm = noise_plus_sines(10 * n, seed=11)
+    fwd = ff.make_plan(n, window=n, hop=H, transform=1, half_spectrum=True)
+    inv = ff.make_plan(n, window=n, hop=H, inverse=1, kernel=kernel, transform=2, half_spectrum=True)
+    got, re, im, _ = ff.stft_frames(fwd, pcm, pad_mode=1)
+    out = np.zeros(got * n, np.float32)
+    fp = lambda a: a.ctypes.data_as(ff.lib.fft_execute.argtypes[1])
+    assert ff.lib.fft_execute_complex_batched(inv.handle, fp(re), fp(im), got, fp(out), 1, 0, 0) == got
+    frames = np.stack([pcm[f * H:f * H + n] for f in range(got)])
+    assert fp64.rel_l2(out.reshape(got, n), frames) <= 2 * fp64.tolerance(n)
+    rgot, rout = ref.execute_complex_batched(re, im, n, inverse=True, kernel=kernel, transform=2, half_spectrum=True)
+    assert rgot == got and fp64.rel_l2(out.reshape(got, n), rout) <= 2 * fp64.tolerance(n)

@@ -1,0 +1,235 @@
+"""Pins of the oracle itself (CPU only).
+
+Each check is chosen so a plausible mistake in the oracle fails it:
+* the FP64 FFT is checked against a direct O(N^2) DFT (no shared code path);
+* the float restatement of the reference (cpu_ref) is checked against the FP64 oracle
+  for both kernels and both directions, and on the reference's own test shapes;
+* the integer metadata closed forms are checked against the maps cpu_ref records
+  inside its butterfly loops, against the N=8 values written out by hand, and by
+  executing the butterfly network they describe, which must reproduce the DFT;
+* windows are compared bit-for-bit between the two restatements and against scipy;
+* the frame count is compared with a brute-force enumeration.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.signal
+
+from oracle import fp64, index_maps as M, ref
+
+POW2 = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+
+
+def cnormal(b, n, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((b, n)) + 1j * rng.standard_normal((b, n))).astype(np.complex64)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 64, 256, 1024])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_fp64_fft_matches_direct_dft(n, inverse):
+    x = cnormal(3, n, 7).astype(np.complex128)
+    a = fp64.fft64(x, inverse)
+    b = fp64.dft_direct(x, inverse)
+    assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
+
+
+def test_fp64_closed_forms():
+    # impulse -> constant, constant -> impulse, single tone -> one bin
+    n = 64
+    e = np.zeros(n); e[0] = 1
+    assert np.allclose(fp64.fft64(e), np.ones(n))
+    assert np.allclose(fp64.fft64(np.ones(n)), n * e)
+    k0 = 5
+    tone = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    X = fp64.fft64(tone)
+    assert abs(X[k0] - n) < 1e-9 and np.max(np.abs(np.delete(X, k0))) < 1e-9
+    # Parseval
+    x = cnormal(1, n, 3)[0].astype(np.complex128)
+    assert math.isclose(np.sum(np.abs(fp64.fft64(x)) ** 2), n * np.sum(np.abs(x) ** 2), rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("n", POW2)
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_cpu_ref_fft_within_tolerance(n, kernel, inverse):
+    x = cnormal(5, n, 11 + n)
+    y = ref.fft_batched(x, inverse=inverse, kernel=kernel)
+    assert fp64.rel_l2(y, fp64.fft64(x, inverse)) <= fp64.tolerance(n)
+
+
+def test_cpu_ref_kernel_codes_ct_vs_stockham_equal_up_to_rounding():
+    # C-level codes 0 and 1 both run Cooley-Tukey (fft_cffi.cpp:799-800): identical bits.
+    x = cnormal(4, 256, 5)
+    assert np.array_equal(ref.fft_batched(x, kernel=0), ref.fft_batched(x, kernel=1))
+    assert fp64.rel_l2(ref.fft_batched(x, kernel=2), ref.fft_batched(x, kernel=0)) < 1e-6
+
+
+def test_reference_roundtrip_small_batches():
+    # fftfree_tests.cpp:177-220: N=16, B=8 forward then inverse, tolerance 1e-5.
+    x = cnormal(8, 16, 42)
+    y = ref.fft_batched(ref.fft_batched(x), inverse=True)
+    assert np.max(np.abs(y - x)) < 1e-5
+
+
+def test_reference_stockham_parallel_large_batch_real_inputs():
+    # fftfree_tests.cpp:719-844: N=256, B=128, real-only inputs, CT and Stockham agree
+    # with an exact DFT within 1e-5 * 256.
+    rng = np.random.default_rng(1337)
+    x = rng.standard_normal((128, 256)).astype(np.float32).astype(np.complex64)
+    want = fp64.fft64(x)
+    for k in (1, 2):
+        assert np.max(np.abs(ref.fft_batched(x, kernel=k) - want)) < 1e-5 * 256
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32])
+@pytest.mark.parametrize("frames", [1, 3, 16])
+def test_reference_extended_consensus(n, frames):
+    # fftfree_tests.cpp:1089-1271: CT vs Stockham vs exact DFT, seed N*1009+frames*97.
+    x = cnormal(frames, n, n * 1009 + frames * 97)
+    want = fp64.fft64(x)
+    for k in (0, 2):
+        assert fp64.rel_l2(ref.fft_batched(x, kernel=k), want) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 64, 256, 1024])
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_index_maps_match_recorded_metadata(n, kernel):
+    S = M.log2_exact(n)
+    ti = ref.metadata(n, M.KIND_TW_INDEX, kernel).real.astype(np.int64).reshape(S, n)
+    assert np.array_equal(ti, M.twiddle_index_map(n, kernel))
+    bp = ref.metadata(n, M.KIND_PAIRS, kernel)
+    bp = np.stack([bp.real, bp.imag], -1).astype(np.int64).reshape(S, n, 2)
+    assert np.array_equal(bp, M.butterfly_pairs(n, kernel))
+    sp = ref.metadata(n, M.KIND_STAGE_PARAMS, kernel)
+    assert np.array_equal(np.stack([sp.real, sp.imag], -1).astype(np.int64), M.stage_params(n))
+    lm = ref.metadata(n, M.KIND_LAYOUT, kernel)
+    assert np.array_equal(lm.real.astype(np.int64), M.layout_map(n)) and not lm.imag.any()
+    tw = ref.metadata(n, M.KIND_TWIDDLE, kernel)
+    plan_w = ref.twiddles(n)
+    assert np.array_equal(tw, plan_w[M.twiddle_metadata_index(n)])
+    assert np.array_equal(ref.bitrev(n), M.bitrev(n))
+
+
+def test_index_maps_known_values_n8():
+    # written out by hand from the loops at eigen_fft.hpp:2981-3011 and 3478-3505
+    assert M.twiddle_index_map(8, 2).tolist() == [[0] * 8, [0, 0, 2, 2, 0, 0, 2, 2], [0, 1, 2, 3, 0, 1, 2, 3]]
+    assert M.twiddle_index_map(8, 0).tolist() == [[0] * 8, [0, 2, 0, 2, 0, 2, 0, 2], [0, 1, 2, 3, 0, 1, 2, 3]]
+    assert M.bitrev(8).tolist() == [0, 4, 2, 6, 1, 5, 3, 7]
+    assert M.butterfly_pairs(8, 0)[0].tolist() == [[0, 1], [0, 1], [2, 3], [2, 3], [4, 5], [4, 5], [6, 7], [6, 7]]
+    assert M.butterfly_pairs(8, 2)[0].tolist() == [[0, 4], [1, 5], [2, 6], [3, 7], [0, 4], [1, 5], [2, 6], [3, 7]]
+    assert M.stage_params(8).tolist() == [[2, 4], [4, 2], [8, 1]]
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 32, 256, 1024])
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_index_maps_describe_the_dft(n, kernel, inverse):
+    x = cnormal(2, n, 99).astype(np.complex128)
+    y = M.run_butterfly_program(x, kernel, M.butterfly_pairs(n, kernel), M.twiddle_index_map(n, kernel), inverse)
+    assert np.max(np.abs(y - fp64.fft64(x, inverse))) < 1e-9 * n
+
+
+def test_index_map_check_has_teeth():
+    # a single wrong twiddle index or pair must break the network (plant in the oracle)
+    n, k = 64, 2
+    x = cnormal(1, n, 5).astype(np.complex128)
+    tw = M.twiddle_index_map(n, k).copy()
+    tw[3, 17] = (tw[3, 17] + 1) % (n // 2)
+    y = M.run_butterfly_program(x, k, M.butterfly_pairs(n, k), tw)
+    assert np.max(np.abs(y - fp64.fft64(x))) > 1e-3
+    pairs = M.butterfly_pairs(n, 0).copy()
+    pairs[2, 5, 1] += 1
+    y = M.run_butterfly_program(x, 0, pairs, M.twiddle_index_map(n, 0))
+    assert np.max(np.abs(y - fp64.fft64(x))) > 1e-3
+
+
+def test_omega_float_formula():
+    n = 1024
+    om = M.omega(n)
+    assert om.dtype == np.float32 and om[0] == 0.0
+    assert np.max(np.abs(om.astype(np.float64) - 2 * np.pi * np.arange(n) / n)) < 1e-5
+    assert len(M.omega(n, transform=1, half_spectrum=True)) == n // 2 + 1
+
+
+@pytest.mark.parametrize("kind", range(6))
+@pytest.mark.parametrize("norm", range(3))
+@pytest.mark.parametrize("W", [2, 7, 256, 1024])
+def test_windows_two_restatements_bit_exact(kind, norm, W):
+    a = fp64.build_window(kind, W, 0.25 if kind == 4 else 0.0, 0.0, norm)
+    b = ref.build_window(kind, W, 0.25 if kind == 4 else 0.0, 0.0, norm)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("W", [16, 513, 1024, 4096])
+def test_windows_against_scipy(W):
+    # symmetric Hann/Hamming/Blackman; the reference's Kaiser is a Hann placeholder
+    for kind, sp in [(1, scipy.signal.windows.hann(W, sym=True)), (2, scipy.signal.windows.hamming(W, sym=True)),
+                     (3, scipy.signal.windows.blackman(W, sym=True)), (5, scipy.signal.windows.hann(W, sym=True))]:
+        w = fp64.build_window(kind, W)
+        assert np.max(np.abs(w.astype(np.float64) - sp)) < 1e-6
+    tk = fp64.build_window(4, W, 0.5)
+    assert np.max(np.abs(tk.astype(np.float64) - scipy.signal.windows.tukey(W, 0.5, sym=True))) < 1e-6
+    l2 = fp64.build_window(1, W, norm=1)
+    assert abs(float(np.sum(l2.astype(np.float64) ** 2)) - 1.0) < 1e-5
+    area = fp64.build_window(1, W, norm=2)
+    assert abs(float(np.sum(area.astype(np.float64))) - 1.0) < 1e-5
+
+
+def brute_frames(L, N, W, H, pad, max_frames):
+    W = W or N
+    H = H or W
+    if W > N or L == 0:
+        return 0
+    if L < W:
+        f = 1 if pad == 1 else 0
+    else:
+        f = sum(1 for s in range(0, L, H) if s + W <= L)
+    if max_frames and f > max_frames:
+        f = max_frames
+    return f
+
+
+def test_frame_count_rule():
+    cases = 0
+    for L in [0, 1, 5, 255, 256, 257, 1000, 1024, 1025, 4096, 5000]:
+        for N, W, H in [(256, 0, 0), (256, 256, 64), (512, 256, 100), (256, 512, 64), (1024, 1024, 256), (8, 3, 5)]:
+            for pad in (0, 1, 2):
+                for mx in (0, 1, 3):
+                    want = brute_frames(L, N, W, H, pad, mx)
+                    assert fp64.frame_count(L, N, W, H, 0, pad, mx) == want
+                    assert ref.frame_count(L, N, W, H, pad, 0, mx) == want  # pad via the context
+                    cases += 1
+    assert cases > 500
+    # headline workload shapes
+    assert fp64.frame_count(1 << 24, 1024, 1024, 256, 0, 0) == 65533
+    assert fp64.frame_count(1 << 18, 2048, 2048, 512, 0, 0) == 509
+
+
+@pytest.mark.parametrize("transform", [0, 1, 2, 3])
+@pytest.mark.parametrize("flags", [(0, 0, 0), (1, 0, 1), (0, 1, 1), (1, 1, 0), (0, 0, 1)])
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_cpu_ref_stft_planes_vs_fp64(transform, flags, kernel, inverse):
+    reduce_mag, polar, half = flags
+    if transform == 2 and half and kernel != 2:
+        pytest.skip("Cooley-Tukey C2R half rebuilds the input spectrum; covered separately")
+    N, W, H = 64, 48, 16
+    rng = np.random.default_rng(3)
+    pcm = rng.standard_normal(500).astype(np.float32)
+    win = fp64.build_window(1, W)
+    taps = np.ones(N, np.float32); taps[:W] = win
+    got, re, im, mg = ref.execute_batched(pcm, N, W, H, ctx_pad=1, pad_mode=0, inverse=inverse, kernel=kernel,
+                                          transform=transform, reduce_magnitude=reduce_mag, store_polar=polar,
+                                          half_spectrum=half, window_taps=taps)
+    frames = fp64.frame_count(pcm.size, N, W, H, 1, 0)
+    assert got == frames
+    ore, oim, omg = fp64.stft_planes(pcm, N, W, H, frames, win, inverse, transform, reduce_mag, polar, half)
+    tol = 4 * fp64.tolerance(N)
+    if polar:
+        z, oz = re * np.exp(1j * im), ore * np.exp(1j * oim)
+        assert fp64.rel_l2(z, oz) <= tol
+    else:
+        assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= tol
+    assert fp64.rel_l2(mg, omg) <= tol
