@@ -112,7 +112,8 @@ def main():
         frames = ff.frame_count(p, L, 0)
         bins = n // 2 + 1
         d_pcm = torch.from_numpy(pcm).cuda()
-        out = torch.empty((3, frames, bins), device="cuda")
+        # three separate plane allocations (each 16-byte aligned, as a caller's buffers are)
+        out = [torch.empty((frames, bins), device="cuda") for _ in range(3)]
 
         def step():
             ff.stft_device(p, d_pcm, out[0], out[1], out[2], pad_mode=0)

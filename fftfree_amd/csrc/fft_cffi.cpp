@@ -92,6 +92,7 @@ struct FftContext {
   hipStream_t stream = nullptr;
   StreamDispatcher dispatcher;
   DeviceBuffer d_awin, d_swin, d_pcm, d_planes, d_cplx;
+  DeviceBuffer d_sink;  // N floats that kernels write skipped output planes to
   bool windows_dirty = true;
 
   ~FftContext() {
@@ -100,6 +101,7 @@ struct FftContext {
     d_pcm.release();
     d_planes.release();
     d_cplx.release();
+    d_sink.release();
     plan.reset();
     cache.clear();
     if (stream) {
@@ -220,6 +222,7 @@ void ensure_gpu(FftContext* c) {
   hip_check(hipSetDevice(c->device), "hipSetDevice");
   c->plan->ensure_device();
   c->plan->set_dispatcher(&c->dispatcher);
+  c->d_sink.reserve(std::max<size_t>(64, (size_t)c->N) * sizeof(float));
   if (c->windows_dirty) {
     c->d_awin.reserve(std::max<size_t>(1, c->analysis_win.size()) * sizeof(float));
     c->d_swin.reserve(std::max<size_t>(1, c->synthesis_win.size()) * sizeof(float));
@@ -286,6 +289,7 @@ StftArgs stft_args(FftContext* c, const float* d_pcm, size_t pcm_len) {
   a.bins = (int)bins_of(c);
   a.out_mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : (c->transform == FFT_TRANSFORM_C2R ? 1 : 0);
   a.scale = c->inverse ? 1.0f / (float)c->N : 1.0f;
+  a.sink = c->d_sink.as<float>();
   return a;
 }
 

@@ -129,84 +129,94 @@ constexpr int product() {
   return (1 * ... * Rs);
 }
 
-// Twiddle + R-point DFT of the E/R butterflies this thread owns in one Stockham radix-R
-// pass over a transform of length L held by T = L/E threads. `v` holds the inputs in
-// butterfly order: v[q*R + r] = src[b_q + r*L/R] with b_q = t + q*T. `tw` is a forward
-// table of exp(-2*pi*i*m/(L*tw_stride)).
-template <int L, int E, int R, int DIR>
-__device__ __forceinline__ void pass_compute(float2* v, int t, int Ns,
-                                             const float2* __restrict__ tw, int tw_stride) {
-  constexpr int T = L / E;
+// LDS layout: one float2 of padding per 16 keeps the autosort scatter stores (stride R
+// per lane) from hitting the same banks; index i lives at pidx(i).
+constexpr int kPadShift = 4;
+__device__ __forceinline__ int pidx(int i) { return i + (i >> kPadShift); }
+template <int L>
+constexpr int padded_len() {
+  return L + (L >> kPadShift);
+}
+
+// The passes of one transform of length L held by T = L/E threads, E values per thread.
+// Pass P has radix R and input span Ns; butterfly b = t + q*T (q < E/R) owns elements
+// v[q*R + r] = src[b + r*L/R]. Twiddles exp(DIR 2 pi i (b mod Ns) r / (Ns R)) come from
+// registers preloaded by load_twiddles (slot P*E + q*(R-1) + r-1), so every global
+// load of a kernel can be issued up front.
+template <int L, int E, int DIR, int Ns, int P, int R, int... Rest>
+struct Stockham {
+  static constexpr int T = L / E;
+  static constexpr bool kLast = sizeof...(Rest) == 0;
+
+  // tw is a forward table exp(-2 pi i m / (L*stride)).
+  __device__ static __forceinline__ void load_twiddles(float2* twr, int t, const float2* __restrict__ tw,
+                                                       int stride) {
+    if constexpr (Ns > 1) {
 #pragma unroll
-  for (int q = 0; q < E / R; ++q) {
-    const int b = t + q * T;
-    const int k = b & (Ns - 1);
-    float2* u = v + q * R;
-    if (Ns > 1) {
-      // element r turns by k*r*(L/(Ns*R)) units of 2*pi/L
-      const int base = k * (L / R / Ns) * tw_stride;
+      for (int q = 0; q < E / R; ++q) {
+        const int b = t + q * T;
+        const int base = (b & (Ns - 1)) * (L / R / Ns) * stride;
 #pragma unroll
-      for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], load_tw<DIR>(tw, base * r));
+        for (int r = 1; r < R; ++r) twr[P * E + q * (R - 1) + r - 1] = load_tw<DIR>(tw, base * r);
+      }
     }
-    reg_dft<R, DIR>(u);
+    if constexpr (!kLast) Stockham<L, E, DIR, Ns * R, P + 1, Rest...>::load_twiddles(twr, t, tw, stride);
   }
-}
 
-// Autosort scatter of a computed pass: dst[(b - b mod Ns)*R + b mod Ns + r*Ns].
-template <int L, int E, int R>
-__device__ __forceinline__ void pass_store(const float2* v, int t, int Ns, float2* dst) {
-  constexpr int T = L / E;
+  // Runs this pass and the rest. Between passes data goes through the LDS ping-pong
+  // pair (a, b) (padded layout), one barrier each. On return v holds the result in
+  // last-pass order v[q*Rl + r] = X[t + q*T + r*L/Rl] (consecutive threads own
+  // consecutive outputs) and `free_buf` is the buffer nobody reads any more.
+  // INPLACE: a == b, one LDS buffer; a second barrier after each gather keeps the next
+  // scatter from overwriting values other threads have not read yet.
+  template <bool INPLACE = false>
+  __device__ static __forceinline__ void run(float2* v, int t, float2* a, float2* b, const float2* twr,
+                                             float2*& free_buf) {
 #pragma unroll
-  for (int q = 0; q < E / R; ++q) {
-    const int b = t + q * T;
-    const int k = b & (Ns - 1);
-    const int o = (b - k) * R + k;
+    for (int q = 0; q < E / R; ++q) {
+      float2* u = v + q * R;
+      if constexpr (Ns > 1) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) dst[o + r * Ns] = v[q * R + r];
+        for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], twr[P * E + q * (R - 1) + r - 1]);
+      }
+      reg_dft<R, DIR>(u);
+    }
+    if constexpr (kLast) {
+      free_buf = a;
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / R; ++q) {
+        const int bb = t + q * T;
+        const int k = bb & (Ns - 1);
+        const int o = (bb - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[pidx(o + r * Ns)] = v[q * R + r];
+      }
+      __syncthreads();
+      constexpr int Rn = FirstOf<Rest...>::value;
+#pragma unroll
+      for (int q = 0; q < E / Rn; ++q) {
+        const int bb = t + q * T;
+#pragma unroll
+        for (int r = 0; r < Rn; ++r) v[q * Rn + r] = a[pidx(bb + r * (L / Rn))];
+      }
+      if constexpr (INPLACE) __syncthreads();
+      Stockham<L, E, DIR, Ns * R, P + 1, Rest...>::template run<INPLACE>(v, t, b, a, twr, free_buf);
+    }
   }
-}
+};
 
-template <int L, int E, int R>
-__device__ __forceinline__ void pass_load(float2* v, int t, const float2* src) {
-  constexpr int T = L / E;
-#pragma unroll
-  for (int q = 0; q < E / R; ++q) {
-    const int b = t + q * T;
-#pragma unroll
-    for (int r = 0; r < R; ++r) v[q * R + r] = src[b + r * (L / R)];
-  }
-}
+template <int L, int E, int DIR, int... Rs>
+using StockhamPlan = Stockham<L, E, DIR, 1, 0, Rs...>;
 
-// Runs the passes Rs on registers `v` (the first pass's inputs are already loaded in
-// butterfly order). Between passes data goes through the LDS ping-pong pair (a, b), one
-// barrier each. On return v holds the result in last-pass order:
-//   v[q*Rl + r] = X[b_q + r*L/Rl],  b_q = t + q*T,  Rl = last radix,
-// i.e. consecutive threads own consecutive outputs. `free_buf` names the LDS buffer
-// that nobody reads any more (safe to write before the next barrier).
-template <int L, int E, int DIR, int Ns, int R, int... Rest>
-__device__ __forceinline__ void run_passes(float2* v, int t, float2* a, float2* b,
-                                           const float2* __restrict__ tw, int tw_stride,
-                                           float2*& free_buf) {
-  pass_compute<L, E, R, DIR>(v, t, Ns, tw, tw_stride);
-  if constexpr (sizeof...(Rest) == 0) {
-    free_buf = a;
-  } else {
-    pass_store<L, E, R>(v, t, Ns, a);
-    __syncthreads();
-    constexpr int Rn = FirstOf<Rest...>::value;
-    pass_load<L, E, Rn>(v, t, a);
-    run_passes<L, E, DIR, Ns * R, Rest...>(v, t, b, a, tw, tw_stride, free_buf);
-  }
-}
-
-// Stores last-pass-ordered registers to LDS in natural order (buffer of length L).
+// Stores last-pass-ordered registers to LDS in natural order (padded layout).
 template <int L, int E, int Rl>
 __device__ __forceinline__ void store_natural(const float2* v, int t, float2* dst) {
   constexpr int T = L / E;
 #pragma unroll
   for (int q = 0; q < E / Rl; ++q) {
 #pragma unroll
-    for (int r = 0; r < Rl; ++r) dst[t + q * T + r * (L / Rl)] = v[q * Rl + r];
+    for (int r = 0; r < Rl; ++r) dst[pidx(t + q * T + r * (L / Rl))] = v[q * Rl + r];
   }
 }
 

@@ -31,36 +31,142 @@ __device__ __forceinline__ float frame_sample(const float* __restrict__ src, con
   return win ? x * win[m] : x;
 }
 
-template <int L, int E, int FPW, int DIR, int... Rs>
-__global__ void __launch_bounds__(FPW*(L / E)) stft_real_kernel(StftArgs a) {
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The pair (A, B) staged for one bin of the post-transformed value, per output mode:
+// complex (re, im), C2R-clear (re, -), magnitude (|z|, -), polar (|z|, arg z).
+// |z| uses the hardware square root (within 1 ulp).
+template <int MODE>
+__device__ __forceinline__ void stage_vals(float2 X, float& A, float& B) {
+  if constexpr (MODE == kOutComplex || MODE == kOutC2RClear) {
+    A = X.x;
+    B = X.y;
+  } else {
+    A = __builtin_amdgcn_sqrtf(__builtin_fmaf(X.x, X.x, X.y * X.y));
+    B = (MODE == kOutPolar) ? atan2f(X.y, X.x) : 0.0f;
+  }
+}
+
+template <int MODE>
+constexpr bool mode_has_b() {
+  return MODE == kOutComplex || MODE == kOutPolar;
+}
+
+// Plane WHICH (0 re, 1 im, 2 mag) from the staged pair: the output copy of
+// fft_cffi.cpp:811-825 (mag plane = |v|, or re when polar) applied to the value.
+template <int MODE, int WHICH>
+__device__ __forceinline__ float plane_value(float A, float B) {
+  if constexpr (WHICH == 0) return A;
+  if constexpr (WHICH == 1) return mode_has_b<MODE>() ? B : 0.0f;
+  if constexpr (MODE == kOutComplex) return __builtin_amdgcn_sqrtf(__builtin_fmaf(A, A, B * B));
+  if constexpr (MODE == kOutC2RClear) return __builtin_fabsf(A);
+  return A;
+}
+
+// Cooperative store of one contiguous run of `count` plane values (frame-major,
+// `bins` per frame) from LDS staging sA/sB laid out [frame][0..L]. For a full spectrum
+// (bins = 2L) bins above L read their mirror N-k, conjugated (B negated).
+template <int L, bool HALF, int MODE, int WHICH>
+__device__ __forceinline__ void store_plane_run(float* __restrict__ dst, const float* sA, const float* sB,
+                                                int count, int tid, int nthreads) {
+  if (!dst) return;
+  constexpr int BINS = HALF ? L + 1 : 2 * L;
+  constexpr bool kB = WHICH != 0 && mode_has_b<MODE>();
+  constexpr bool kZero = WHICH == 1 && !mode_has_b<MODE>();
+  auto value = [&](int i) -> float {
+    if constexpr (kZero) {
+      return 0.0f;
+    } else if constexpr (HALF) {
+      return plane_value<MODE, WHICH>(sA[i], kB ? sB[i] : 0.0f);
+    } else {
+      const int f = i / BINS;
+      const int k = i - f * BINS;
+      const int j = f * (L + 1) + (k <= L ? k : 2 * L - k);
+      const float b = kB ? (k <= L ? sB[j] : -sB[j]) : 0.0f;
+      return plane_value<MODE, WHICH>(sA[j], b);
+    }
+  };
+  if ((((uintptr_t)dst) & 15) == 0) {
+    const int n4 = count >> 2;
+    for (int i = tid; i < n4; i += nthreads) {
+      f32x4 o;
+      if constexpr (kZero) {
+        o = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      } else if constexpr (HALF) {
+        const f32x4 a4 = reinterpret_cast<const f32x4*>(sA)[i];
+        f32x4 b4 = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (kB) b4 = reinterpret_cast<const f32x4*>(sB)[i];
+        o.x = plane_value<MODE, WHICH>(a4.x, b4.x);
+        o.y = plane_value<MODE, WHICH>(a4.y, b4.y);
+        o.z = plane_value<MODE, WHICH>(a4.z, b4.z);
+        o.w = plane_value<MODE, WHICH>(a4.w, b4.w);
+      } else {
+        o.x = value(4 * i);
+        o.y = value(4 * i + 1);
+        o.z = value(4 * i + 2);
+        o.w = value(4 * i + 3);
+      }
+      reinterpret_cast<f32x4*>(dst)[i] = o;
+    }
+    for (int i = (n4 << 2) + tid; i < count; i += nthreads) dst[i] = value(i);
+  } else {
+    for (int i = tid; i < count; i += nthreads) dst[i] = value(i);
+  }
+}
+
+// Fused real-input STFT. One workgroup owns FPW consecutive frames (FPW*T threads).
+//  1. every global load is issued up front: the frame's samples times the window
+//     (z[n] = x[2n] + i x[2n+1]: the real DFT of size N = 2L is computed as a complex
+//     DFT of size L), the pass twiddles and the split twiddles, all into registers;
+//  2. Stockham passes through padded LDS (ping-pong pair, or one buffer when IP);
+//  3. split, two bins per thread step: with e = (Z_p + conj Z_{L-p})/2 and
+//     o = -i (Z_p - conj Z_{L-p})/2, X_p = e + w^p o and X_{L-p} = conj(e - w^p o);
+//     inverse plans scale by 1/N; bins 0 and L are real (imag +0);
+//  4. post-transform values are staged in LDS [frame][bin] and each output plane is
+//     written as one contiguous run of FPW*bins floats (16-byte stores when aligned).
+// Stores are the last memory operations of every wave: on CDNA the vector-memory
+// counter also counts stores, so a load issued after a store would wait for it.
+template <int L, int E, int FPW, int IP, int MODE, int HALF, int DIR, int... Rs>
+__global__ void __launch_bounds__(FPW*(L / E)) stft_r2c_kernel(StftArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
+  constexpr int NT = FPW * T;
+  constexpr int NP = sizeof...(Rs);
   constexpr int R0 = FirstOf<Rs...>::value;
   constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
+  constexpr int NBUF = IP ? 1 : 2;
+  constexpr int J = (L / 2 + T) / T;  // split steps: pairs p = t + j*T, p <= L/2
+  constexpr int STAGE = (mode_has_b<MODE>() ? 2 : 1) * FPW * (L + 1);  // [frame][0..L]
+  // staging goes to the ping-pong buffer not holding Z, or (in place) over the one buffer
+  constexpr int STAGE_ROOM = IP ? 2 * FPW * LP : 2 * FPW * LP;
+  constexpr bool kStageInLds = STAGE <= STAGE_ROOM;
   static_assert(product<Rs...>() == L, "radices must multiply to L");
-  __shared__ float2 lds[2][FPW][L];
+  static_assert((T & (T - 1)) == 0, "threads per transform must be a power of two");
+  __shared__ float2 lds[NBUF][FPW][LP];
+  __shared__ float stage_extra[kStageInLds ? 1 : STAGE];
 
-  const int tr = threadIdx.x / T;
-  const int t = threadIdx.x - tr * T;
+  const int tid = threadIdx.x;
+  const int tr = tid / T;
+  const int t = tid & (T - 1);
   const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
   const size_t f = tile * FPW + tr;
-  const bool active = f < a.frames;
   const size_t start = (f + a.frame_offset) * (size_t)a.hop;
   size_t cnt = 0;
-  if (active && start < a.pcm_len) {
+  if (f < a.frames && start < a.pcm_len) {
     const size_t rem = a.pcm_len - start;
     cnt = rem < (size_t)a.window ? rem : (size_t)a.window;
   }
-  const float* __restrict__ src = a.pcm + (active ? start : 0);
+  const float* __restrict__ src = a.pcm + (cnt ? start : 0);
   const float* __restrict__ win = a.win;
 
-  // z[n] = x[2n] + i x[2n+1], loaded in first-pass butterfly order.
+  // 1. loads
   float2 v[E];
   if (cnt == (size_t)N && ((((uintptr_t)src) & 7) == 0)) {
     const float2* __restrict__ s2 = reinterpret_cast<const float2*>(src);
     const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win);
 #pragma unroll
-    for (int q = 0; q < E / R0; ++q) {
+    for (int q = 0; q < E / R0; ++q)
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
         const int n = t + q * T + r * (L / R0);
@@ -72,42 +178,346 @@ __global__ void __launch_bounds__(FPW*(L / E)) stft_real_kernel(StftArgs a) {
         }
         v[q * R0 + r] = x;
       }
-    }
   } else {
 #pragma unroll
-    for (int q = 0; q < E / R0; ++q) {
+    for (int q = 0; q < E / R0; ++q)
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
         const int n = t + q * T + r * (L / R0);
         v[q * R0 + r] = make_float2(frame_sample(src, win, cnt, 2 * n), frame_sample(src, win, cnt, 2 * n + 1));
       }
-    }
+  }
+  float2 twr[E * NP];
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, 2);
+  float2 wsp[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int p = t + j * T;
+    wsp[j] = load_tw<DIR>(a.tw, p <= L / 2 ? p : 0);
   }
 
-  float2* free_buf;
-  run_passes<L, E, DIR, 1, Rs...>(v, t, &lds[0][tr][0], &lds[1][tr][0], a.tw, 2, free_buf);
-  store_natural<L, E, Rl>(v, t, free_buf);
+  // 2. passes
+  float2* Zb;
+  float2* buf0 = &lds[0][tr][0];
+  float2* buf1 = &lds[NBUF - 1][tr][0];
+  StockhamPlan<L, E, DIR, Rs...>::template run<IP != 0>(v, t, buf0, buf1, twr, Zb);
+  store_natural<L, E, Rl>(v, t, Zb);
   __syncthreads();
 
-  // Split: E_k = (Z_k + conj Z_{L-k})/2, O_k = -i (Z_k - conj Z_{L-k})/2,
-  // X_k = E_k + w_N^k O_k for k = 0..L; bins above L (full-spectrum output) are
-  // conj(X_{N-k}) because the input is real.
-  const float2* Z = free_buf;
-  const size_t obase = f * (size_t)a.bins;
-  for (int k = t; k < a.bins; k += T) {
-    const int kk = k <= L ? k : N - k;
-    const float2 zk = Z[kk & (L - 1)];
-    const float2 zc = c_conj(Z[(L - kk) & (L - 1)]);
-    const float2 e = c_scale(c_add(zk, zc), 0.5f);
-    const float2 d = c_sub(zk, zc);
-    const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
-    const float2 w = kk < L ? load_tw<DIR>(a.tw, kk) : make_float2(-1.0f, 0.0f);
-    float2 X = c_add(e, c_mul(w, o));
-    X = c_scale(X, a.scale);
-    if (kk == 0 || kk == L) X.y = 0.0f;  // exact for real input; R2C half rule
-    if (k > L) X = c_conj(X);
-    if (active) store_planes(X, a.out_mode, obase + k, a.o_re, a.o_im, a.o_mag);
+  // 3. split + post-transform values into registers
+  constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
+  float Av[2 * J], Bv[2 * J];
+  const float2* Z = Zb;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int p = t + j * T;
+    const float2 zp = Z[pidx(p & (L - 1))];
+    const float2 zc = c_conj(Z[pidx((L - p) & (L - 1))]);
+    const float2 e = c_scale(c_add(zp, zc), 0.5f);
+    const float2 d = c_sub(zp, zc);
+    const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
+    float2 X1 = c_add(e, wo);
+    float2 X2 = c_conj(c_sub(e, wo));
+    if (DIR > 0) {
+      X1 = c_scale(X1, scale);
+      X2 = c_scale(X2, scale);
+    }
+    if (p == 0) {  // bins 0 and L
+      X1.y = 0.0f;
+      X2.y = 0.0f;
+    }
+    stage_vals<MODE>(X1, Av[2 * j], Bv[2 * j]);
+    stage_vals<MODE>(X2, Av[2 * j + 1], Bv[2 * j + 1]);
   }
+  float* sA = kStageInLds ? reinterpret_cast<float*>(&lds[NP & (NBUF - 1) ? 1 : 0][0][0]) : stage_extra;
+  if constexpr (IP) {
+    if (kStageInLds) __syncthreads();  // Z is read; the one buffer becomes staging
+    sA = kStageInLds ? reinterpret_cast<float*>(&lds[0][0][0]) : stage_extra;
+  }
+  float* sB = sA + FPW * (L + 1);
+  float* rowA = sA + tr * (L + 1);
+  float* rowB = sB + tr * (L + 1);
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int p = t + j * T;
+    if (p <= L / 2) {
+      rowA[p] = Av[2 * j];
+      rowA[L - p] = Av[2 * j + 1];
+      if constexpr (mode_has_b<MODE>()) {
+        rowB[p] = Bv[2 * j];
+        rowB[L - p] = Bv[2 * j + 1];
+      }
+    }
+  }
+  __syncthreads();
+
+  // 4. plane stores
+  const size_t f0 = tile * FPW;
+  if (f0 < a.frames) {
+    const size_t left = a.frames - f0;
+    constexpr int BINS = HALF ? L + 1 : N;
+    const int count = (int)((left < (size_t)FPW ? left : (size_t)FPW) * BINS);
+    const size_t base = f0 * (size_t)BINS;
+    store_plane_run<L, HALF != 0, MODE, 0>(a.o_re ? a.o_re + base : nullptr, sA, sB, count, tid, NT);
+    store_plane_run<L, HALF != 0, MODE, 1>(a.o_im ? a.o_im + base : nullptr, sA, sB, count, tid, NT);
+    store_plane_run<L, HALF != 0, MODE, 2>(a.o_mag ? a.o_mag + base : nullptr, sA, sB, count, tid, NT);
+  }
+}
+
+// Multi-frame fused real-input STFT for transforms that fit one wave (T = L/E <= 64).
+// A workgroup of NT = 256 threads runs P = NT/T transform slots; slot s transforms the K
+// consecutive frames f0 + s*K + i, i < K, one after another. Per thread:
+//  * the window taps of its elements, every pass twiddle and the split twiddles are
+//    loaded once into registers and reused for all K frames;
+//  * the P frames of one iteration are consecutive; the samples they span are loaded
+//    once with 16-byte loads (one iteration ahead) and staged in LDS;
+//  * at iteration i the P slots hold P consecutive frames; their bins are staged in LDS
+//    and written as one contiguous run per plane with 16-byte stores, the same number
+//    of stores in every wave and iteration, so a wait for prefetched samples never
+//    covers more than the stores of the last few iterations (CDNA's vector-memory
+//    counter covers loads and stores, in issue order, and saturates at 63);
+//  * those stores are issued during the next iteration, between its FFT passes, so each
+//    wave keeps a steady store stream instead of computing and storing in turns.
+// Only whole groups of K*P regular frames come here (window spans N, every frame inside
+// the PCM, hop a multiple of 4 and at most N/4, samples and planes 16-byte aligned);
+// the launcher sends the rest to stft_r2c_kernel.
+// LDS exchanges between passes stay inside one wave, so they are ordered by the wave's
+// in-order LDS queue plus a compiler fence; no workgroup barrier is needed.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L, int E, int DIR, int Ns, int P, int R, int... Rest>
+struct WaveStockham {
+  static constexpr int T = L / E;
+  static constexpr bool kLast = sizeof...(Rest) == 0;
+  // Runs this pass and the rest in one wave; result in last-pass order in v. hook(P) runs
+  // after pass P's arithmetic (used to slot in independent stores).
+  template <class Hook>
+  __device__ static __forceinline__ void run(float2* v, int t, float2* buf, const float2* twr, Hook&& hook) {
+#pragma unroll
+    for (int q = 0; q < E / R; ++q) {
+      float2* u = v + q * R;
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], twr[P * E + q * (R - 1) + r - 1]);
+      }
+      reg_dft<R, DIR>(u);
+    }
+    hook(P);
+    if constexpr (!kLast) {
+#pragma unroll
+      for (int q = 0; q < E / R; ++q) {
+        const int bb = t + q * T;
+        const int k = bb & (Ns - 1);
+        const int o = (bb - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pidx(o + r * Ns)] = v[q * R + r];
+      }
+      wave_lds_fence();
+      constexpr int Rn = FirstOf<Rest...>::value;
+#pragma unroll
+      for (int q = 0; q < E / Rn; ++q) {
+        const int bb = t + q * T;
+#pragma unroll
+        for (int r = 0; r < Rn; ++r) v[q * Rn + r] = buf[pidx(bb + r * (L / Rn))];
+      }
+      wave_lds_fence();
+      WaveStockham<L, E, DIR, Ns * R, P + 1, Rest...>::run(v, t, buf, twr, hook);
+    }
+  }
+};
+
+template <int L, int E, int K, int MODE, int HALF, int DIR, int... Rs>
+__global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
+  constexpr int T = L / E;
+  constexpr int N = 2 * L;
+  constexpr int NT = 256;
+  constexpr int P = NT / T;            // transform slots per workgroup
+  constexpr int NP = sizeof...(Rs);
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
+  constexpr int JF = (L / 2) / T;      // split steps covering p in [0, L/2)
+  constexpr int BINS = HALF ? L + 1 : N;
+  constexpr bool kB = mode_has_b<MODE>();
+  constexpr int ROW = L + 1;           // staged bins per frame
+  constexpr int STAGE = (kB ? 2 : 1) * P * ROW;
+  constexpr int RUN = P * BINS;        // floats per plane per iteration
+  constexpr int N4 = RUN / 4;
+  constexpr int C4 = (N4 + NT - 1) / NT;
+  constexpr int SPAN = (P - 1) * (N / 4) + N;  // samples of P frames, hop <= N/4
+  constexpr int SPAN4 = SPAN / 4;
+  constexpr int SR = (SPAN4 + NT - 1) / NT;    // 16-byte loads per thread per iteration
+  static_assert(T <= 64 && (64 % T) == 0, "one transform per wave or less");
+  static_assert(JF * T == L / 2, "T must divide L/2");
+  static_assert(RUN % 4 == 0 && SPAN % 4 == 0, "runs must be whole 16-byte words");
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  __shared__ float2 lds[P][LP];
+  __shared__ __attribute__((aligned(16))) float stage[2][STAGE];
+  __shared__ __attribute__((aligned(16))) float span[2][SPAN];
+
+  const int tid = threadIdx.x;
+  const int slot = tid / T;
+  const int t = tid & (T - 1);
+  float2* buf = &lds[slot][0];
+  const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
+  const size_t f0 = tile * (size_t)(K * P);
+  const int hop = a.hop;
+  const float* __restrict__ win = a.win;
+  const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win);
+
+  // per-thread constants: window taps, pass twiddles, split twiddles
+  float2 wr[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      wr[q * R0 + r] = win ? w2[t + q * T + r * (L / R0)] : make_float2(1.0f, 1.0f);
+  float2 twr[E * NP];
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, 2);
+  float2 wsp[JF];
+#pragma unroll
+  for (int j = 0; j < JF; ++j) wsp[j] = load_tw<DIR>(a.tw, t + j * T);
+  const float2 wmid = load_tw<DIR>(a.tw, L / 2);
+
+  // iteration it covers frames f0 + it*P .. +P-1, i.e. samples [s, s + (P-1)*hop + N)
+  // of the PCM (the host guarantees 16-byte alignment and that they lie inside it);
+  // SR 16-byte loads per thread, spare lanes re-read the last word
+  const int span4 = ((P - 1) * hop + N) / 4;
+  auto load_span = [&](int it, f32x4* dst) {
+    const f32x4* __restrict__ src =
+        reinterpret_cast<const f32x4*>(a.pcm + (f0 + (size_t)it * P + a.frame_offset) * (size_t)hop);
+#pragma unroll
+    for (int c = 0; c < SR; ++c) {
+      const int idx = tid + c * NT;
+      dst[c] = src[idx < span4 ? idx : span4 - 1];
+    }
+  };
+  auto stash_span = [&](int it, const f32x4* val) {
+#pragma unroll
+    for (int c = 0; c < SR; ++c) {
+      const int idx = tid + c * NT;
+      if (idx < span4) reinterpret_cast<f32x4*>(&span[it & 1][0])[idx] = val[c];
+    }
+  };
+
+  f32x4 nxt[SR];
+  load_span(0, nxt);
+  stash_span(0, nxt);
+  __syncthreads();
+  if (K > 1) load_span(1, nxt);
+  constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
+
+  // Store round c (c < C4) of iteration it's staged run: the P frames of an iteration
+  // are one contiguous, 16-byte-aligned run per plane; spare lanes of the last round
+  // rewrite the run's final word, so every wave issues the same stores.
+  auto store_round = [&](int it, int c) {
+    const float* sA = &stage[it & 1][0];
+    const float* sB = sA + P * ROW;
+    const size_t base = (f0 + (size_t)it * P) * (size_t)BINS;
+    int idx = tid + c * NT;
+    idx = idx < N4 ? idx : N4 - 1;
+    f32x4 A4, B4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (HALF) {
+      A4 = *reinterpret_cast<const f32x4*>(sA + 4 * idx);
+      if constexpr (kB) B4 = *reinterpret_cast<const f32x4*>(sB + 4 * idx);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int g = 4 * idx + u;
+        const int fr = g / BINS;
+        const int k = g - fr * BINS;
+        const int jj = fr * ROW + (k <= L ? k : N - k);
+        A4[u] = sA[jj];
+        if constexpr (kB) B4[u] = k <= L ? sB[jj] : -sB[jj];
+      }
+    }
+    f32x4 o0, o1, o2;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      o0[u] = plane_value<MODE, 0>(A4[u], B4[u]);
+      o1[u] = plane_value<MODE, 1>(A4[u], B4[u]);
+      o2[u] = plane_value<MODE, 2>(A4[u], B4[u]);
+    }
+    // a skipped plane lands in the sink (N floats, never read)
+    const int sidx = idx & (N / 4 - 1);
+    reinterpret_cast<f32x4*>(a.o_re ? a.o_re + base : a.sink)[a.o_re ? idx : sidx] = o0;
+    reinterpret_cast<f32x4*>(a.o_im ? a.o_im + base : a.sink)[a.o_im ? idx : sidx] = o1;
+    reinterpret_cast<f32x4*>(a.o_mag ? a.o_mag + base : a.sink)[a.o_mag ? idx : sidx] = o2;
+  };
+  // rounds of the previous iteration spread over the NP passes of the current one
+  constexpr int PER_PASS = (C4 + NP - 1) / NP;
+
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    // this slot's frame from the staged span, times the window, first-pass order
+    float2 v[E];
+    const float2* fs = reinterpret_cast<const float2*>(&span[i & 1][slot * hop]);
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const int e = q * R0 + r;
+        const float2 x = fs[t + q * T + r * (L / R0)];
+        v[e] = make_float2(x.x * wr[e].x, x.y * wr[e].y);
+      }
+
+    WaveStockham<L, E, DIR, 1, 0, Rs...>::run(v, t, buf, twr, [&](int pass) {
+      if (i > 0) {
+#pragma unroll
+        for (int c = pass * PER_PASS; c < (pass + 1) * PER_PASS && c < C4; ++c) store_round(i - 1, c);
+      }
+    });
+    store_natural<L, E, Rl>(v, t, buf);
+    wave_lds_fence();
+
+    // split into this iteration's staging buffer (double-buffered across iterations)
+    float* rowA = &stage[i & 1][0] + slot * ROW;
+    float* rowB = &stage[i & 1][0] + P * ROW + slot * ROW;
+    auto put = [&](int k, float2 X) {
+      if (DIR > 0) X = c_scale(X, scale);
+      float A, B;
+      stage_vals<MODE>(X, A, B);
+      rowA[k] = A;
+      if constexpr (kB) rowB[k] = B;
+    };
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      const float2 zp = buf[pidx(p)];
+      const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
+      const float2 e = c_scale(c_add(zp, zc), 0.5f);
+      const float2 d = c_sub(zp, zc);
+      const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
+      float2 X1 = c_add(e, wo);
+      float2 X2 = c_conj(c_sub(e, wo));
+      if (p == 0) {  // bins 0 and L are real
+        X1.y = 0.0f;
+        X2.y = 0.0f;
+      }
+      put(p, X1);
+      put(L - p, X2);
+    }
+    {  // bin L/2: every lane computes and writes the same value
+      const float2 zm = buf[pidx(L / 2)];
+      const float2 zc = c_conj(zm);
+      const float2 e = c_scale(c_add(zm, zc), 0.5f);
+      const float2 d = c_sub(zm, zc);
+      put(L / 2, c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x))));
+    }
+    wave_lds_fence();
+    // next iteration's samples: stash what was prefetched, request the one after
+    if (i + 1 < K) {
+      stash_span(i + 1, nxt);
+      if (i + 2 < K) load_span(i + 2, nxt);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < C4; ++c) store_round(K - 1, c);
 }
 
 // Pre-transform hook on load (eigen_fft.hpp:2484-2533): mode 1 clears imag (R2C);
@@ -147,10 +557,12 @@ __device__ __forceinline__ float2 post_op(float2 X, int idx, int N, const C2CArg
 template <int L, int E, int FPW, int DIR, int... Rs>
 __global__ void __launch_bounds__(FPW*(L / E)) c2c_kernel(C2CArgs a) {
   constexpr int T = L / E;
+  constexpr int NP = sizeof...(Rs);
   constexpr int R0 = FirstOf<Rs...>::value;
   constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
   static_assert(product<Rs...>() == L, "radices must multiply to L");
-  __shared__ float2 lds[2][FPW][L];
+  __shared__ float2 lds[2][FPW][LP];
 
   const int tr = threadIdx.x / T;
   const int t = threadIdx.x - tr * T;
@@ -168,8 +580,10 @@ __global__ void __launch_bounds__(FPW*(L / E)) c2c_kernel(C2CArgs a) {
       v[q * R0 + r] = active ? load_pre(col, n, L, a.pre_mode, a.pre_polar) : make_float2(0.0f, 0.0f);
     }
   }
+  float2 twr[E * NP];
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, 1);
   float2* free_buf;
-  run_passes<L, E, DIR, 1, Rs...>(v, t, &lds[0][tr][0], &lds[1][tr][0], a.tw, 1, free_buf);
+  StockhamPlan<L, E, DIR, Rs...>::run(v, t, &lds[0][tr][0], &lds[1][tr][0], twr, free_buf);
   if (active) {
     float2* __restrict__ dst = a.out + f * (size_t)L;
 #pragma unroll
@@ -311,14 +725,80 @@ __global__ void planes_kernel(const float2* in, size_t frames, int n, int bins, 
   if (o_mag) o_mag[gid] = polar ? v.x : __builtin_sqrtf(__builtin_fmaf(v.x, v.x, v.y * v.y));
 }
 
-template <int L, int E, int FPW, int DIR, int... Rs>
+template <int L, int E, int FPW, int IP, int DIR, int... Rs>
 hipError_t stft_launch_impl(const StftArgs& a, hipStream_t s) {
   const size_t grid = (a.frames + FPW - 1) / FPW;
   if (grid == 0) return hipSuccess;
   if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((stft_real_kernel<L, E, FPW, DIR, Rs...>), dim3((unsigned)grid), dim3(FPW * (L / E)), 0, s,
-                     a);
+  const dim3 g((unsigned)grid), blk(FPW * (L / E));
+  const bool half = a.bins == L + 1;
+#define FFTFREE_STFT_GO(MODE)                                                                     \
+  do {                                                                                            \
+    if (half)                                                                                     \
+      hipLaunchKernelGGL((stft_r2c_kernel<L, E, FPW, IP, MODE, 1, DIR, Rs...>), g, blk, 0, s, a); \
+    else                                                                                          \
+      hipLaunchKernelGGL((stft_r2c_kernel<L, E, FPW, IP, MODE, 0, DIR, Rs...>), g, blk, 0, s, a); \
+  } while (0)
+  switch (a.out_mode) {
+    case kOutComplex: FFTFREE_STFT_GO(kOutComplex); break;
+    case kOutC2RClear: FFTFREE_STFT_GO(kOutC2RClear); break;
+    case kOutMagnitude: FFTFREE_STFT_GO(kOutMagnitude); break;
+    default: FFTFREE_STFT_GO(kOutPolar); break;
+  }
+#undef FFTFREE_STFT_GO
   return hipGetLastError();
+}
+
+// Multi-frame launch: the leading run of regular frames (window spans N, inside the
+// PCM, 8-byte aligned samples) goes to stft_r2c_mf_kernel, the rest to stft_r2c_kernel.
+template <int L, int E, int K, int DIR, int... Rs>
+hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
+  constexpr int N = 2 * L;
+  constexpr int P = 256 / (L / E);
+  size_t nreg = 0;
+  const bool aligned = ((((uintptr_t)a.pcm) & 15) == 0) && (a.hop % 4) == 0 && a.hop <= N / 4;
+  auto al16 = [](const float* q) { return (((uintptr_t)q) & 15) == 0; };
+  const bool planes_ok = al16(a.o_re) && al16(a.o_im) && al16(a.o_mag) && ((P * a.bins) % 4 == 0);
+  if (a.window == N && aligned && planes_ok && a.pcm_len >= (size_t)N) {
+    const size_t fit = (a.pcm_len - N) / (size_t)a.hop + 1;  // frames fully inside the PCM
+    nreg = fit > a.frame_offset ? fit - a.frame_offset : 0;
+    if (nreg > a.frames) nreg = a.frames;
+    nreg -= nreg % ((size_t)P * K);  // whole workgroups only
+  }
+  if (nreg > 0) {
+    if (!a.sink) return hipErrorInvalidValue;
+    StftArgs r = a;
+    r.frames = nreg;
+    const size_t grid = nreg / ((size_t)P * K);
+    if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+    const dim3 g((unsigned)grid), blk(256);
+    const bool half = a.bins == L + 1;
+#define FFTFREE_MF_GO(MODE)                                                                    \
+  do {                                                                                         \
+    if (half)                                                                                  \
+      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, MODE, 1, DIR, Rs...>), g, blk, 0, s, r); \
+    else                                                                                       \
+      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, MODE, 0, DIR, Rs...>), g, blk, 0, s, r); \
+  } while (0)
+    switch (a.out_mode) {
+      case kOutComplex: FFTFREE_MF_GO(kOutComplex); break;
+      case kOutC2RClear: FFTFREE_MF_GO(kOutC2RClear); break;
+      case kOutMagnitude: FFTFREE_MF_GO(kOutMagnitude); break;
+      default: FFTFREE_MF_GO(kOutPolar); break;
+    }
+#undef FFTFREE_MF_GO
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (nreg == a.frames) return hipSuccess;
+  StftArgs rest = a;
+  rest.frames = a.frames - nreg;
+  rest.frame_offset = a.frame_offset + nreg;
+  const size_t skip = nreg * (size_t)a.bins;
+  if (rest.o_re) rest.o_re += skip;
+  if (rest.o_im) rest.o_im += skip;
+  if (rest.o_mag) rest.o_mag += skip;
+  return stft_launch_impl<L, E, P, 1, DIR, Rs...>(rest, s);
 }
 
 template <int L, int E, int FPW, int DIR, int... Rs>
@@ -344,32 +824,38 @@ struct C2CEntry {
   C2CFn fwd, inv;
 };
 
-#define FFTFREE_STFT(L, E, FPW, RS, ...)                                                  \
-  {2 * (L), {L, E, FPW, RS}, &stft_launch_impl<L, E, FPW, -1, __VA_ARGS__>,               \
-   &stft_launch_impl<L, E, FPW, 1, __VA_ARGS__>}
+#define FFTFREE_STFT(L, E, FPW, IP, RS, ...)                                              \
+  {2 * (L), {L, E, FPW, RS}, &stft_launch_impl<L, E, FPW, IP, -1, __VA_ARGS__>,           \
+   &stft_launch_impl<L, E, FPW, IP, 1, __VA_ARGS__>}
+#define FFTFREE_STFT_MF(L, E, K, RS, ...)                                                \
+  {2 * (L), {L, E, K, RS}, &stft_mf_launch_impl<L, E, K, -1, __VA_ARGS__>,                \
+   &stft_mf_launch_impl<L, E, K, 1, __VA_ARGS__>}
 #define FFTFREE_C2C(L, E, FPW, RS, ...)                                                   \
   {L, {L, E, FPW, RS}, &c2c_launch_impl<L, E, FPW, -1, __VA_ARGS__>,                     \
    &c2c_launch_impl<L, E, FPW, 1, __VA_ARGS__>}
 
 // Real-input variants per N (first one per N is the default).
 const StftEntry kStft[] = {
-    FFTFREE_STFT(8, 8, 64, "8", 8),
-    FFTFREE_STFT(16, 16, 64, "16", 16),
-    FFTFREE_STFT(32, 8, 64, "8x4", 8, 4),
-    FFTFREE_STFT(64, 8, 32, "8x8", 8, 8),
-    FFTFREE_STFT(64, 16, 64, "16x4", 16, 4),
-    FFTFREE_STFT(128, 16, 32, "16x8", 16, 8),
-    FFTFREE_STFT(128, 8, 16, "8x8x2", 8, 8, 2),
-    FFTFREE_STFT(256, 16, 16, "16x16", 16, 16),
-    FFTFREE_STFT(256, 8, 8, "8x8x4", 8, 8, 4),
-    FFTFREE_STFT(512, 8, 4, "8x8x8", 8, 8, 8),
-    FFTFREE_STFT(512, 16, 8, "16x16x2", 16, 16, 2),
-    FFTFREE_STFT(512, 8, 8, "8x8x8/8", 8, 8, 8),
-    FFTFREE_STFT(1024, 16, 4, "16x16x4", 16, 16, 4),
-    FFTFREE_STFT(1024, 8, 2, "8x8x8x2", 8, 8, 8, 2),
-    FFTFREE_STFT(2048, 16, 2, "16x16x8", 16, 16, 8),
-    FFTFREE_STFT(2048, 8, 1, "8x8x8x4", 8, 8, 8, 4),
-    FFTFREE_STFT(4096, 16, 1, "16x16x16", 16, 16, 16),
+    FFTFREE_STFT(8, 8, 64, 0, "8", 8),
+    FFTFREE_STFT(16, 16, 64, 0, "16", 16),
+    FFTFREE_STFT(32, 8, 64, 1, "8x4 inplace", 8, 4),
+    FFTFREE_STFT(64, 8, 32, 1, "8x8 inplace", 8, 8),
+    FFTFREE_STFT(128, 16, 32, 1, "16x8 inplace", 16, 8),
+    FFTFREE_STFT_MF(256, 16, 4, "16x16 multiframe k=4", 16, 16),
+    FFTFREE_STFT(256, 16, 16, 1, "16x16 inplace", 16, 16),
+    FFTFREE_STFT_MF(512, 8, 8, "8x8x8 multiframe k=8", 8, 8, 8),
+    FFTFREE_STFT_MF(512, 8, 4, "8x8x8 multiframe k=4", 8, 8, 8),
+    FFTFREE_STFT_MF(512, 8, 16, "8x8x8 multiframe k=16", 8, 8, 8),
+    FFTFREE_STFT(512, 8, 4, 1, "8x8x8 inplace", 8, 8, 8),
+    FFTFREE_STFT(512, 8, 4, 0, "8x8x8 pingpong", 8, 8, 8),
+    FFTFREE_STFT(512, 8, 8, 1, "8x8x8 inplace", 8, 8, 8),
+    FFTFREE_STFT(512, 16, 8, 1, "16x16x2 inplace", 16, 16, 2),
+    FFTFREE_STFT_MF(1024, 16, 8, "16x16x4 multiframe k=8", 16, 16, 4),
+    FFTFREE_STFT(1024, 16, 4, 1, "16x16x4 inplace", 16, 16, 4),
+    FFTFREE_STFT(1024, 8, 2, 1, "8x8x8x2 inplace", 8, 8, 8, 2),
+    FFTFREE_STFT(2048, 16, 2, 1, "16x16x8 inplace", 16, 16, 8),
+    FFTFREE_STFT(2048, 8, 1, 1, "8x8x8x4 inplace", 8, 8, 8, 4),
+    FFTFREE_STFT(4096, 16, 1, 1, "16x16x16 inplace", 16, 16, 16),
 };
 
 const C2CEntry kC2C[] = {
@@ -388,6 +874,7 @@ const C2CEntry kC2C[] = {
 
 #undef FFTFREE_STFT
 #undef FFTFREE_C2C
+#undef FFTFREE_STFT_MF
 
 template <class Entry, size_t K>
 const Entry* find_variant(const Entry (&tab)[K], int n, int idx) {

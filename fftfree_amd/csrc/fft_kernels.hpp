@@ -27,6 +27,7 @@ struct StftArgs {
   float* o_im = nullptr;
   float* o_mag = nullptr;
   size_t frame_offset = 0;     // first frame index (ragged / chunked callers)
+  float* sink = nullptr;       // N floats of scratch that skipped planes are written to
 };
 
 // Complex batched transform (fft_inplace_batched semantics, eigen_fft.hpp:3152-3175):

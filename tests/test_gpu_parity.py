@@ -262,3 +262,40 @@ def test_complex_batched_inverse_roundtrip(ff, kernel):
     assert fp64.rel_l2(out.reshape(got, n), frames) <= 2 * fp64.tolerance(n)
     rgot, rout = ref.execute_complex_batched(re, im, n, inverse=True, kernel=kernel, transform=2, half_spectrum=True)
     assert rgot == got and fp64.rel_l2(out.reshape(got, n), rout) <= 2 * fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("mode", [(1, 0, 1), (0, 0, 0), (0, 1, 1), (0, 0, 1)])
+def test_stft_every_variant_matches_oracle(ff, mode):
+    """Every kernel variant the autotuner may pick (radix pattern, frames per
+    workgroup, multi-frame) meets the oracle bound; variants with the same radix pattern
+    agree to well inside it (different kernels may fuse multiply-adds differently, so
+    bits can differ); misaligned output planes take the scalar path."""
+    import torch
+    reduce_mag, polar, half = mode
+    n, W, H = 1024, 1024, 256
+    pcm = noise_plus_sines(200 * H + W + 77, seed=21)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=bool(reduce_mag),
+                     store_polar=bool(polar), half_spectrum=bool(half), apply_windows=True, analysis_window=1)
+    frames = ff.frame_count(p, pcm.size, 1)
+    bins = p.bins
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, H, frames, fp64.build_window(1, W), False, 1, reduce_mag, polar,
+                                     half)
+    d = torch.from_numpy(pcm).cuda()
+    names = ff.variants(p)
+    by_radix = {}
+    for vi, name in enumerate(names):
+        assert ff.set_variant(p, vi)
+        for offset in (0, 1):
+            buf = torch.zeros((3, frames * bins + offset), device="cuda")
+            planes = [buf[i, offset:].view(frames, bins) for i in range(3)]
+            assert ff.stft_device(p, d, *planes, pad_mode=1) == frames
+            torch.cuda.synchronize()
+            re, im, mg = (x.cpu().numpy() for x in planes)
+            assert spectra_close(re, im, ore, oim, polar, fp64.tolerance(n)), name
+            assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n), name
+            key = name.split(" ")[0]
+            if key in by_radix:
+                assert fp64.rel_l2(mg, by_radix[key]) <= fp64.tolerance(n) / 8, name
+            else:
+                by_radix[key] = mg
+    ff.set_variant(p, -1)
