@@ -11,7 +11,8 @@ timed region; the step is one fft_execute_batched_device call (one kernel launch
 of the three output planes written once); `cpu_baseline` times the oracle's restatement
 of the reference CPU path on the same workload on this host's cores.
 
-Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham).
+Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham); c2c_large
+(forward + inverse round trip of 16 complex transforms of 2^22 points, or 2^--lg).
 """
 from __future__ import annotations
 
@@ -32,7 +33,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "c2c1024"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "c2c1024", "c2c_large"])
+    ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--variant", type=int, default=-1)
@@ -147,6 +149,37 @@ def main():
         }
         if not args.no_cpu:
             line["cpu_baseline"] = cpu_baseline_stft(pcm, n, W, H, args.cpu_seconds)
+    elif args.workload == "c2c_large":
+        n, B = 1 << args.lg, 16
+        fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
+        inv = ff.make_plan(n, inverse=1, kernel=2, radix=4, transform=0, half_spectrum=False)
+        if args.variant >= 0:
+            assert ff.set_variant(fwd, args.variant) and ff.set_variant(inv, args.variant)
+        x = torch.from_numpy(complex_normal(B, n, seed=1)).cuda()
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+
+        def step():
+            ff.c2c_device(fwd, x, y)
+            ff.c2c_device(inv, y, z)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per_step = gpu_s / args.steps
+        torch.cuda.synchronize()
+        rt_err = float(torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x))
+        # each of the four passes per step (two per transform) reads and writes B*N complex64
+        pass_bytes = 2 * 8 * n * B
+        achieved = 4 * pass_bytes / per_step / 1e9
+        line = {"metric": f"C2C round trips/s, N=2^{args.lg} batch 16", "value": B * args.steps / gpu_s,
+                "unit": "round trips/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": per_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
+                "config": {"workload": f"C2C forward + inverse N=2^{args.lg} x 16, two LDS passes per transform",
+                           "n": n, "batch": B, "roundtrip_rel_l2": rt_err,
+                           "variant": ff.variants(fwd)[max(args.variant, 0)]},
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_large_2e{args.lg}"),
+                             "algorithmic_bytes_per_launch": pass_bytes}}
     else:
         n, B = 1024, 64
         p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)

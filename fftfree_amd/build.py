@@ -7,13 +7,14 @@ from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfft_cffi.so"
-SOURCES = ["fft_kernels.hip", "plan.cpp", "fft_cffi.cpp"]
+SOURCES = ["fft_kernels.hip", "fft_large.hip", "plan.cpp", "fft_cffi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
@@ -39,12 +40,12 @@ def _compile(src: Path, obj: Path, verbose: bool) -> None:
 def build(verbose: bool = False) -> Path:
     out = PKG / "_build"
     out.mkdir(exist_ok=True)
-    objs = []
-    for name in SOURCES:
-        src = CSRC / name
-        obj = out / (name + ".o")
-        _compile(src, obj, verbose)
-        objs.append(str(obj))
+    objs = [out / (name + ".o") for name in SOURCES]
+    # translation units are independent: compile them side by side
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as pool:
+        for f in [pool.submit(_compile, CSRC / name, obj, verbose) for name, obj in zip(SOURCES, objs)]:
+            f.result()
+    objs = [str(o) for o in objs]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:

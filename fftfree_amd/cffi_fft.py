@@ -168,9 +168,9 @@ class CffiFft:
 
     def variants(self, plan: FftPlan):
         names = []
-        buf = ctypes.create_string_buffer(64)
+        buf = ctypes.create_string_buffer(128)
         for i in range(int(self.lib.fft_variant_count(plan.handle))):
-            self.lib.fft_variant_name(plan.handle, i, buf, 64)
+            self.lib.fft_variant_name(plan.handle, i, buf, 128)
             names.append(buf.value.decode())
         return names
 

@@ -351,7 +351,16 @@ FftContext* make_context(size_t n, int threads, int lanes, int inverse, int kern
   return c.release();
 }
 
+// Which kernel family fft_set_variant tunes: the fused STFT kernel when the context's
+// STFT runs fused, else the complex transform (single-launch LDS or two-pass large).
+int variant_count(const FftContext* c) {
+  if (fused_ok(c)) return stft_variant_count(c->N);
+  if (c->N > c2c_max_lds_n()) return large_variant_count(c->N);
+  return c2c_variant_count(c->N);
+}
+
 const char* variant_desc(const FftContext* c, int idx, char* buf, size_t len) {
+  if (!fused_ok(c) && c->N > c2c_max_lds_n()) return large_variant_desc(c->N, idx, buf, len) ? buf : nullptr;
   const VariantInfo* v = fused_ok(c) ? stft_variant(c->N, idx) : c2c_variant(c->N, idx);
   if (!v) return nullptr;
   std::snprintf(buf, len, "%s fpw=%d", v->radices, v->fpw);
@@ -620,11 +629,9 @@ size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_
   try {
     fftfree::ensure_gpu(c);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t N = (size_t)c->N;
-    if (d_in != d_out)
-      fftfree::hip_check(hipMemcpyAsync(d_out, d_in, batch * N * sizeof(float2), hipMemcpyDeviceToDevice, s), "D2D");
     fftfree::AxisLayout l;
     l.base = reinterpret_cast<float2*>(d_out);
+    if (d_in != d_out) l.source = reinterpret_cast<const float2*>(d_in);
     l.axis_size = c->N;
     l.batch_size = (int64_t)batch;
     l.batch_stride = c->N;
@@ -866,20 +873,18 @@ int fft_set_variant(void* handle, int variant) {
     c->plan->c2c_variant = -1;
     return 1;
   }
-  if (fftfree::fused_ok(c)) {
-    if (variant >= fftfree::stft_variant_count(c->N)) return 0;
+  if (variant >= fftfree::variant_count(c)) return 0;
+  if (fftfree::fused_ok(c))
     c->plan->stft_variant = variant;
-  } else {
-    if (variant >= fftfree::c2c_variant_count(c->N)) return 0;
+  else
     c->plan->c2c_variant = variant;
-  }
   return 1;
 }
 
 int fft_variant_count(void* handle) {
   if (!handle) return 0;
   auto* c = static_cast<FftContext*>(handle);
-  return fftfree::fused_ok(c) ? fftfree::stft_variant_count(c->N) : fftfree::c2c_variant_count(c->N);
+  return fftfree::variant_count(c);
 }
 
 int fft_variant_name(void* handle, int variant, char* buf, size_t buflen) {

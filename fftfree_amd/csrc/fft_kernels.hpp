@@ -71,6 +71,15 @@ hipError_t launch_planes(const float2* in, size_t frames, int n, int bins, int p
 hipError_t launch_metadata(int n, int kernel, int kind, int r2c_half, const float2* tw, int inverse,
                            float2* out, size_t capacity, size_t* count, hipStream_t s);
 
+// Complex DFT for N above c2c_max_lds_n(): two LDS-resident passes (columns, then rows)
+// through `tmp` (batch*N complex values). Supports pre_mode 0/1 and every post hook.
+bool large_supported(size_t n);
+void large_split(size_t n, int* n1, int* n2, int* s);
+hipError_t launch_c2c_large(const C2CArgs& a, int inverse, int variant, float2* tmp, hipStream_t s);
+// Line-kernel choices of the two passes (autotuner); variant -1 = default.
+int large_variant_count(size_t n);
+bool large_variant_desc(size_t n, int variant, char* buf, size_t len);
+
 // Variant tables for the autotuner.
 struct VariantInfo {
   int l;            // complex length the kernel transforms

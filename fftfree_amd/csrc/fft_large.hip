@@ -1,0 +1,332 @@
+// fft_large.hip -- batched complex DFT for N beyond one workgroup's LDS (N = N1 * N2).
+//
+// With n = n1*N2 + n2 and k = k1 + N1*k2:
+//   X[k1 + N1 k2] = sum_{n2} W_N2^{n2 k2} [ W_N^{n2 k1} sum_{n1} x[n1 N2 + n2] W_N1^{n1 k1} ]
+// Pass A transforms the N2 columns (length N1, stride N2) and multiplies by W_N^{n2 k1};
+// pass B transforms the N1 rows (length N2, contiguous) and writes X[k1 + N1 k2].
+// Both passes hold whole lines in LDS. Column data and pass B's output are strided in
+// HBM, so a workgroup takes C adjacent lines and moves them through LDS: global reads
+// and writes are runs of C consecutive complex values. The inter-pass twiddle is the
+// product of two entries of the plan's FP64-built table (a few KB of it stay hot):
+// W_N^m = W_N^{m with its low S bits cleared} * W_N^{m mod 2^S}.
+//
+// The reference runs every N through lgN radix-2 stages in place (eigen_fft.hpp:2902-3057
+// Cooley-Tukey, 3472-3532 Stockham); the sum computed is the same DFT.
+#include "fft_device.hpp"
+#include "fft_kernels.hpp"
+
+#include <cstdio>
+
+namespace fftfree {
+using namespace dev;
+
+namespace {
+
+// Pre-transform hook on load: mode 1 clears imag (R2C).
+__device__ __forceinline__ float2 pre1(float2 z, int mode) { return mode == 1 ? make_float2(z.x, 0.0f) : z; }
+
+// Post-transform hook (eigen_fft.hpp:2535-2596) at global frequency idx of total length N.
+__device__ __forceinline__ float2 post_large(float2 X, size_t idx, size_t N, const C2CArgs& a) {
+  X = c_scale(X, a.scale);
+  if (a.post_half) {
+    if (idx == 0 || idx == (N >> 1)) X.y = 0.0f;
+    if (idx > (N >> 1)) X = make_float2(0.0f, 0.0f);
+  }
+  if (a.post_mode >= 2 && idx < (size_t)a.post_upper) {
+    const float m = __builtin_sqrtf(__builtin_fmaf(X.x, X.x, X.y * X.y));
+    X = make_float2(m, a.post_mode == 3 ? atan2f(X.y, X.x) : 0.0f);
+  } else if (a.post_mode == 1) {
+    X.y = 0.0f;
+  }
+  return X;
+}
+
+// Pre-transform hook of C2R half-spectrum plans (eigen_fft.hpp:2505-2531): bins 0..N/2
+// (optionally (mag, phase)) become the full Hermitian spectrum. Thread n <= N/2 reads
+// src[n] only and writes dst[n] and dst[N-n], so src == dst is safe.
+__global__ void __launch_bounds__(256) hermitian_rebuild_kernel(const float2* in, float2* out, size_t n, int polar) {
+  const size_t h = n >> 1;
+  const size_t b = blockIdx.y;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i > h) return;
+  const float2* __restrict__ src = in + b * n;
+  float2* __restrict__ dst = out + b * n;
+  float2 z = src[i];
+  if (polar) {
+    float sn, cs;
+    sincosf(z.y, &sn, &cs);
+    z = make_float2(z.x * cs, z.x * sn);
+  }
+  if (i == 0 || i == h) z.y = 0.0f;
+  dst[i] = z;
+  if (i != 0 && i != h) dst[n - i] = c_conj(z);
+}
+
+struct LargeArgs {
+  const float2* in;
+  float2* out;
+  size_t batch;
+  size_t n;            // N
+  int n1, n2;          // column length, row length
+  const float2* tw;    // forward table exp(-2 pi i m / N), m < N
+  int s;               // S
+  C2CArgs hooks;       // pre_mode (pass A), scale/post (pass B)
+};
+
+template <int DIR>
+__device__ __forceinline__ float2 big_twiddle(const LargeArgs& a, size_t m) {
+  const size_t mask = (size_t(1) << a.s) - 1;
+  const float2 hi = load_tw<DIR>(a.tw, (int)(m & ~mask));
+  const float2 lo = load_tw<DIR>(a.tw, (int)(m & mask));
+  return c_mul(hi, lo);
+}
+
+// Pass A: C adjacent columns (length L = N1, stride N2) per workgroup.
+template <int L, int E, int C, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E)) col_pass_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int NP = sizeof...(Rs);
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  __shared__ float2 lds[C][LP];
+
+  const int tid = threadIdx.x;
+  // 1-D grid of batch * (N2 / C) tiles; each XCD walks a contiguous run of tiles, so the
+  // neighbouring column groups that share HBM lines are read through the same L2
+  const int groups = a.n2 / C;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t b = (size_t)(tile / groups);
+  const size_t c0 = (size_t)(tile - (int)b * groups) * C;
+  const size_t N2 = (size_t)a.n2;
+  const float2* __restrict__ src = a.in + b * a.n + c0;
+  float2* __restrict__ dst = a.out + b * a.n + c0;
+
+  // coalesced column load: thread = (col = tid % C, rows tid / C + k*T)
+  {
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = row0 + k * (NT / C);
+      lds[col][pidx(j)] = pre1(src[(size_t)j * N2 + col], a.hooks.pre_mode);
+    }
+  }
+  __syncthreads();
+
+  const int slot = tid / T;
+  const int t = tid - slot * T;
+  float2* buf = &lds[slot][0];
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
+  float2 twr[E * NP];
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, (int)(a.n / L));
+  __syncthreads();  // every line read before the first in-place scatter
+  float2* Zb;
+  StockhamPlan<L, E, DIR, Rs...>::template run<true>(v, t, buf, buf, twr, Zb);
+  // twiddle W_N^{n2 k1} and natural-order store into LDS
+  const size_t n2 = c0 + slot;
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int k1 = t + q * T + r * (L / Rl);
+      buf[pidx(k1)] = c_mul(v[q * Rl + r], big_twiddle<DIR>(a, n2 * (size_t)k1));
+    }
+  __syncthreads();
+  {
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = row0 + k * (NT / C);
+      dst[(size_t)j * N2 + col] = lds[col][pidx(j)];
+    }
+  }
+}
+
+// Pass B: C adjacent rows (length L = N2, contiguous) per workgroup; output element
+// (row k1, k2) goes to k1 + N1*k2, written as runs of C consecutive values.
+template <int L, int E, int C, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E)) row_pass_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int NP = sizeof...(Rs);
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  __shared__ float2 lds[C][LP];
+
+  const int tid = threadIdx.x;
+  const int groups = a.n1 / C;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t b = (size_t)(tile / groups);
+  const size_t r0 = (size_t)(tile - (int)b * groups) * C;
+  const size_t N1 = (size_t)a.n1;
+  const int slot = tid / T;
+  const int t = tid - slot * T;
+  float2* buf = &lds[slot][0];
+  const float2* __restrict__ row = a.in + b * a.n + (r0 + slot) * (size_t)L;
+
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[q * R0 + r] = row[t + q * T + r * (L / R0)];
+  float2 twr[E * NP];
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, (int)(a.n / L));
+  float2* Zb;
+  StockhamPlan<L, E, DIR, Rs...>::template run<true>(v, t, buf, buf, twr, Zb);
+  const size_t k1 = r0 + slot;
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int k2 = t + q * T + r * (L / Rl);
+      buf[pidx(k2)] = post_large(v[q * Rl + r], k1 + N1 * (size_t)k2, a.n, a.hooks);
+    }
+  __syncthreads();
+  float2* __restrict__ dst = a.out + b * a.n + r0;
+  {
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int k2 = row0 + k * (NT / C);
+      dst[(size_t)k2 * N1 + col] = lds[col][pidx(k2)];
+    }
+  }
+}
+
+using LineFn = hipError_t (*)(const LargeArgs&, int lines, hipStream_t);
+
+template <int L, int E, int C, int DIR, int... Rs>
+hipError_t col_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((col_pass_kernel<L, E, C, DIR, Rs...>), dim3((unsigned)((lines / C) * a.batch)), dim3(C * (L / E)),
+                     0, s, a);
+  return hipGetLastError();
+}
+template <int L, int E, int C, int DIR, int... Rs>
+hipError_t row_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((row_pass_kernel<L, E, C, DIR, Rs...>), dim3((unsigned)((lines / C) * a.batch)), dim3(C * (L / E)),
+                     0, s, a);
+  return hipGetLastError();
+}
+
+struct LineEntry {
+  int l, e, c;
+  const char* radices;
+  LineFn col_fwd, col_inv, row_fwd, row_inv;
+};
+
+// Per line length, the first entry is the default; the others are autotuner candidates.
+#define FFTFREE_LINE(L, E, C, NAME, ...)                                                           \
+  {L, E, C, NAME, &col_launch<L, E, C, -1, __VA_ARGS__>, &col_launch<L, E, C, 1, __VA_ARGS__>,    \
+   &row_launch<L, E, C, -1, __VA_ARGS__>, &row_launch<L, E, C, 1, __VA_ARGS__>}
+const LineEntry kLines[] = {
+    FFTFREE_LINE(64, 8, 32, "8x8", 8, 8),
+    FFTFREE_LINE(128, 16, 32, "16x8", 16, 8),
+    FFTFREE_LINE(256, 16, 16, "16x16", 16, 16),
+    FFTFREE_LINE(512, 8, 8, "8x8x8", 8, 8, 8),
+    FFTFREE_LINE(1024, 16, 8, "16x16x4", 16, 16, 4),
+    FFTFREE_LINE(1024, 8, 8, "8x8x4x4", 8, 8, 4, 4),
+    FFTFREE_LINE(1024, 4, 4, "4x4x4x4x4", 4, 4, 4, 4, 4),
+    FFTFREE_LINE(2048, 16, 4, "16x16x8", 16, 16, 8),
+    FFTFREE_LINE(2048, 8, 4, "8x8x8x4", 8, 8, 8, 4),
+    FFTFREE_LINE(4096, 16, 4, "16x16x16", 16, 16, 16),
+};
+#undef FFTFREE_LINE
+
+int line_count(int l) {
+  int c = 0;
+  for (const auto& e : kLines) c += e.l == l;
+  return c;
+}
+
+const LineEntry* line_entry(int l, int idx = 0) {
+  for (const auto& e : kLines)
+    if (e.l == l && idx-- == 0) return &e;
+  return nullptr;
+}
+
+}  // namespace
+
+void large_split(size_t n, int* n1, int* n2, int* s) {
+  int lg = 0;
+  while ((size_t(1) << lg) < n) ++lg;
+  const int lg1 = lg / 2, lg2 = lg - lg1;
+  *n1 = 1 << lg1;
+  *n2 = 1 << lg2;
+  *s = (lg + 1) / 2;
+}
+
+bool large_supported(size_t n) {
+  if (n <= (size_t)c2c_max_lds_n()) return false;
+  int n1, n2, s;
+  large_split(n, &n1, &n2, &s);
+  return line_entry(n1) && line_entry(n2);
+}
+
+// Variant v picks column entry v % count(N1) and row entry v / count(N1).
+int large_variant_count(size_t n) {
+  if (!large_supported(n)) return 0;
+  int n1, n2, s;
+  large_split(n, &n1, &n2, &s);
+  return line_count(n1) * line_count(n2);
+}
+
+bool large_variant_desc(size_t n, int v, char* buf, size_t len) {
+  if (v < 0 || v >= large_variant_count(n)) return false;
+  int n1, n2, s;
+  large_split(n, &n1, &n2, &s);
+  const int k1 = line_count(n1);
+  const LineEntry* c = line_entry(n1, v % k1);
+  const LineEntry* r = line_entry(n2, v / k1);
+  std::snprintf(buf, len, "columns %d: %s e=%d lines=%d | rows %d: %s e=%d lines=%d", n1, c->radices, c->e, c->c, n2,
+                r->radices, r->e, r->c);
+  return true;
+}
+
+hipError_t launch_c2c_large(const C2CArgs& a, int inverse, int variant, float2* tmp, hipStream_t s) {
+  if (a.batch == 0) return hipSuccess;
+  LargeArgs la;
+  la.batch = a.batch;
+  la.n = (size_t)a.n;
+  large_split(la.n, &la.n1, &la.n2, &la.s);
+  la.tw = a.tw;
+  la.hooks = a;
+  const int k1 = line_count(la.n1);
+  if (variant >= k1 * line_count(la.n2)) return hipErrorInvalidValue;
+  const LineEntry* e1 = line_entry(la.n1, variant < 0 ? 0 : variant % k1);
+  const LineEntry* e2 = line_entry(la.n2, variant < 0 ? 0 : variant / k1);
+  if (!e1 || !e2) return hipErrorInvalidValue;
+  la.in = a.in;
+  if (a.pre_mode == 2) {  // rebuild into out, then run the plain transform from there
+    const size_t h = la.n / 2;
+    hipLaunchKernelGGL(hermitian_rebuild_kernel, dim3((unsigned)((h + 256) / 256), (unsigned)a.batch), dim3(256), 0, s,
+                       a.in, a.out, la.n, a.pre_polar);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    la.in = a.out;
+    la.hooks.pre_mode = 0;
+  }
+  // pass A: in -> tmp (columns, length N1, N2 of them)
+  la.out = tmp;
+  hipError_t err = (inverse ? e1->col_inv : e1->col_fwd)(la, la.n2, s);
+  if (err != hipSuccess) return err;
+  // pass B: tmp -> out (rows, length N2, N1 of them)
+  la.in = tmp;
+  la.out = a.out;
+  la.hooks.pre_mode = 0;
+  return (inverse ? e2->row_inv : e2->row_fwd)(la, la.n1, s);
+}
+
+}  // namespace fftfree

@@ -1,6 +1,7 @@
 // plan.cpp -- GPU Plan, kernel descriptors, fft_apply_axis and the plan cache.
 #include "plan.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -73,10 +74,21 @@ Plan::Plan(int n, bool inv, const PlanRuntimeConfig& cfg) : N(n), inverse(inv) {
 }
 
 Plan::~Plan() {
-  if (W) {
-    (void)hipSetDevice(device);
-    (void)hipFree(W);
+  if (W || scratch) (void)hipSetDevice(device);
+  if (W) (void)hipFree(W);
+  if (scratch) (void)hipFree(scratch);
+}
+
+float2* Plan::ensure_scratch(size_t elems) const {
+  auto* self = const_cast<Plan*>(this);
+  if (elems > scratch_elems) {
+    if (scratch) (void)hipFree(self->scratch);
+    self->scratch = nullptr;
+    self->scratch_elems = 0;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&self->scratch), elems * sizeof(float2)), "hipMalloc(scratch)");
+    self->scratch_elems = elems;
   }
+  return scratch;
 }
 
 void Plan::ensure_device() {
@@ -150,11 +162,27 @@ void gpu_execute_axis(const Plan& P, const AxisLayout& layout, bool cooley_tukey
     throw std::invalid_argument("GPU kernels need contiguous transforms (axis_stride 1, batch_stride N)");
   if (!P.W) throw std::logic_error("plan twiddles not on the device");
   C2CArgs a = make_c2c_args(P, cooley_tukey);
+  const bool large = P.N > c2c_max_lds_n();
+  if (large && !large_supported(static_cast<size_t>(P.N))) throw std::invalid_argument("transform size not supported");
   P.dispatcher()->parallel_for(static_cast<size_t>(layout.batch_size), 1, [&](size_t b, size_t e, int) {
-    a.in = layout.base + b * static_cast<size_t>(P.N);
-    a.out = const_cast<float2*>(a.in);
-    a.batch = e - b;
-    hip_check(launch_c2c(a, P.inverse ? 1 : 0, P.c2c_variant, P.stream()), "launch_c2c");
+    const float2* src = layout.source ? layout.source : layout.base;
+    if (!large) {
+      a.in = src + b * static_cast<size_t>(P.N);
+      a.out = layout.base + b * static_cast<size_t>(P.N);
+      a.batch = e - b;
+      hip_check(launch_c2c(a, P.inverse ? 1 : 0, P.c2c_variant, P.stream()), "launch_c2c");
+      return;
+    }
+    // two-pass large transform, in chunks that bound the scratch (<= 512 MiB)
+    const size_t per = std::max<size_t>(1, std::min<size_t>(65535, (size_t(64) << 20) / static_cast<size_t>(P.N)));
+    for (size_t c = b; c < e; c += per) {
+      const size_t cnt = std::min(per, e - c);
+      float2* tmp = P.ensure_scratch(cnt * static_cast<size_t>(P.N));
+      a.in = src + c * static_cast<size_t>(P.N);
+      a.out = layout.base + c * static_cast<size_t>(P.N);
+      a.batch = cnt;
+      hip_check(launch_c2c_large(a, P.inverse ? 1 : 0, P.c2c_variant, tmp, P.stream()), "launch_c2c_large");
+    }
   });
 }
 

@@ -42,6 +42,8 @@ struct AxisLayout {
   int64_t batch_size = 0;
   int64_t axis_stride = 1;
   int64_t batch_stride = 0;
+  // Out-of-place input with the same layout (a GPU addition; nullptr = in place).
+  const float2* source = nullptr;
   bool valid() const { return base != nullptr && axis_size > 0; }
 };
 
@@ -118,6 +120,8 @@ struct Plan {
   std::vector<int> radix_pattern;  // copied, a hint only
   int device = 0;
   float2* W = nullptr;             // device table exp(-2 pi i m / N), m < N, FP64-built
+  float2* scratch = nullptr;       // device scratch for N > 4096 (pass-A output), grown on use
+  size_t scratch_elems = 0;
   int stft_variant = -1;           // kernel variant (autotuner), -1 = default
   int c2c_variant = -1;
 
@@ -128,6 +132,8 @@ struct Plan {
 
   // Uploads the twiddle table on first use (lazy, so host-only queries need no GPU).
   void ensure_device();
+  // Scratch of at least `elems` complex values (large-N two-pass transforms).
+  float2* ensure_scratch(size_t elems) const;
 
   void use_kernel(KernelKind kind);
   const KernelDescriptor& kernel() const { return *kernel_desc_; }

@@ -134,7 +134,9 @@ size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_l
 
 /* additive. Batched complex transform of `batch` columns of N interleaved complex64
  * values (the C++ fft_inplace_batched path, eigen_fft.hpp:3152-3175, including the
- * plan's pre/post hooks). in may equal out. Host pointers / device pointers. */
+ * plan's pre/post hooks). in and out are the same buffer or do not overlap. N above
+ * 4096 runs as two passes through a device scratch of batch*N complex values owned by
+ * the context (grown on first use). Host pointers / device pointers. */
 size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t batch);
 size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream);
 
