@@ -12,7 +12,11 @@ of the three output planes written once); `cpu_baseline` times the oracle's rest
 of the reference CPU path on the same workload on this host's cores.
 
 Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham); c2c_large
-(forward + inverse round trip of 16 complex transforms of 2^22 points, or 2^--lg).
+(forward + inverse round trip of 16 complex transforms of 2^22 points, or 2^--lg);
+stft_host (the default workload through the reference's host-pointer entry
+fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); copy
+(a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
+WRITE_SIZE counters).
 """
 from __future__ import annotations
 
@@ -33,7 +37,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "c2c1024", "c2c_large"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -180,6 +184,48 @@ def main():
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_large_2e{args.lg}"),
                              "algorithmic_bytes_per_launch": pass_bytes}}
+    elif args.workload == "stft_host":
+        n, W, H, L = 1024, 1024, 256, 1 << 24
+        p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,
+                         half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=0)
+        pcm = noise_plus_sines(L, seed=2)
+        frames = ff.frame_count(p, L, 0)
+        bins = n // 2 + 1
+        planes = np.empty((3, frames * bins), np.float32)
+        fp = lambda a: a.ctypes.data_as(ff.lib.fft_execute.argtypes[1])
+
+        def step():
+            got = ff.lib.fft_execute_batched(p.handle, fp(pcm), L, fp(planes[0]), fp(planes[1]), fp(planes[2]), 0, 0,
+                                             frames)
+            assert got == frames
+
+        for _ in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        wall = time.perf_counter() - t0
+        per = wall / args.steps
+        line = {"metric": "STFT frames/s through the host-pointer entry (PCIe included)",
+                "value": frames * args.steps / wall, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": per * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic: 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
+                "config": {"workload": "fft_execute_batched on pageable host buffers: 64 MiB PCM in, "
+                                       "3 x 134 MB planes out, N=1024 Hann hop 256 half + magnitude"}}
+    elif args.workload == "copy":
+        nbytes = 512 << 20
+        x = torch.ones(nbytes // 4, device="cuda")
+        y = torch.empty_like(x)
+
+        def step():
+            torch.mul(x, 1.0, out=y)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per = gpu_s / args.steps
+        line = {"metric": "device copy GB/s (counter calibration)", "value": 2 * nbytes / per / 1e9, "unit": "GB/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": per * 1e3,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "ones",
+                "config": {"workload": "elementwise copy, 512 MiB read + 512 MiB written per launch"}}
     else:
         n, B = 1024, 64
         p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)

@@ -278,9 +278,11 @@ __global__ void __launch_bounds__(FPW*(L / E)) stft_r2c_kernel(StftArgs a) {
 //    counter covers loads and stores, in issue order, and saturates at 63);
 //  * those stores are issued during the next iteration, between its FFT passes, so each
 //    wave keeps a steady store stream instead of computing and storing in turns.
-// Only whole groups of K*P regular frames come here (window spans N, every frame inside
-// the PCM, hop a multiple of 4 and at most N/4, samples and planes 16-byte aligned);
-// the launcher sends the rest to stft_r2c_kernel.
+// Whole tiles of K*P regular frames (every frame inside the PCM) run as above; they
+// need the window to span N, hop a multiple of 4 and at most N/4, and 16-byte aligned
+// samples and planes. The frames after the last whole tile (a few dozen) go to
+// workgroup 0, which is dispatched first: it walks them P at a time with per-frame
+// bounds, zero fill and direct plane stores, while the tiles run.
 // LDS exchanges between passes stay inside one wave, so they are ordered by the wave's
 // in-order LDS queue plus a compiler fence; no workgroup barrier is needed.
 __device__ __forceinline__ void wave_lds_fence() {
@@ -363,11 +365,13 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   const int slot = tid / T;
   const int t = tid & (T - 1);
   float2* buf = &lds[slot][0];
-  const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
+  const int has_rest = a.frames > a.lead ? 1 : 0;
+  const size_t tile = (size_t)xcd_tile((int)blockIdx.x - has_rest, (int)gridDim.x - has_rest);
   const size_t f0 = tile * (size_t)(K * P);
   const int hop = a.hop;
   const float* __restrict__ win = a.win;
   const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win);
+  constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
 
   // per-thread constants: window taps, pass twiddles, split twiddles
   float2 wr[E];
@@ -382,6 +386,74 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
 #pragma unroll
   for (int j = 0; j < JF; ++j) wsp[j] = load_tw<DIR>(a.tw, t + j * T);
   const float2 wmid = load_tw<DIR>(a.tw, L / 2);
+
+  if (has_rest && blockIdx.x == 0) {
+    for (size_t fb = a.lead; fb < a.frames; fb += P) {
+      const size_t f = fb + slot;
+      const bool active = f < a.frames;
+      const size_t start = (f + a.frame_offset) * (size_t)hop;
+      size_t cnt = 0;
+      if (active && start < a.pcm_len) {
+        const size_t rem = a.pcm_len - start;
+        cnt = rem < (size_t)a.window ? rem : (size_t)a.window;
+      }
+      const float* __restrict__ src = a.pcm + (cnt ? start : 0);
+      float2 v[E];
+#pragma unroll
+      for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const int n = t + q * T + r * (L / R0);
+          v[q * R0 + r] = make_float2(frame_sample(src, win, cnt, 2 * n), frame_sample(src, win, cnt, 2 * n + 1));
+        }
+      WaveStockham<L, E, DIR, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
+      store_natural<L, E, Rl>(v, t, buf);
+      wave_lds_fence();
+      if (active) {
+        float* __restrict__ ore = a.o_re ? a.o_re + f * (size_t)BINS : nullptr;
+        float* __restrict__ oim = a.o_im ? a.o_im + f * (size_t)BINS : nullptr;
+        float* __restrict__ omg = a.o_mag ? a.o_mag + f * (size_t)BINS : nullptr;
+        auto put1 = [&](int k, float2 X) {
+          float A, B;
+          stage_vals<MODE>(X, A, B);
+          if (ore) ore[k] = plane_value<MODE, 0>(A, B);
+          if (oim) oim[k] = plane_value<MODE, 1>(A, B);
+          if (omg) omg[k] = plane_value<MODE, 2>(A, B);
+        };
+        auto put = [&](int k, float2 X) {
+          if (DIR > 0) X = c_scale(X, scale);
+          put1(k, X);
+          if constexpr (!HALF) {
+            if (k > 0 && k < L) put1(N - k, c_conj(X));
+          }
+        };
+#pragma unroll
+        for (int j = 0; j < JF; ++j) {
+          const int p = t + j * T;
+          const float2 zp = buf[pidx(p)];
+          const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
+          const float2 e = c_scale(c_add(zp, zc), 0.5f);
+          const float2 d = c_sub(zp, zc);
+          const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
+          float2 X1 = c_add(e, wo);
+          float2 X2 = c_conj(c_sub(e, wo));
+          if (p == 0) {
+            X1.y = 0.0f;
+            X2.y = 0.0f;
+          }
+          put(p, X1);
+          put(L - p, X2);
+        }
+        const float2 zm = buf[pidx(L / 2)];
+        const float2 zc = c_conj(zm);
+        const float2 e = c_scale(c_add(zm, zc), 0.5f);
+        const float2 d = c_sub(zm, zc);
+        put(L / 2, c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x))));
+      }
+      wave_lds_fence();
+    }
+    return;
+  }
 
   // iteration it covers frames f0 + it*P .. +P-1, i.e. samples [s, s + (P-1)*hop + N)
   // of the PCM (the host guarantees 16-byte alignment and that they lie inside it);
@@ -409,7 +481,6 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   stash_span(0, nxt);
   __syncthreads();
   if (K > 1) load_span(1, nxt);
-  constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
 
   // Store round c (c < C4) of iteration it's staged run: the P frames of an iteration
   // are one contiguous, 16-byte-aligned run per plane; spare lanes of the last round
@@ -749,8 +820,8 @@ hipError_t stft_launch_impl(const StftArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Multi-frame launch: the leading run of regular frames (window spans N, inside the
-// PCM, 8-byte aligned samples) goes to stft_r2c_mf_kernel, the rest to stft_r2c_kernel.
+// Multi-frame launch: whole tiles of regular frames plus a short remainder go to one
+// stft_r2c_mf_kernel launch; anything it cannot take goes to stft_r2c_kernel.
 template <int L, int E, int K, int DIR, int... Rs>
 hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
   constexpr int N = 2 * L;
@@ -759,17 +830,21 @@ hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
   const bool aligned = ((((uintptr_t)a.pcm) & 15) == 0) && (a.hop % 4) == 0 && a.hop <= N / 4;
   auto al16 = [](const float* q) { return (((uintptr_t)q) & 15) == 0; };
   const bool planes_ok = al16(a.o_re) && al16(a.o_im) && al16(a.o_mag) && ((P * a.bins) % 4 == 0);
-  if (a.window == N && aligned && planes_ok && a.pcm_len >= (size_t)N) {
+  const bool eligible = a.window == N && aligned && planes_ok;
+  if (eligible && a.pcm_len >= (size_t)N) {
     const size_t fit = (a.pcm_len - N) / (size_t)a.hop + 1;  // frames fully inside the PCM
     nreg = fit > a.frame_offset ? fit - a.frame_offset : 0;
     if (nreg > a.frames) nreg = a.frames;
     nreg -= nreg % ((size_t)P * K);  // whole workgroups only
   }
-  if (nreg > 0) {
+  // a short remainder rides along in workgroup 0 of the same launch
+  const size_t take = eligible && a.frames - nreg <= 2 * (size_t)P * K ? a.frames : nreg;
+  if (take > 0) {
     if (!a.sink) return hipErrorInvalidValue;
     StftArgs r = a;
-    r.frames = nreg;
-    const size_t grid = nreg / ((size_t)P * K);
+    r.frames = take;
+    r.lead = nreg;
+    const size_t grid = nreg / ((size_t)P * K) + (take > nreg ? 1 : 0);
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid), blk(256);
     const bool half = a.bins == L + 1;
@@ -790,11 +865,11 @@ hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (nreg == a.frames) return hipSuccess;
+  if (take == a.frames) return hipSuccess;
   StftArgs rest = a;
-  rest.frames = a.frames - nreg;
-  rest.frame_offset = a.frame_offset + nreg;
-  const size_t skip = nreg * (size_t)a.bins;
+  rest.frames = a.frames - take;
+  rest.frame_offset = a.frame_offset + take;
+  const size_t skip = take * (size_t)a.bins;
   if (rest.o_re) rest.o_re += skip;
   if (rest.o_im) rest.o_im += skip;
   if (rest.o_mag) rest.o_mag += skip;

@@ -28,6 +28,7 @@ struct StftArgs {
   float* o_mag = nullptr;
   size_t frame_offset = 0;     // first frame index (ragged / chunked callers)
   float* sink = nullptr;       // N floats of scratch that skipped planes are written to
+  size_t lead = 0;             // multi-frame kernel: frames covered by whole tiles
 };
 
 // Complex batched transform (fft_inplace_batched semantics, eigen_fft.hpp:3152-3175):
@@ -72,7 +73,7 @@ hipError_t launch_metadata(int n, int kernel, int kind, int r2c_half, const floa
                            float2* out, size_t capacity, size_t* count, hipStream_t s);
 
 // Complex DFT for N above c2c_max_lds_n(): two LDS-resident passes (columns, then rows)
-// through `tmp` (batch*N complex values). Supports pre_mode 0/1 and every post hook.
+// through `tmp` (batch*N complex values), with every pre and post hook.
 bool large_supported(size_t n);
 void large_split(size_t n, int* n1, int* n2, int* s);
 hipError_t launch_c2c_large(const C2CArgs& a, int inverse, int variant, float2* tmp, hipStream_t s);

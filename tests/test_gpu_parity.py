@@ -299,3 +299,28 @@ def test_stft_every_variant_matches_oracle(ff, mode):
             else:
                 by_radix[key] = mg
     ff.set_variant(p, -1)
+
+
+@pytest.mark.parametrize("frames", [1, 7, 8, 63, 64, 65, 127, 129, 200, 1000])
+@pytest.mark.parametrize("half", [1, 0])
+def test_stft_multiframe_remainders(ff, frames, half):
+    """Frame counts around the multi-frame kernel's tile size (the whole tiles and the
+    remainder share one launch) and a PCM shorter than one frame, through the device
+    entry with separately allocated (aligned) planes, against the FP64 oracle."""
+    import torch
+    n, W, H = 1024, 1024, 256
+    L = W + (frames - 1) * H + 17 if frames > 1 else W - 100
+    pcm = noise_plus_sines(L, seed=frames)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, half_spectrum=bool(half), apply_windows=True,
+                     analysis_window=1, pad_mode=1)
+    got = ff.frame_count(p, L, 1)
+    assert got == fp64.frame_count(L, n, W, H, 0, 1) == frames
+    bins = n // 2 + 1 if half else n
+    d = torch.from_numpy(pcm).cuda()
+    out = [torch.full((got, bins), float("nan"), device="cuda") for _ in range(3)]
+    assert ff.stft_device(p, d, out[0], out[1], out[2], pad_mode=1) == got
+    torch.cuda.synchronize()
+    re, im, mg = (o.cpu().numpy() for o in out)
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, fp64.build_window(1, W), False, 1, False, False, bool(half))
+    assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= fp64.tolerance(n)
+    assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
