@@ -13,7 +13,9 @@ of the reference CPU path on the same workload on this host's cores.
 
 Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham); c2c_large
 (forward + inverse round trip of 16 complex transforms of 2^22 points, or 2^--lg);
-stft_host (the default workload through the reference's host-pointer entry
+stft_grad (differentiable STFT: 32 signals of 2^18 samples, window 2048, hop 512, polar
+planes; forward kernel alone vs forward + adjoint kernel, and a whole torch autograd step
+of the loss sum (|X| - |Y|)^2); stft_host (the default workload through the reference's host-pointer entry
 fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
 WRITE_SIZE counters).
@@ -37,7 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "copy"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "stft_grad", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -212,6 +214,53 @@ def main():
                 "vs_baseline": None, "dtype": "f32", "data": "synthetic: 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
                 "config": {"workload": "fft_execute_batched on pageable host buffers: 64 MiB PCM in, "
                                        "3 x 134 MB planes out, N=1024 Hann hop 256 half + magnitude"}}
+    elif args.workload == "stft_grad":
+        from fftfree_amd.autograd import magnitude_loss, polar_plan
+        n, W, H, B, L = 2048, 2048, 512, 32, 1 << 18
+        p = polar_plan(n, W, H)
+        x = torch.from_numpy(np.stack([noise_plus_sines(L, seed=1000 + i) for i in range(B)])).cuda()
+        y = torch.from_numpy(np.stack([noise_plus_sines(L, seed=2000 + i) for i in range(B)])).cuda()
+        frames = ff.frame_count(p, L, 0)
+        bins = n // 2 + 1
+        mag = torch.empty((B, frames, bins), device="cuda")
+        phase = torch.empty_like(mag)
+        ymag = torch.empty_like(mag)
+        ff.stft_signals_device(p, y, ymag, None, None)
+        gmag = torch.empty_like(mag)
+        gx = torch.empty_like(x)
+
+        def fwd():
+            ff.stft_signals_device(p, x, mag, phase, None)
+
+        def adj():
+            ff.stft_adjoint_device(p, x, gmag, None, gx)
+
+        fwd_s, _ = timed(fwd, args.steps, args.warmup, torch)
+        torch.mul(mag - ymag, 2.0, out=gmag)
+        adj_s, _ = timed(adj, args.steps, args.warmup, torch)
+        xt = x.clone().requires_grad_(True)
+
+        def train_step():
+            xt.grad = None
+            magnitude_loss(xt, ymag, n, W, H).backward()
+
+        step_s, wall = timed(train_step, args.steps, args.warmup, torch)
+        fwd_ms, adj_ms = fwd_s / args.steps * 1e3, adj_s / args.steps * 1e3
+        fwd_bytes = B * L * 4 + B * frames * bins * 8
+        adj_bytes = B * frames * bins * 4 + B * L * 8
+        line = {"metric": "differentiable STFT: (fwd + adjoint) / fwd time, and signals/s", "value": B / (step_s / args.steps),
+                "unit": "signals/s (whole autograd step)", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": step_s / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32", "data": "synthetic: noise + 8 sinusoids, seeds 1000.. and 2000..",
+                "config": {"workload": "32 signals x 2^18 samples, Hann 2048, hop 512, polar planes, 509 frames x 1025 bins",
+                           "fwd_kernel_ms": fwd_ms, "adjoint_kernel_ms": adj_ms,
+                           "fwd_plus_adjoint_over_fwd": (fwd_ms + adj_ms) / fwd_ms},
+                "roofline": {"bound": "hbm", "achieved": adj_bytes / (adj_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "traffic": None, "algorithmic_bytes_per_launch": adj_bytes, "kernel": "stft_adjoint_kernel",
+                             "forward": {"achieved": fwd_bytes / (fwd_ms * 1e-3) / 1e9,
+                                         "frac": fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                         "algorithmic_bytes_per_launch": fwd_bytes}}}
     elif args.workload == "copy":
         nbytes = 512 << 20
         x = torch.ones(nbytes // 4, device="cuda")

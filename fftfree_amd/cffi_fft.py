@@ -203,3 +203,30 @@ class CffiFft:
         if got == 0:
             raise RuntimeError("fft_execute_c2c_batched_device failed")
         return out
+
+    def stft_signals_device(self, plan: FftPlan, pcm, out_re=None, out_im=None, out_mag=None,
+                            pad_mode: int = 0) -> int:
+        """fft_stft_signals_device: pcm is a (signals, length) float32 CUDA tensor; planes
+        are (signals, frames, bins) tensors or None. Returns frames per signal."""
+        assert pcm.dim() == 2 and pcm.is_contiguous()
+        ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        got = int(self.lib.fft_stft_signals_device(plan.handle, ctypes.c_void_p(pcm.data_ptr()), pcm.shape[0],
+                                                   pcm.shape[1], ptr(out_re), ptr(out_im), ptr(out_mag), pad_mode,
+                                                   self._stream()))
+        if got == 0:
+            raise RuntimeError("fft_stft_signals_device failed")
+        return got
+
+    def stft_adjoint_device(self, plan: FftPlan, pcm, grad_re=None, grad_im=None, out=None, pad_mode: int = 0):
+        """fft_stft_adjoint_device: gradient of the samples from the gradients of the
+        first two planes ((signals, frames, bins) tensors or None). Returns out."""
+        import torch
+        assert pcm.dim() == 2 and pcm.is_contiguous()
+        out = torch.empty_like(pcm) if out is None else out
+        ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        got = int(self.lib.fft_stft_adjoint_device(plan.handle, ctypes.c_void_p(pcm.data_ptr()), pcm.shape[0],
+                                                   pcm.shape[1], ptr(grad_re), ptr(grad_im),
+                                                   ctypes.c_void_p(out.data_ptr()), pad_mode, self._stream()))
+        if got == 0:
+            raise RuntimeError("fft_stft_adjoint_device failed")
+        return out

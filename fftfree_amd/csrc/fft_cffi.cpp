@@ -623,6 +623,79 @@ size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_l
   }
 }
 
+// The signals entries: a forward R2C half-spectrum context; N and the window/hop must
+// be ones the signal kernels take (signals_supported).
+bool signals_ok(const FftContext* c, bool adjoint) {
+  if (c->inverse || c->transform != FFT_TRANSFORM_R2C || !c->half_spectrum) return false;
+  const int W = c->window ? c->window : c->N;
+  const int H = c->hop ? c->hop : W;
+  return fftfree::signals_supported(c->N, W, H, adjoint);
+}
+
+fftfree::SignalArgs signal_args(FftContext* c, const float* d_pcm, size_t signals, size_t pcm_len, size_t frames) {
+  const fftfree::StftArgs s = fftfree::stft_args(c, d_pcm, pcm_len);
+  fftfree::SignalArgs a;
+  a.pcm = d_pcm;
+  a.len = pcm_len;
+  a.signals = signals;
+  a.frames = frames;
+  a.win = s.win;
+  a.window = s.window;
+  a.hop = s.hop;
+  a.n = c->N;
+  a.tw = s.tw;
+  return a;
+}
+
+size_t fft_stft_signals_device(void* handle, const float* d_pcm, size_t signals, size_t pcm_len, float* d_out_real,
+                               float* d_out_imag, float* d_out_mag, int pad_mode, void* stream) {
+  if (!handle || !d_pcm || signals == 0 || pcm_len == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (!signals_ok(c, false)) return 0;
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, 0);
+  if (frames == 0) return 0;
+  try {
+    fftfree::ensure_gpu(c);
+    fftfree::SignalArgs a = signal_args(c, d_pcm, signals, pcm_len, frames);
+    a.o_re = d_out_real;
+    a.o_im = d_out_imag;
+    a.o_mag = d_out_mag;
+    const int mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : 0;  // polar, magnitude, complex
+    fftfree::hip_check(fftfree::launch_stft_signals(a, mode, static_cast<hipStream_t>(stream)), "launch_stft_signals");
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stft_signals_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
+                               const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,
+                               void* stream) {
+  if (!handle || !d_pcm || !d_grad_pcm || signals == 0 || pcm_len == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (!signals_ok(c, true)) return 0;
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, 0);
+  if (frames == 0) return 0;
+  try {
+    fftfree::ensure_gpu(c);
+    fftfree::SignalArgs a = signal_args(c, d_pcm, signals, pcm_len, frames);
+    a.g_mag = d_grad_real;
+    a.g_phase = d_grad_imag;
+    a.grad = d_grad_pcm;
+    const int mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : 0;  // polar, magnitude, complex
+    fftfree::hip_check(fftfree::launch_stft_adjoint(a, mode, static_cast<hipStream_t>(stream)), "launch_stft_adjoint");
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stft_adjoint_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
 size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream) {
   if (!handle || !d_in || !d_out || batch == 0) return 0;
   auto* c = static_cast<FftContext*>(handle);

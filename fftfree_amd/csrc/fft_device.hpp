@@ -260,5 +260,95 @@ __device__ __forceinline__ void store_planes(float2 z, int mode, size_t idx, flo
   if (o_mag) o_mag[idx] = c;
 }
 
+// Reads framed, windowed real samples x[m] = pcm[start+m] * win[m] for m < cnt, else 0
+// (fft_cffi.cpp:766-791; the product is a single float multiply as in the reference).
+__device__ __forceinline__ float frame_sample(const float* __restrict__ src, const float* __restrict__ win,
+                                             size_t cnt, int m) {
+  if ((size_t)m >= cnt) return 0.0f;
+  const float x = src[m];
+  return win ? x * win[m] : x;
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// The pair (A, B) staged for one bin of the post-transformed value, per output mode:
+// complex (re, im), C2R-clear (re, -), magnitude (|z|, -), polar (|z|, arg z).
+// |z| uses the hardware square root (within 1 ulp).
+template <int MODE>
+__device__ __forceinline__ void stage_vals(float2 X, float& A, float& B) {
+  if constexpr (MODE == kOutComplex || MODE == kOutC2RClear) {
+    A = X.x;
+    B = X.y;
+  } else {
+    A = __builtin_amdgcn_sqrtf(__builtin_fmaf(X.x, X.x, X.y * X.y));
+    B = (MODE == kOutPolar) ? atan2f(X.y, X.x) : 0.0f;
+  }
+}
+
+template <int MODE>
+constexpr bool mode_has_b() {
+  return MODE == kOutComplex || MODE == kOutPolar;
+}
+
+// Plane WHICH (0 re, 1 im, 2 mag) from the staged pair: the output copy of
+// fft_cffi.cpp:811-825 (mag plane = |v|, or re when polar) applied to the value.
+template <int MODE, int WHICH>
+__device__ __forceinline__ float plane_value(float A, float B) {
+  if constexpr (WHICH == 0) return A;
+  if constexpr (WHICH == 1) return mode_has_b<MODE>() ? B : 0.0f;
+  if constexpr (MODE == kOutComplex) return __builtin_amdgcn_sqrtf(__builtin_fmaf(A, A, B * B));
+  if constexpr (MODE == kOutC2RClear) return __builtin_fabsf(A);
+  return A;
+}
+
+// LDS exchanges between passes stay inside one wave, so they are ordered by the wave's
+// in-order LDS queue plus a compiler fence; no workgroup barrier is needed.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int L, int E, int DIR, int Ns, int P, int R, int... Rest>
+struct WaveStockham {
+  static constexpr int T = L / E;
+  static constexpr bool kLast = sizeof...(Rest) == 0;
+  // Runs this pass and the rest in one wave; result in last-pass order in v. hook(P) runs
+  // after pass P's arithmetic (used to slot in independent stores).
+  template <class Hook>
+  __device__ static __forceinline__ void run(float2* v, int t, float2* buf, const float2* twr, Hook&& hook) {
+#pragma unroll
+    for (int q = 0; q < E / R; ++q) {
+      float2* u = v + q * R;
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], twr[P * E + q * (R - 1) + r - 1]);
+      }
+      reg_dft<R, DIR>(u);
+    }
+    hook(P);
+    if constexpr (!kLast) {
+#pragma unroll
+      for (int q = 0; q < E / R; ++q) {
+        const int bb = t + q * T;
+        const int k = bb & (Ns - 1);
+        const int o = (bb - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pidx(o + r * Ns)] = v[q * R + r];
+      }
+      wave_lds_fence();
+      constexpr int Rn = FirstOf<Rest...>::value;
+#pragma unroll
+      for (int q = 0; q < E / Rn; ++q) {
+        const int bb = t + q * T;
+#pragma unroll
+        for (int r = 0; r < Rn; ++r) v[q * Rn + r] = buf[pidx(bb + r * (L / Rn))];
+      }
+      wave_lds_fence();
+      WaveStockham<L, E, DIR, Ns * R, P + 1, Rest...>::run(v, t, buf, twr, hook);
+    }
+  }
+};
+
 }  // namespace dev
 }  // namespace fftfree

@@ -48,6 +48,27 @@ struct C2CArgs {
   float scale = 1.0f;
 };
 
+// Batched real-input STFT of `signals` equal-length signals (signal s at pcm + s*len)
+// and its adjoint; planes and upstream gradients are [signal][frame][N/2+1] floats.
+struct SignalArgs {
+  const float* pcm = nullptr;
+  size_t len = 0;              // samples per signal
+  size_t signals = 0;
+  size_t frames = 0;           // frames per signal
+  const float* win = nullptr;  // N taps (taps >= window are 1), or nullptr when windows are off
+  int window = 0;
+  int hop = 0;
+  int n = 0;
+  const float2* tw = nullptr;  // forward table exp(-2 pi i m / N), m < N
+  float* o_re = nullptr;       // forward planes (any may be nullptr)
+  float* o_im = nullptr;
+  float* o_mag = nullptr;
+  const float* g_mag = nullptr;    // adjoint: gradient of plane re (|X| when polar/magnitude)
+  const float* g_phase = nullptr;  // adjoint: gradient of plane im (arg X when polar), may be nullptr
+  float* grad = nullptr;           // adjoint output: dL/dpcm, signals*len floats
+  int tiles = 0;                   // set by the launchers
+};
+
 // Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
 enum MetaKind : int {
   kMetaTwiddle = 0,
@@ -80,6 +101,12 @@ hipError_t launch_c2c_large(const C2CArgs& a, int inverse, int variant, float2* 
 // Line-kernel choices of the two passes (autotuner); variant -1 = default.
 int large_variant_count(size_t n);
 bool large_variant_desc(size_t n, int variant, char* buf, size_t len);
+
+// Signals STFT and adjoint: N in {512, 1024, 2048}, forward R2C half spectrum, output
+// mode kOutComplex / kOutMagnitude / kOutPolar; the adjoint needs (window-1)/hop <= 4.
+bool signals_supported(int n, int window, int hop, bool adjoint);
+hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s);
+hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s);
 
 // Variant tables for the autotuner.
 struct VariantInfo {

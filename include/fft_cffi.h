@@ -140,6 +140,25 @@ size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_l
 size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t batch);
 size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream);
 
+/* additive. Batched STFT of `signals` signals of pcm_len samples each (signal s at
+ * d_pcm + s*pcm_len), for a forward R2C half-spectrum context with N in {512, 1024,
+ * 2048}: every signal is framed with the context's window/hop/pad rule (frames per
+ * signal = fft_frame_count) and its planes, as fft_execute_batched writes them, go to
+ * d_out_* + s*frames*(N/2+1). Any plane may be NULL. Returns frames per signal, 0 on
+ * error. Device pointers; enqueued on `stream`. */
+size_t fft_stft_signals_device(void* handle, const float* d_pcm, size_t signals, size_t pcm_len, float* d_out_real,
+                               float* d_out_imag, float* d_out_mag, int pad_mode, void* stream);
+
+/* additive. Adjoint of fft_stft_signals_device: given the gradients of a loss with
+ * respect to the first two output planes (re and im; |X| and arg X when store_polar;
+ * |X| when reduce_magnitude, where d_grad_imag is ignored), laid out like those planes,
+ * writes dL/dpcm (signals*pcm_len floats) to d_grad_pcm. Each frame's spectrum is
+ * recomputed from d_pcm. Either gradient may be NULL (zero). Needs (window-1)/hop <= 4.
+ * Deterministic (fixed summation order). Returns frames per signal, 0 on error. */
+size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
+                               const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,
+                               void* stream);
+
 /* additive. Index metadata of the context's plan (MetadataRequest kinds of the C++ API,
  * eigen_fft.hpp:1496-1501), written by a GPU index kernel as complex<float> pairs into
  * the host buffer `out` (capacity in complex entries). Returns entries written; 0 if the

@@ -203,3 +203,74 @@ def stft_magnitude_grad(pcm, y_mag, N, W, H, frames, win, grad_phase=None):
         if c > 0:
             g[s:s + c] += g_frames[f, :c] * win64[:c]
     return g
+
+
+def _frames64(pcm, N, W, H, frames, win):
+    """frames x N float64 matrix of windowed, zero-padded frames and the valid counts."""
+    pcm = np.asarray(pcm, dtype=np.float64)
+    L = pcm.shape[0]
+    w = np.asarray(win, dtype=np.float64) if win is not None else np.ones(N)
+    x = np.zeros((frames, N))
+    cnt = np.zeros(frames, dtype=np.int64)
+    for f in range(frames):
+        s = f * H
+        c = min(W, L - s) if s < L else 0
+        cnt[f] = c
+        if c > 0:
+            x[f, :c] = pcm[s:s + c] * w[:c]
+    return x, cnt, w
+
+
+def value_planes64(pcm, N, W, H, frames, win, mode):
+    """The two value planes of a forward R2C half-spectrum STFT in float64, per output mode
+    ('complex': Re X, Im X; 'polar': |X|, arg X; 'magnitude': |X|, 0), with the post
+    hook's real DC/Nyquist bins (eigen_fft.hpp:2535-2596)."""
+    x, _, _ = _frames64(pcm, N, W, H, frames, win)
+    X = np.fft.rfft(x, axis=-1)
+    X[:, 0] = X[:, 0].real
+    X[:, -1] = X[:, -1].real
+    if mode == "complex":
+        return X.real, X.imag
+    if mode == "polar":
+        return np.abs(X), np.angle(X)
+    return np.abs(X), np.zeros(X.shape)
+
+
+def stft_adjoint64(pcm, N, W, H, frames, win, mode, ga, gb=None, ola_window=True):
+    """FP64 gradient of sum(ga * A + gb * B) over the value planes (A, B) of
+    value_planes64 with respect to the samples (hand derivation):
+      dL/dx[fH+n] = w[n] * sum_{k=0}^{N/2} Re(V_fk e^{+2 pi i kn/N}),
+      V = ga + i gb (complex), ga X/|X| + i gb X/|X|^2 (polar), ga X/|X| (magnitude),
+    V = 0 where |X| = 0 and no imaginary-part terms at bins 0 and N/2 (their imaginary
+    parts are pinned to 0). The sum over k is (N/2) * irfft(V') with V'_0 = 2 V_0 and
+    V'_{N/2} = 2 V_{N/2}. ola_window=False leaves the window out of the overlap-add: a
+    planted mistake for the tests that show the finite-difference check catches it."""
+    x, cnt, w = _frames64(pcm, N, W, H, frames, win)
+    X = np.fft.rfft(x, axis=-1)
+    X[:, 0] = X[:, 0].real
+    X[:, -1] = X[:, -1].real
+    ga = np.asarray(ga, dtype=np.float64)
+    gb = np.zeros_like(ga) if gb is None else np.asarray(gb, dtype=np.float64)
+    gb = gb.copy()
+    gb[:, 0] = 0.0
+    gb[:, -1] = 0.0
+    if mode == "complex":
+        V = ga + 1j * gb
+    else:
+        mag = np.abs(X)
+        safe = np.where(mag > 0, mag, 1.0)
+        V = ga * X / safe
+        if mode == "polar":
+            V = V + 1j * gb * X / (safe * safe)
+        V = np.where(mag > 0, V, 0)
+    Vp = V.copy()
+    Vp[:, 0] *= 2
+    Vp[:, -1] *= 2
+    r = (N / 2) * np.fft.irfft(Vp, n=N, axis=-1)
+    L = np.asarray(pcm).shape[0]
+    g = np.zeros(L)
+    for f in range(frames):
+        c = cnt[f]
+        if c > 0:
+            g[f * H:f * H + c] += r[f, :c] * (w[:c] if ola_window else 1.0)
+    return g

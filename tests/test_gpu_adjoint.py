@@ -1,0 +1,223 @@
+"""Batched signals STFT and its adjoint kernel (fft_stft_signals_device,
+fft_stft_adjoint_device), and the torch autograd wrapper, against the FP64 oracle.
+
+Tolerances: forward planes rel-L2 <= 2e-7*log2(N) (fp64.tolerance); gradients rel-L2 <=
+4*fp64.tolerance(N) against the FP64 adjoint (the gradient chains a forward DFT, a
+normalisation by |X|, an inverse DFT and an overlap-add of up to 5 frames); finite
+differences of the FP64 loss agree per coordinate within 1e-3 relative (+1e-5 of the
+coordinates' RMS).
+
+X -> X/|X| and X -> X/|X|^2 are ill-conditioned where |X| is near 0: there the FP32
+spectrum's direction is noise whatever the algorithm. Tests that choose their own
+upstream gradients for the polar and magnitude maps therefore give zero weight to bins
+below 1% of the RMS magnitude and scale the phase gradient by |X|/RMS(|X|), so that the
+comparison measures the kernel, not the conditioning of those bins."""
+import numpy as np
+import pytest
+
+from fftfree_amd import CffiFft
+from fftfree_amd.workloads import noise_plus_sines
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+MODES = {"complex": dict(), "polar": dict(store_polar=True), "magnitude": dict(reduce_magnitude=True)}
+
+
+@pytest.fixture(scope="module")
+def ff():
+    f = CffiFft()
+    yield f
+    f.close()
+
+
+def make(ff, n, W, H, mode, pad=1, window_kind=1):
+    return ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, half_spectrum=True, apply_windows=window_kind != 0,
+                        analysis_window=window_kind, pad_mode=pad, **MODES[mode])
+
+
+def signals(count, length, seed):
+    return np.stack([noise_plus_sines(length, seed=seed + i) for i in range(count)])
+
+
+def win_n(n, W, kind=1):
+    if not kind:
+        return None
+    return np.concatenate([fp64.build_window(kind, W), np.ones(n - W, np.float32)])
+
+
+CASES = [(512, 512, 128, 5000), (1024, 1024, 256, 9000), (2048, 2048, 512, 20000), (1024, 768, 200, 7001),
+         (2048, 2048, 512, 1500)]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("mode", ["complex", "polar", "magnitude"])
+def test_signals_forward_vs_oracle(ff, case, mode):
+    import torch
+    n, W, H, length = case
+    print(f"signals fwd n={n} W={W} H={H} len={length} mode={mode}", flush=True)
+    p = make(ff, n, W, H, mode)
+    x = signals(3, length, seed=n + W)
+    frames = ff.frame_count(p, length, 0)
+    assert frames == fp64.frame_count(length, n, W, H, 1, 0)
+    bins = n // 2 + 1
+    d = torch.from_numpy(x).cuda()
+    out = [torch.full((3, frames, bins), float("nan"), device="cuda") for _ in range(3)]
+    assert ff.stft_signals_device(p, d, *out) == frames
+    torch.cuda.synchronize()
+    re, im, mg = (o.cpu().numpy() for o in out)
+    for s in range(3):
+        A, B = fp64.value_planes64(x[s], n, W, H, frames, win_n(n, W), mode)
+        if mode == "polar":
+            assert fp64.rel_l2(re[s] * np.exp(1j * im[s]), A * np.exp(1j * B)) <= fp64.tolerance(n)
+            assert np.array_equal(mg[s], re[s])
+        elif mode == "complex":
+            assert fp64.rel_l2(re[s] + 1j * im[s], A + 1j * B) <= fp64.tolerance(n)
+            assert fp64.rel_l2(mg[s], np.abs(A + 1j * B)) <= fp64.tolerance(n)
+        else:
+            assert fp64.rel_l2(re[s], A) <= fp64.tolerance(n) and not im[s].any()
+
+
+def test_signals_forward_matches_single_signal_entry(ff):
+    import torch
+    n, W, H, length = 2048, 2048, 512, 30000
+    p = make(ff, n, W, H, "polar")
+    x = signals(2, length, seed=5)
+    frames = ff.frame_count(p, length, 0)
+    bins = n // 2 + 1
+    d = torch.from_numpy(x).cuda()
+    mag = torch.empty((2, frames, bins), device="cuda")
+    ph = torch.empty_like(mag)
+    ff.stft_signals_device(p, d, mag, ph, None)
+    for s in range(2):
+        r = torch.empty((frames, bins), device="cuda")
+        i = torch.empty_like(r)
+        ff.stft_device(p, d[s].contiguous(), r, i, None, pad_mode=0)
+        torch.cuda.synchronize()
+        a = r.cpu().numpy() * np.exp(1j * i.cpu().numpy())
+        b = mag[s].cpu().numpy() * np.exp(1j * ph[s].cpu().numpy())
+        assert fp64.rel_l2(b, a) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("mode", ["complex", "polar", "magnitude"])
+def test_adjoint_vs_fp64_adjoint(ff, case, mode):
+    import torch
+    n, W, H, length = case
+    print(f"adjoint n={n} W={W} H={H} len={length} mode={mode}", flush=True)
+    p = make(ff, n, W, H, mode)
+    x = signals(3, length, seed=2 * n + W)
+    frames = ff.frame_count(p, length, 0)
+    bins = n // 2 + 1
+    rng = np.random.default_rng(n + H)
+    ga = rng.standard_normal((3, frames, bins))
+    gb = rng.standard_normal((3, frames, bins))
+    if mode != "complex":
+        for s in range(3):
+            A, _ = fp64.value_planes64(x[s], n, W, H, frames, win_n(n, W), "magnitude")
+            rms = np.sqrt(np.mean(A ** 2))
+            keep = A > 1e-2 * rms
+            ga[s] *= keep
+            gb[s] *= keep * A / rms
+    ga, gb = ga.astype(np.float32), gb.astype(np.float32)
+    d = torch.from_numpy(x).cuda()
+    g = torch.full_like(d, float("nan"))
+    ff.stft_adjoint_device(p, d, torch.from_numpy(ga).cuda(), torch.from_numpy(gb).cuda(), g)
+    torch.cuda.synchronize()
+    got = g.cpu().numpy()
+    for s in range(3):
+        want = fp64.stft_adjoint64(x[s], n, W, H, frames, win_n(n, W), mode, ga[s],
+                                   gb[s] if mode != "magnitude" else None)
+        assert fp64.rel_l2(got[s], want) <= 4 * fp64.tolerance(n), s
+    # a second run gives the same bits (fixed summation order)
+    g2 = torch.empty_like(d)
+    ff.stft_adjoint_device(p, d, torch.from_numpy(ga).cuda(), torch.from_numpy(gb).cuda(), g2)
+    torch.cuda.synchronize()
+    assert torch.equal(g, g2)
+
+
+def test_adjoint_null_gradients_and_no_window(ff):
+    import torch
+    n, W, H, length = 1024, 1024, 256, 6000
+    p = make(ff, n, W, H, "polar", window_kind=0)
+    x = signals(2, length, seed=77)
+    frames = ff.frame_count(p, length, 0)
+    bins = n // 2 + 1
+    gb = np.random.default_rng(1).standard_normal((2, frames, bins))
+    for s in range(2):
+        A, _ = fp64.value_planes64(x[s], n, W, H, frames, None, "magnitude")
+        rms = np.sqrt(np.mean(A ** 2))
+        gb[s] *= (A > 1e-2 * rms) * A / rms
+    gb = gb.astype(np.float32)
+    d = torch.from_numpy(x).cuda()
+    g = ff.stft_adjoint_device(p, d, None, torch.from_numpy(gb).cuda())
+    torch.cuda.synchronize()
+    for s in range(2):
+        want = fp64.stft_adjoint64(x[s], n, W, H, frames, None, "polar", np.zeros((frames, bins)), gb[s])
+        assert fp64.rel_l2(g[s].cpu().numpy(), want) <= 4 * fp64.tolerance(n)
+
+
+def test_adjoint_refuses_unsupported(ff):
+    import torch
+    d = torch.zeros((1, 10000), device="cuda")
+    p = make(ff, 4096, 4096, 1024, "polar")  # N outside the signal kernels
+    with pytest.raises(RuntimeError):
+        ff.stft_adjoint_device(p, d, torch.zeros((1, 7, 2049), device="cuda"), None)
+    p = make(ff, 1024, 1024, 128, "polar")  # 8 frames overlap a sample: ring too small
+    with pytest.raises(RuntimeError):
+        ff.stft_adjoint_device(p, d, torch.zeros((1, 71, 513), device="cuda"), None)
+
+
+def test_config4_autograd_gradient(ff):
+    """32 signals of 2^18 samples, window 2048, hop 512, polar; loss = sum (|X| - |Y|)^2
+    for a seeded target y; gradient through torch.autograd against FP64 central finite
+    differences on 64 random coordinates and the FP64 reference gradient."""
+    import torch
+    from fftfree_amd.autograd import magnitude_loss, stft_polar
+    n, W, H, B, length = 2048, 2048, 512, 32, 1 << 18
+    x = signals(B, length, seed=1000)
+    y = signals(B, length, seed=2000)
+    xt = torch.from_numpy(x).cuda().requires_grad_(True)
+    with torch.no_grad():
+        ymag, _ = stft_polar(torch.from_numpy(y).cuda(), n, W, H)
+    loss = magnitude_loss(xt, ymag, n, W, H)
+    loss.backward()
+    torch.cuda.synchronize()
+    grad = xt.grad.cpu().numpy()
+    frames = ymag.shape[1]
+    assert frames == 509
+    ym = ymag.cpu().numpy().astype(np.float64)
+    win = win_n(n, W)
+    # FP64 reference gradient on two of the signals
+    for s in (0, 17):
+        A, _ = fp64.value_planes64(x[s], n, W, H, frames, win, "magnitude")
+        want = fp64.stft_adjoint64(x[s], n, W, H, frames, win, "magnitude", 2 * (A - ym[s]))
+        err = fp64.rel_l2(grad[s], want)
+        print(f"config4 signal {s}: grad rel_l2 = {err:.3e}")
+        assert err <= 4 * fp64.tolerance(n)
+    # central finite differences of the FP64 loss, recomputing only the frames that
+    # contain the perturbed sample
+    rng = np.random.default_rng(4)
+    picks = [(int(rng.integers(B)), int(rng.integers(length))) for _ in range(64)]
+    h = 1e-3
+    fds, gs = [], []
+    for s, j in picks:
+        f_lo = max(0, (j - W) // H + 1)
+        f_hi = min(frames - 1, j // H)
+        if f_lo > f_hi:  # sample in no frame: zero gradient
+            assert grad[s, j] == 0.0
+            continue
+        seg0 = f_lo * H
+        seg = x[s, seg0:min(length, f_hi * H + W)].astype(np.float64)
+        nf = f_hi - f_lo + 1
+
+        def local_loss(v):
+            A, _ = fp64.value_planes64(v, n, W, H, nf, win, "magnitude")
+            return np.sum((A - ym[s, f_lo:f_hi + 1]) ** 2)
+        sp, sm = seg.copy(), seg.copy()
+        sp[j - seg0] += h
+        sm[j - seg0] -= h
+        fds.append((local_loss(sp) - local_loss(sm)) / (2 * h))
+        gs.append(grad[s, j])
+    fds, gs = np.array(fds), np.array(gs)
+    rms = np.sqrt(np.mean(fds ** 2))
+    assert np.all(np.abs(gs - fds) <= 1e-3 * np.abs(fds) + 1e-5 * rms), np.max(np.abs(gs - fds) / (np.abs(fds) + rms))

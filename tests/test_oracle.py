@@ -233,3 +233,69 @@ def test_cpu_ref_stft_planes_vs_fp64(transform, flags, kernel, inverse):
     else:
         assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= tol
     assert fp64.rel_l2(mg, omg) <= tol
+
+
+# -- the FP64 adjoint (gradient) oracle, pinned by finite differences ---------------------
+
+def _adj_case(seed, L=300, N=64, W=64, H=16, win_kind=1):
+    rng = np.random.default_rng(seed)
+    pcm = rng.standard_normal(L).astype(np.float32)
+    frames = fp64.frame_count(L, N, W, H, 0, 1)
+    win = fp64.build_window(win_kind, W) if win_kind else None
+    win_n = None if win is None else np.concatenate([win, np.ones(N - W, np.float32)])
+    return rng, pcm, frames, win_n
+
+
+def _fd_check(pcm, N, W, H, frames, win, mode, ga, gb, grad, coords, h=1e-6):
+    def loss(x):
+        A, B = fp64.value_planes64(x, N, W, H, frames, win, mode)
+        return np.sum(ga * A) + np.sum(gb * B)
+    x0 = pcm.astype(np.float64)
+    worst = 0.0
+    for j in coords:
+        xp = x0.copy(); xp[j] += h
+        xm = x0.copy(); xm[j] -= h
+        fd = (loss(xp) - loss(xm)) / (2 * h)
+        worst = max(worst, abs(fd - grad[j]) / (1e-6 + abs(fd)))
+    return worst
+
+
+@pytest.mark.parametrize("mode", ["complex", "polar", "magnitude"])
+@pytest.mark.parametrize("geom", [(64, 64, 16), (64, 48, 12), (32, 32, 8)])
+def test_adjoint_oracle_matches_finite_differences(mode, geom):
+    N, W, H = geom
+    rng, pcm, frames, win = _adj_case(7 + N + W, N=N, W=W, H=H)
+    bins = N // 2 + 1
+    ga = rng.standard_normal((frames, bins))
+    gb = rng.standard_normal((frames, bins))
+    g = fp64.stft_adjoint64(pcm, N, W, H, frames, win, mode, ga, gb)
+    coords = rng.choice(pcm.size, 40, replace=False)
+    assert _fd_check(pcm, N, W, H, frames, win, mode, ga, gb, g, coords) < 1e-5
+
+
+def test_adjoint_oracle_fd_check_has_teeth():
+    # plausible mistakes planted in the oracle: the window left out of the overlap-add,
+    # and the complex-output V used for a polar loss; the FD check must reject both
+    N, W, H = 64, 64, 16
+    rng, pcm, frames, win = _adj_case(3)
+    bins = N // 2 + 1
+    ga = rng.standard_normal((frames, bins))
+    gb = rng.standard_normal((frames, bins))
+    coords = rng.choice(pcm.size, 20, replace=False)
+    good = fp64.stft_adjoint64(pcm, N, W, H, frames, win, "polar", ga, gb)
+    no_win = fp64.stft_adjoint64(pcm, N, W, H, frames, win, "polar", ga, gb, ola_window=False)
+    wrong_v = fp64.stft_adjoint64(pcm, N, W, H, frames, win, "complex", ga, gb)
+    assert _fd_check(pcm, N, W, H, frames, win, "polar", ga, gb, good, coords) < 1e-5
+    assert _fd_check(pcm, N, W, H, frames, win, "polar", ga, gb, no_win, coords) > 1e-2
+    assert _fd_check(pcm, N, W, H, frames, win, "polar", ga, gb, wrong_v, coords) > 1e-2
+
+
+def test_adjoint_oracle_agrees_with_direct_magnitude_gradient():
+    # the direct O(N^2) restatement (stft_magnitude_grad) and the irfft form agree
+    N, W, H = 64, 64, 16
+    rng, pcm, frames, win = _adj_case(11)
+    y = np.abs(rng.standard_normal((frames, N // 2 + 1)))
+    A, _ = fp64.value_planes64(pcm, N, W, H, frames, win, "magnitude")
+    direct = fp64.stft_magnitude_grad(pcm, y, N, W, H, frames, win)
+    fast = fp64.stft_adjoint64(pcm, N, W, H, frames, win, "magnitude", 2 * (A - y))
+    assert np.max(np.abs(direct - fast)) <= 1e-9 * np.max(np.abs(direct))
