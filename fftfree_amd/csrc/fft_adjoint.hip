@@ -37,10 +37,8 @@ using namespace dev;
 
 namespace {
 
-constexpr int kWaves = 4;       // frames in flight per workgroup (one per wave)
-constexpr int kRing = 8;        // LDS ring slots (4 current + 4 previous frames)
-constexpr int kFwdTile = 32;    // frames per forward workgroup
-constexpr int kAdjSteps = 8;    // adjoint steps per workgroup (kWaves frame positions each)
+constexpr int kWaves = 4;       // forward: frames in flight per workgroup (one per wave)
+constexpr int kFwdTile = 32;    // forward: frames per workgroup
 
 // Upstream-gradient spectrum V for one bin (see the header comment).
 template <int MODE>
@@ -61,47 +59,73 @@ __device__ __forceinline__ float2 grad_spectrum(float2 X, float ga, float gb, bo
   }
 }
 
-// One frame's samples, windowed, in first-pass order: z[m] = x[2m] w[2m] + i x[2m+1] w[2m+1].
+// One frame's samples, windowed, in first-pass order: z[m] = x[2m] w[2m] + i x[2m+1] w[2m+1],
+// zero past the `cnt` valid samples. Window taps come from LDS (wl, N floats).
 template <int L, int E, int R0>
 __device__ __forceinline__ void load_frame(float2* v, int t, const float* __restrict__ x, size_t len, size_t start,
-                                           int window, const float* __restrict__ win) {
+                                           int window, const float* wl) {
   constexpr int T = L / E;
-  constexpr int N = 2 * L;
   const size_t rem = len - start;
-  const size_t cnt = rem < (size_t)window ? rem : (size_t)window;
+  const int cnt = (int)(rem < (size_t)window ? rem : (size_t)window);
   const float* __restrict__ src = x + start;
-  if (cnt == (size_t)N && ((((uintptr_t)src) & 7) == 0)) {
-    const float2* __restrict__ s2 = reinterpret_cast<const float2*>(src);
-    const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win);
 #pragma unroll
-    for (int q = 0; q < E / R0; ++q)
+  for (int q = 0; q < E / R0; ++q)
 #pragma unroll
-      for (int r = 0; r < R0; ++r) {
-        const int m = t + q * T + r * (L / R0);
-        float2 z = s2[m];
-        if (win) {
-          const float2 w = w2[m];
-          z.x *= w.x;
-          z.y *= w.y;
-        }
-        v[q * R0 + r] = z;
-      }
-  } else {
-#pragma unroll
-    for (int q = 0; q < E / R0; ++q)
-#pragma unroll
-      for (int r = 0; r < R0; ++r) {
-        const int m = t + q * T + r * (L / R0);
-        v[q * R0 + r] = make_float2(frame_sample(src, win, cnt, 2 * m), frame_sample(src, win, cnt, 2 * m + 1));
-      }
-  }
+    for (int r = 0; r < R0; ++r) {
+      const int m = t + q * T + r * (L / R0);
+      const float a = 2 * m < cnt ? src[2 * m] : 0.0f;
+      const float b = 2 * m + 1 < cnt ? src[2 * m + 1] : 0.0f;
+      v[q * R0 + r] = make_float2(a * wl[2 * m], b * wl[2 * m + 1]);
+    }
 }
 
-// Forward real DFT of one frame by one wave: on return buf holds Z (natural order) and
-// X1[j] = X_p, X2[j] = X_{L-p} for p = t + j*T (j < JF), Xm = X_{L/2}.
-template <int L, int E, int... Rs>
-__device__ __forceinline__ void frame_spectrum(float2* v, int t, float2* buf, const float2* twr, const float2* wsp,
-                                               float2 wmid, float2* X1, float2* X2, float2& Xm) {
+// Raw samples of one frame in first-pass order (zero past `cnt`), for prefetching; the
+// window is applied at use (window_frame).
+template <int L, int E, int R0>
+__device__ __forceinline__ void fetch_frame(float2* v, int t, const float* __restrict__ x, size_t len, size_t start,
+                                            int window, bool valid) {
+  constexpr int T = L / E;
+  int cnt = 0;
+  if (valid) {
+    const size_t rem = len - start;
+    cnt = (int)(rem < (size_t)window ? rem : (size_t)window);
+  }
+  const float* __restrict__ src = x + (valid ? start : 0);
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int m = t + q * T + r * (L / R0);
+      v[q * R0 + r] = make_float2(2 * m < cnt ? src[2 * m] : 0.0f, 2 * m + 1 < cnt ? src[2 * m + 1] : 0.0f);
+    }
+}
+
+template <int L, int E, int R0>
+__device__ __forceinline__ void window_frame(float2* v, int t, const float* wl) {
+  constexpr int T = L / E;
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int m = t + q * T + r * (L / R0);
+      v[q * R0 + r].x *= wl[2 * m];
+      v[q * R0 + r].y *= wl[2 * m + 1];
+    }
+}
+
+// Window taps into LDS (1.0 when windows are off); the caller synchronises.
+template <int N>
+__device__ __forceinline__ void stage_window(float* wl, const float* __restrict__ win) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) wl[i] = win ? win[i] : 1.0f;
+}
+
+// Forward real DFT of one frame by one wave, then the split into bins: pair(j, p, X_p,
+// X_{L-p}) runs for p = t + j*T < L/2 and mid(X_{L/2}) on lane 0 only. Each call reads
+// only Z_p and Z_{L-p} (lane 0: also Z_{L/2}), so a callback may overwrite exactly
+// those buf entries in place.
+template <int L, int E, int... Rs, class Pair, class Mid>
+__device__ __forceinline__ void frame_bins(float2* v, int t, float2* buf, const float2* twr, const float2* wsp,
+                                           float2 wmid, Pair&& pair, Mid&& mid) {
   constexpr int T = L / E;
   constexpr int JF = (L / 2) / T;
   constexpr int Rl = LastOf<Rs...>::value;
@@ -116,18 +140,21 @@ __device__ __forceinline__ void frame_spectrum(float2* v, int t, float2* buf, co
     const float2 e = c_scale(c_add(zp, zc), 0.5f);
     const float2 d = c_sub(zp, zc);
     const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
-    X1[j] = c_add(e, wo);
-    X2[j] = c_conj(c_sub(e, wo));
+    float2 X1 = c_add(e, wo);
+    float2 X2 = c_conj(c_sub(e, wo));
     if (p == 0) {  // bins 0 and L are real
-      X1[j].y = 0.0f;
-      X2[j].y = 0.0f;
+      X1.y = 0.0f;
+      X2.y = 0.0f;
     }
+    pair(j, p, X1, X2);
   }
-  const float2 zm = buf[pidx(L / 2)];
-  const float2 zc = c_conj(zm);
-  const float2 e = c_scale(c_add(zm, zc), 0.5f);
-  const float2 d = c_sub(zm, zc);
-  Xm = c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x)));
+  if (t == 0) {
+    const float2 zm = buf[pidx(L / 2)];
+    const float2 zc = c_conj(zm);
+    const float2 e = c_scale(c_add(zm, zc), 0.5f);
+    const float2 d = c_sub(zm, zc);
+    mid(c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x))));
+  }
 }
 
 template <int L, int E, int MODE, int... Rs>
@@ -140,10 +167,13 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
   static_assert(T == 64, "one transform per wave");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
   __shared__ float2 lds[kWaves][LP];
+  __shared__ float wl[2 * L];
 
   const int wave = threadIdx.x >> 6;
   const int t = threadIdx.x & 63;
   float2* buf = &lds[wave][0];
+  stage_window<2 * L>(wl, a.win);
+  __syncthreads();
   const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
   const size_t sig = tile / (size_t)a.tiles;
   const size_t f0 = (tile - sig * a.tiles) * kFwdTile;
@@ -160,9 +190,7 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
     const size_t f = f0 + (size_t)(i * kWaves + wave);
     if (f >= a.frames) break;  // uniform per wave
     float2 v[E];
-    load_frame<L, E, FirstOf<Rs...>::value>(v, t, x, a.len, f * (size_t)a.hop, a.window, a.win);
-    float2 X1[JF], X2[JF], Xm;
-    frame_spectrum<L, E, Rs...>(v, t, buf, twr, wsp, wmid, X1, X2, Xm);
+    load_frame<L, E, FirstOf<Rs...>::value>(v, t, x, a.len, f * (size_t)a.hop, a.window, wl);
     const size_t row = (sig * a.frames + f) * (size_t)BINS;
     float* __restrict__ ore = a.o_re ? a.o_re + row : nullptr;
     float* __restrict__ oim = a.o_im ? a.o_im + row : nullptr;
@@ -174,45 +202,62 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
       if (oim) oim[k] = plane_value<MODE, 1>(A, B);
       if (omg) omg[k] = plane_value<MODE, 2>(A, B);
     };
-#pragma unroll
-    for (int j = 0; j < JF; ++j) {
-      const int p = t + j * T;
-      put(p, X1[j]);
-      put(L - p, X2[j]);
-    }
-    if (t == 0) put(L / 2, Xm);
+    frame_bins<L, E, Rs...>(
+        v, t, buf, twr, wsp, wmid,
+        [&](int, int p, float2 X1, float2 X2) {
+          put(p, X1);
+          put(L - p, X2);
+        },
+        [&](float2 Xm) { put(L / 2, Xm); });
     wave_lds_fence();  // buf is rewritten by the next frame
   }
 }
 
-template <int L, int E, int MODE, int... Rs>
+// Adjoint kernel. T = L/E threads transform one frame; a step transforms FPS = 256/T
+// consecutive frame positions, one per thread group, with workgroup barriers between
+// passes. Per step:
+//  1. each group recomputes X for its frame, forms V, overwrites Z with conj(Y) in place
+//     (each thread owns the pair p, L-p), inverse-transforms and leaves r_f in its buffer;
+//  2. every sample the step's frames touch gets their windowed r_f added to an LDS
+//     accumulator ring, frames in ascending order (a fixed order: deterministic);
+//  3. the FPS*H samples that no later frame touches are written out and their ring
+//     entries cleared for reuse.
+// Warm-up steps run the frames before the workgroup's run (they touch its first
+// samples) without writing anything of their own.
+template <int L, int E, int MODE, int ACCM, int... Rs>
 __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
+  constexpr int FPS = 256 / T;
   constexpr int NP = sizeof...(Rs);
   constexpr int R0 = FirstOf<Rs...>::value;
   constexpr int Rl = LastOf<Rs...>::value;
   constexpr int LP = padded_len<L>();
   constexpr int JF = (L / 2) / T;
   constexpr int BINS = L + 1;
-  static_assert(T == 64, "one transform per wave");
+  constexpr int ACC = ACCM * N;  // accumulator ring (samples); the host checks W + FPS*H <= ACC
+  static_assert(T * FPS == 256 && JF * T == L / 2, "thread layout");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
-  __shared__ float2 ring[kRing][LP];
+  __shared__ float2 fbuf[FPS][LP];
+  __shared__ float acc[ACC];
+  __shared__ float wl[N];
 
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int t = tid & 63;
+  const int slot = tid / T;
+  const int t = tid - slot * T;
+  float2* buf = &fbuf[slot][0];
   const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
   const size_t sig = tile / (size_t)a.tiles;
-  const long long f0 = (long long)(tile - sig * a.tiles) * (kWaves * kAdjSteps);
+  const long long steps = a.steps;
+  const long long f0 = (long long)(tile - sig * a.tiles) * FPS * steps;
   const float* __restrict__ x = a.pcm + sig * a.len;
   const float* __restrict__ ga = a.g_mag ? a.g_mag + sig * a.frames * BINS : nullptr;
   const float* __restrict__ gb = a.g_phase ? a.g_phase + sig * a.frames * BINS : nullptr;
   float* __restrict__ gx = a.grad + sig * a.len;
-  const float2* __restrict__ w2 = reinterpret_cast<const float2*>(a.win);
   const long long frames = (long long)a.frames;
+  const long long len = (long long)a.len;
   const long long hop = a.hop;
-  const long long W = a.window;
+  const int W = a.window;
 
   float2 twr[E * NP];
   StockhamPlan<L, E, -1, Rs...>::load_twiddles(twr, t, a.tw, 2);
@@ -220,89 +265,144 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
 #pragma unroll
   for (int j = 0; j < JF; ++j) wsp[j] = load_tw<-1>(a.tw, t + j * T);
   const float2 wmid = load_tw<-1>(a.tw, L / 2);
+  stage_window<N>(wl, a.win);
+  for (int i = tid; i < ACC; i += 256) acc[i] = 0.0f;
+  __syncthreads();
 
-  for (int step = -1; step < kAdjSteps; ++step) {
-    const long long fbase = f0 + (long long)step * kWaves;
-    const long long f = fbase + wave;
-    float2* buf = &ring[(int)(f & (kRing - 1))][0];
-    if (f >= 0 && f < frames) {  // uniform per wave
-      float2 v[E];
-      load_frame<L, E, R0>(v, t, x, a.len, (size_t)(f * hop), a.window, a.win);
-      float2 X1[JF], X2[JF], Xm;
-      frame_spectrum<L, E, Rs...>(v, t, buf, twr, wsp, wmid, X1, X2, Xm);
-      const size_t row = (size_t)f * BINS;
-      auto gA = [&](int k) { return ga ? ga[row + k] : 0.0f; };
-      auto gB = [&](int k) { return gb ? gb[row + k] : 0.0f; };
+  const long long warm = ((W - 1) / hop + FPS - 1) / FPS;  // steps of frames before f0
+  auto frame_valid = [&](long long f) { return f >= 0 && f < frames; };
+  // samples of the first step's frame; later steps' samples are prefetched a step ahead
+  float2 nxt[E];
+  {
+    const long long f = f0 - warm * FPS + slot;
+    fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(f) ? (size_t)(f * hop) : 0, W, frame_valid(f));
+  }
+  for (long long step = -warm; step < steps; ++step) {
+    const long long fbase = f0 + step * FPS;
+    const long long f = fbase + slot;
+    const bool valid = frame_valid(f);  // invalid groups run on zeros (barriers stay uniform)
+
+    // upstream gradients of this frame's bins, requested before the transform
+    const size_t row = valid ? (size_t)f * BINS : 0;
+    float gAp[JF], gAq[JF], gBp[JF], gBq[JF];
 #pragma unroll
-      for (int j = 0; j < JF; ++j) {
-        const int p = t + j * T;
-        X1[j] = grad_spectrum<MODE>(X1[j], gA(p), gB(p), p == 0);
-        X2[j] = grad_spectrum<MODE>(X2[j], gA(L - p), gB(L - p), p == 0);
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      gAp[j] = (ga && valid) ? ga[row + p] : 0.0f;
+      gAq[j] = (ga && valid) ? ga[row + L - p] : 0.0f;
+      gBp[j] = (gb && valid) ? gb[row + p] : 0.0f;
+      gBq[j] = (gb && valid) ? gb[row + L - p] : 0.0f;
+    }
+    const float gAm = (t == 0 && ga && valid) ? ga[row + L / 2] : 0.0f;
+    const float gBm = (t == 0 && gb && valid) ? gb[row + L / 2] : 0.0f;
+
+    // 1. r_f into buf
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = nxt[e];
+    window_frame<L, E, R0>(v, t, wl);
+    float2* fb;
+    StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+    store_natural<L, E, Rl>(v, t, buf);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      const float2 zp = buf[pidx(p)];
+      const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
+      const float2 e = c_scale(c_add(zp, zc), 0.5f);
+      const float2 d = c_sub(zp, zc);
+      const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
+      float2 X1 = c_add(e, wo);
+      float2 X2 = c_conj(c_sub(e, wo));
+      if (p == 0) {  // bins 0 and L are real
+        X1.y = 0.0f;
+        X2.y = 0.0f;
       }
-      Xm = grad_spectrum<MODE>(Xm, gA(L / 2), gB(L / 2), false);
-      wave_lds_fence();  // every lane has read Z
-      // conj(Y) into buf, natural order
-#pragma unroll
-      for (int j = 0; j < JF; ++j) {
-        const int p = t + j * T;
-        const float2 Vp = X1[j], Vq = X2[j];  // V_p, V_{L-p}
-        if (p == 0) {
-          buf[pidx(0)] = make_float2(Vp.x + Vq.x, -(Vp.x - Vq.x));
-        } else {
-          const float2 ep = c_conj(wsp[j]);  // e^{+2 pi i p/N}
-          const float2 s = c_scale(c_add(Vp, c_conj(Vq)), 0.5f);
-          const float2 d = c_scale(c_sub(Vp, c_conj(Vq)), 0.5f);
-          const float2 ed = c_mul(ep, d);
-          const float2 Yp = make_float2(s.x - ed.y, s.y + ed.x);  // s + i e d
-          // Y_{L-p} with e' = -conj(e): (V_q + conj V_p)/2 + i e' (V_q - conj V_p)/2
-          const float2 s2 = c_conj(s);
-          const float2 d2 = c_scale(c_conj(d), -1.0f);
-          const float2 ed2 = c_mul(make_float2(-ep.x, ep.y), d2);
-          const float2 Yq = make_float2(s2.x - ed2.y, s2.y + ed2.x);
-          buf[pidx(p)] = c_conj(Yp);
-          buf[pidx(L - p)] = c_conj(Yq);
-        }
+      const float2 Vp = grad_spectrum<MODE>(X1, gAp[j], gBp[j], p == 0);
+      const float2 Vq = grad_spectrum<MODE>(X2, gAq[j], gBq[j], p == 0);  // V_{L-p}
+      if (p == 0) {
+        buf[pidx(0)] = make_float2(Vp.x + Vq.x, -(Vp.x - Vq.x));
+      } else {
+        const float2 ep = c_conj(wsp[j]);  // e^{+2 pi i p/N}
+        const float2 sv = c_scale(c_add(Vp, c_conj(Vq)), 0.5f);
+        const float2 dv = c_scale(c_sub(Vp, c_conj(Vq)), 0.5f);
+        const float2 ed = c_mul(ep, dv);
+        const float2 Yp = make_float2(sv.x - ed.y, sv.y + ed.x);  // s + i e d
+        // Y_{L-p} with e' = -conj(e): (V_q + conj V_p)/2 + i e' (V_q - conj V_p)/2
+        const float2 s2 = c_conj(sv);
+        const float2 d2 = c_scale(c_conj(dv), -1.0f);
+        const float2 ed2 = c_mul(make_float2(-ep.x, ep.y), d2);
+        const float2 Yq = make_float2(s2.x - ed2.y, s2.y + ed2.x);
+        buf[pidx(p)] = c_conj(Yp);
+        buf[pidx(L - p)] = c_conj(Yq);
       }
-      if (t == 0) {
-        // p = L/2: e = i, so Y = Re V + i * i * (i Im V) = conj(V)
-        buf[pidx(L / 2)] = Xm;  // conj(Y) = V
-      }
-      wave_lds_fence();
-#pragma unroll
-      for (int q = 0; q < E / R0; ++q)
-#pragma unroll
-        for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
-      wave_lds_fence();
-      WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
-      wave_lds_fence();
-      // y = conj(result): r[2m] = v.x, r[2m+1] = -v.y; windowed contributions into the slot
-#pragma unroll
-      for (int q = 0; q < E / Rl; ++q)
-#pragma unroll
-        for (int r = 0; r < Rl; ++r) {
-          const int m = t + q * T + r * (L / Rl);
-          const float2 y = v[q * Rl + r];
-          const float2 w = w2 ? w2[m] : make_float2(1.0f, 1.0f);
-          buf[pidx(m)] = make_float2(y.x * w.x, -y.y * w.y);
-        }
+    }
+    if (t == 0) {  // p = L/2: e = i, so Y = Re V + i * i * (i Im V) = conj(V): conj(Y) = V
+      const float2 zm = buf[pidx(L / 2)];
+      const float2 zc = c_conj(zm);
+      const float2 e = c_scale(c_add(zm, zc), 0.5f);
+      const float2 d = c_sub(zm, zc);
+      const float2 Xm = c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x)));
+      buf[pidx(L / 2)] = grad_spectrum<MODE>(Xm, gAm, gBm, false);
     }
     __syncthreads();
-    if (step >= 0) {
-      // samples [fbase*H, (fbase+4)*H): covered by frames fbase-4 .. fbase+3, all in the ring
-      const long long j0 = fbase * hop;
-      for (int u = tid; u < kWaves * hop; u += 256) {
-        const long long j = j0 + u;
-        if (j >= (long long)a.len) break;
-        long long fmax = j / hop;
-        if (fmax > frames - 1) fmax = frames - 1;
-        const long long fmin = j >= W ? (j - W + hop) / hop : 0;  // ceil((j-W+1)/H)
-        float sum = 0.0f;
-        for (long long g = fmin; g <= fmax; ++g) {
-          const int n = (int)(j - g * hop);
-          const float2 c = ring[(int)(g & (kRing - 1))][pidx(n >> 1)];
-          sum += (n & 1) ? c.y : c.x;
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
+    __syncthreads();  // every value read before the first in-place scatter
+    StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+    // y = conj(result): r[2m] = v.x, r[2m+1] = -v.y
+#pragma unroll
+    for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+      for (int r = 0; r < Rl; ++r) {
+        const float2 y = v[q * Rl + r];
+        buf[pidx(t + q * T + r * (L / Rl))] = make_float2(y.x, -y.y);
+      }
+    if (step + 1 < steps) {  // the next step's samples, in flight during steps 2 and 3
+      const long long fn = f + FPS;
+      fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(fn) ? (size_t)(fn * hop) : 0, W, frame_valid(fn));
+    }
+    __syncthreads();
+
+    // 2. accumulate the step's frames, ascending, over the samples they touch
+    const long long j0 = fbase * hop;
+    const int ulo = j0 < 0 ? (int)(-j0) : 0;  // the step's samples clipped to [0, len)
+    const int rb = (int)(j0 & (ACC - 1));     // ring position of sample j0
+    {
+      const int span = (FPS - 1) * (int)hop + W;
+      const long long room = len - j0;
+      const int uhi = room < span ? (int)room : span;
+      bool vg[FPS];
+#pragma unroll
+      for (int g = 0; g < FPS; ++g) vg[g] = fbase + g >= 0 && fbase + g < frames;
+      for (int u = ulo + tid; u < uhi; u += 256) {
+        const int ri = (rb + u) & (ACC - 1);
+        float sum = acc[ri];
+#pragma unroll
+        for (int g = 0; g < FPS; ++g) {
+          const int n = u - g * (int)hop;
+          if (vg[g] && n >= 0 && n < W) {
+            const float2 c = fbuf[g][pidx(n >> 1)];
+            sum = __builtin_fmaf((n & 1) ? c.y : c.x, wl[n], sum);
+          }
         }
-        gx[j] = sum;
+        acc[ri] = sum;
+      }
+    }
+    __syncthreads();
+
+    // 3. samples [fbase*H, (fbase+FPS)*H) are complete: write (own steps) and clear
+    {
+      const long long room = len - j0;
+      const int uhi = room < FPS * (int)hop ? (int)room : FPS * (int)hop;
+      float* __restrict__ out = gx + (step >= 0 ? j0 : 0);
+      for (int u = ulo + tid; u < uhi; u += 256) {
+        const int ri = (rb + u) & (ACC - 1);
+        if (step >= 0) out[u] = acc[ri];
+        acc[ri] = 0.0f;
       }
     }
     __syncthreads();
@@ -330,34 +430,58 @@ hipError_t signals_launch(const SignalArgs& a, int mode, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int L, int E, int... Rs>
-hipError_t adjoint_launch(const SignalArgs& a, int mode, hipStream_t s) {
+// Sizes the grid to the chip: each workgroup runs `steps` steps, chosen so that the
+// workgroups of all signals fill the resident capacity (CUs x workgroups per CU) about
+// once. Longer runs cost fewer warm-up frames; more workgroups hide more latency.
+template <int L, int E, int ACCM, int... Rs>
+hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
+  constexpr int FPS = 256 / (L / E);
+  const void* kern = nullptr;
+  switch (mode) {
+    case kOutComplex: kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutComplex, ACCM, Rs...>); break;
+    case kOutMagnitude:
+      kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutMagnitude, ACCM, Rs...>);
+      break;
+    case kOutPolar: kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutPolar, ACCM, Rs...>); break;
+    default: return hipErrorInvalidValue;
+  }
+  int dev = 0, cus = 0, occ = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0);
+  if (e != hipSuccess) return e;
+  const size_t capacity = (size_t)(cus > 0 ? cus : 1) * (size_t)(occ > 0 ? occ : 1);
+  const size_t positions = (a.len + a.hop - 1) / a.hop;  // frame positions covering every sample
+  const size_t per_signal = (capacity + a.signals - 1) / a.signals;
+  size_t steps = (positions + FPS * per_signal - 1) / (FPS * per_signal);
+  steps = steps < 4 ? 4 : (steps > 64 ? 64 : steps);
+  a.steps = (int)steps;
+  a.tiles = (int)((positions + FPS * steps - 1) / (FPS * steps));
   const size_t grid = a.signals * (size_t)a.tiles;
   if (grid == 0) return hipSuccess;
   if (grid > 0x7fffffffull) return hipErrorInvalidValue;
-  switch (mode) {
-    case kOutComplex:
-      hipLaunchKernelGGL((stft_adjoint_kernel<L, E, kOutComplex, Rs...>), dim3((unsigned)grid), dim3(256), 0, s, a);
-      break;
-    case kOutMagnitude:
-      hipLaunchKernelGGL((stft_adjoint_kernel<L, E, kOutMagnitude, Rs...>), dim3((unsigned)grid), dim3(256), 0, s, a);
-      break;
-    case kOutPolar:
-      hipLaunchKernelGGL((stft_adjoint_kernel<L, E, kOutPolar, Rs...>), dim3((unsigned)grid), dim3(256), 0, s, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  void* args[] = {&a};
+  return hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
+}
+
+// The smallest accumulator ring (2N, 4N or 8N samples) that holds one step's span.
+template <int L, int E, int... Rs>
+hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s) {
+  if (span <= 4 * L) return adjoint_launch_acc<L, E, 2, Rs...>(a, mode, s);
+  if (span <= 8 * L) return adjoint_launch_acc<L, E, 4, Rs...>(a, mode, s);
+  return adjoint_launch_acc<L, E, 8, Rs...>(a, mode, s);
 }
 
 }  // namespace
 
+// Frames per adjoint step for each N (256 / threads per frame).
+int adjoint_fps(int n) { return n == 2048 ? 2 : 4; }
+
 bool signals_supported(int n, int window, int hop, bool adjoint) {
   if (n != 512 && n != 1024 && n != 2048) return false;
   if (window <= 0 || window > n || hop <= 0) return false;
-  // the adjoint's ring holds the 4 frames before a step: a sample may span at most 5
-  return !adjoint || (window - 1) / hop <= kRing - kWaves;
+  // the adjoint's accumulator ring (at most 8N samples) must hold one step's span
+  return !adjoint || (long long)window + (long long)adjoint_fps(n) * hop <= 8LL * n;
 }
 
 hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s) {
@@ -374,14 +498,14 @@ hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s) {
 hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s) {
   const int n = a.n;
   if (!signals_supported(n, a.window, a.hop, true) || a.len == 0 || !a.grad) return hipErrorInvalidValue;
-  // frame positions up to ceil(len / hop) so that every sample is written
-  const size_t positions = (a.len + a.hop - 1) / a.hop;
-  const size_t per = (size_t)kWaves * kAdjSteps;
-  a.tiles = (int)((positions + per - 1) / per);
+  const int fps = adjoint_fps(n);
+  const int span = a.window + fps * a.hop;
+  // with no phase gradient, polar V = Ga X/|X| is the magnitude map (fewer registers)
+  if (mode == kOutPolar && !a.g_phase) mode = kOutMagnitude;
   switch (n) {
-    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, mode, s);
-    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, mode, s);
-    default: return adjoint_launch<1024, 16, 16, 16, 4>(a, mode, s);
+    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, span, mode, s);
+    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, span, mode, s);
+    default: return adjoint_launch<1024, 8, 8, 8, 8, 2>(a, span, mode, s);
   }
 }
 

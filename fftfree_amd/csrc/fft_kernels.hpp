@@ -66,7 +66,8 @@ struct SignalArgs {
   const float* g_mag = nullptr;    // adjoint: gradient of plane re (|X| when polar/magnitude)
   const float* g_phase = nullptr;  // adjoint: gradient of plane im (arg X when polar), may be nullptr
   float* grad = nullptr;           // adjoint output: dL/dpcm, signals*len floats
-  int tiles = 0;                   // set by the launchers
+  int tiles = 0;                   // set by the launchers: workgroups per signal
+  int steps = 0;                   // adjoint: steps per workgroup
 };
 
 // Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
@@ -103,7 +104,8 @@ int large_variant_count(size_t n);
 bool large_variant_desc(size_t n, int variant, char* buf, size_t len);
 
 // Signals STFT and adjoint: N in {512, 1024, 2048}, forward R2C half spectrum, output
-// mode kOutComplex / kOutMagnitude / kOutPolar; the adjoint needs (window-1)/hop <= 4.
+// mode kOutComplex / kOutMagnitude / kOutPolar; the adjoint needs window + k*hop <= 8N
+// (k = 2 for N = 2048, else 4).
 bool signals_supported(int n, int window, int hop, bool adjoint);
 hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s);
 hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s);

@@ -153,7 +153,8 @@ size_t fft_stft_signals_device(void* handle, const float* d_pcm, size_t signals,
  * respect to the first two output planes (re and im; |X| and arg X when store_polar;
  * |X| when reduce_magnitude, where d_grad_imag is ignored), laid out like those planes,
  * writes dL/dpcm (signals*pcm_len floats) to d_grad_pcm. Each frame's spectrum is
- * recomputed from d_pcm. Either gradient may be NULL (zero). Needs (window-1)/hop <= 4.
+ * recomputed from d_pcm. Either gradient may be NULL (zero). Needs window + k*hop <= 8N
+ * (k = 2 for N = 2048, 4 otherwise).
  * Deterministic (fixed summation order). Returns frames per signal, 0 on error. */
 size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
                                const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,

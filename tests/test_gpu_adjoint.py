@@ -46,7 +46,7 @@ def win_n(n, W, kind=1):
 
 
 CASES = [(512, 512, 128, 5000), (1024, 1024, 256, 9000), (2048, 2048, 512, 20000), (1024, 768, 200, 7001),
-         (2048, 2048, 512, 1500)]
+         (2048, 2048, 512, 1500), (1024, 1024, 512, 9000), (1024, 1024, 1024, 9000), (512, 256, 700, 6000)]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -162,9 +162,9 @@ def test_adjoint_refuses_unsupported(ff):
     p = make(ff, 4096, 4096, 1024, "polar")  # N outside the signal kernels
     with pytest.raises(RuntimeError):
         ff.stft_adjoint_device(p, d, torch.zeros((1, 7, 2049), device="cuda"), None)
-    p = make(ff, 1024, 1024, 128, "polar")  # 8 frames overlap a sample: ring too small
+    p = make(ff, 1024, 1024, 2048, "polar")  # one step spans more than the 8N ring
     with pytest.raises(RuntimeError):
-        ff.stft_adjoint_device(p, d, torch.zeros((1, 71, 513), device="cuda"), None)
+        ff.stft_adjoint_device(p, d, torch.zeros((1, 5, 513), device="cuda"), None)
 
 
 def test_config4_autograd_gradient(ff):
