@@ -22,13 +22,12 @@
 // (the mirror image of the forward split). The inverse runs as conj(DFT(conj Y)) on the
 // forward engine with the forward twiddles.
 //
-// Overlap-add is a gather in a fixed frame order, so the result is deterministic. A
-// workgroup owns a run of consecutive frame positions of one signal; each of its four
-// waves transforms one frame per step into a slot of an 8-slot LDS ring, and after each
-// step the workgroup writes the samples that the last four frames complete, each as the
-// ascending-frame sum of the (at most five) windowed r_f that cover it. The four frames
-// before its run are recomputed as a warm-up instead of exchanged with the neighbouring
-// workgroup, so every sample is written exactly once.
+// Overlap-add runs in a fixed order, so the result is deterministic. A workgroup owns a
+// run of consecutive frame positions of one signal and accumulates its frames' windowed
+// r_f, in ascending frame order, in an LDS ring; each sample is written once. The W-H
+// samples at the start of a run also receive the previous run's last frames: each of
+// the two workgroups leaves its partial sums for them in a halo slot, and a second,
+// small kernel (stft_halo_kernel) writes previous + own, in that order.
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 
@@ -222,8 +221,9 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
 //     accumulator ring, frames in ascending order (a fixed order: deterministic);
 //  3. the FPS*H samples that no later frame touches are written out and their ring
 //     entries cleared for reuse.
-// Warm-up steps run the frames before the workgroup's run (they touch its first
-// samples) without writing anything of their own.
+// The first W-H samples of the run go to the left halo slot, and after the last step
+// the samples past the run that its frames touch go to the right halo slot;
+// stft_halo_kernel completes them.
 template <int L, int E, int MODE, int ACCM, int... Rs>
 __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   constexpr int T = L / E;
@@ -249,7 +249,9 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
   const size_t sig = tile / (size_t)a.tiles;
   const long long steps = a.steps;
-  const long long f0 = (long long)(tile - sig * a.tiles) * FPS * steps;
+  const int tix = (int)(tile - sig * a.tiles);  // run index within the signal
+  const long long f0 = (long long)tix * FPS * steps;
+  const long long fend = f0 + FPS * steps;
   const float* __restrict__ x = a.pcm + sig * a.len;
   const float* __restrict__ ga = a.g_mag ? a.g_mag + sig * a.frames * BINS : nullptr;
   const float* __restrict__ gb = a.g_phase ? a.g_phase + sig * a.frames * BINS : nullptr;
@@ -269,15 +271,15 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   for (int i = tid; i < ACC; i += 256) acc[i] = 0.0f;
   __syncthreads();
 
-  const long long warm = ((W - 1) / hop + FPS - 1) / FPS;  // steps of frames before f0
+  const int halo = W > (int)hop ? W - (int)hop : 0;  // samples shared with the previous run
   auto frame_valid = [&](long long f) { return f >= 0 && f < frames; };
   // samples of the first step's frame; later steps' samples are prefetched a step ahead
   float2 nxt[E];
   {
-    const long long f = f0 - warm * FPS + slot;
+    const long long f = f0 + slot;
     fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(f) ? (size_t)(f * hop) : 0, W, frame_valid(f));
   }
-  for (long long step = -warm; step < steps; ++step) {
+  for (long long step = 0; step < steps; ++step) {
     const long long fbase = f0 + step * FPS;
     const long long f = fbase + slot;
     const bool valid = frame_valid(f);  // invalid groups run on zeros (barriers stay uniform)
@@ -378,35 +380,90 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
       bool vg[FPS];
 #pragma unroll
       for (int g = 0; g < FPS; ++g) vg[g] = fbase + g >= 0 && fbase + g < frames;
-      for (int u = ulo + tid; u < uhi; u += 256) {
-        const int ri = (rb + u) & (ACC - 1);
-        float sum = acc[ri];
+      if ((hop & 1) == 0) {
+        // even hop: sample pairs (u, u+1), u even, sit in one float2 of every frame buffer
+        // and of the ring (j0 and rb are even)
+        float2* acc2 = reinterpret_cast<float2*>(acc);
+        const float2* wl2 = reinterpret_cast<const float2*>(wl);
+        for (int u = ulo + 2 * tid; u < uhi; u += 512) {
+          const int ri = ((rb + u) & (ACC - 1)) >> 1;
+          float2 sum = acc2[ri];
+          const bool both = u + 1 < uhi;
 #pragma unroll
-        for (int g = 0; g < FPS; ++g) {
-          const int n = u - g * (int)hop;
-          if (vg[g] && n >= 0 && n < W) {
-            const float2 c = fbuf[g][pidx(n >> 1)];
-            sum = __builtin_fmaf((n & 1) ? c.y : c.x, wl[n], sum);
+          for (int g = 0; g < FPS; ++g) {
+            const int n = u - g * (int)hop;
+            if (vg[g] && n >= 0 && n < W) {
+              const float2 c = fbuf[g][pidx(n >> 1)];
+              const float2 w = wl2[n >> 1];
+              sum.x = __builtin_fmaf(c.x, w.x, sum.x);
+              if (n + 1 < W) sum.y = __builtin_fmaf(c.y, w.y, sum.y);
+            }
+          }
+          if (both) {
+            acc2[ri] = sum;
+          } else {
+            acc[ri << 1] = sum.x;
           }
         }
-        acc[ri] = sum;
+      } else {
+        for (int u = ulo + tid; u < uhi; u += 256) {
+          const int ri = (rb + u) & (ACC - 1);
+          float sum = acc[ri];
+#pragma unroll
+          for (int g = 0; g < FPS; ++g) {
+            const int n = u - g * (int)hop;
+            if (vg[g] && n >= 0 && n < W) {
+              const float2 c = fbuf[g][pidx(n >> 1)];
+              sum = __builtin_fmaf((n & 1) ? c.y : c.x, wl[n], sum);
+            }
+          }
+          acc[ri] = sum;
+        }
       }
     }
     __syncthreads();
 
-    // 3. samples [fbase*H, (fbase+FPS)*H) are complete: write (own steps) and clear
+    // 3. samples [fbase*H, (fbase+FPS)*H) have all of this run's frames: write them
+    //    (the first `halo` samples of a run after the first: to the left halo slot)
+    //    and clear their ring entries
     {
       const long long room = len - j0;
       const int uhi = room < FPS * (int)hop ? (int)room : FPS * (int)hop;
-      float* __restrict__ out = gx + (step >= 0 ? j0 : 0);
-      for (int u = ulo + tid; u < uhi; u += 256) {
+      const int lh = tix > 0 ? (int)(f0 * hop + halo - j0) : 0;  // u below lh: halo
+      float* __restrict__ out = gx + j0;
+      float* __restrict__ hl = a.halo + ((size_t)(sig * a.tiles + tix) * 2 + 1) * halo + (j0 - f0 * hop);
+      for (int u = tid; u < uhi; u += 256) {
         const int ri = (rb + u) & (ACC - 1);
-        if (step >= 0) out[u] = acc[ri];
+        if (u < lh)
+          hl[u] = acc[ri];
+        else
+          out[u] = acc[ri];
         acc[ri] = 0.0f;
       }
     }
     __syncthreads();
   }
+
+  // right halo: the samples past the run that its last frames touch
+  const long long jr = fend * hop;
+  const bool right = tix + 1 < a.tiles && jr < len && halo > 0;
+  if (right) {
+    float* __restrict__ hr = a.halo + (size_t)(sig * a.tiles + tix + 1) * 2 * halo;
+    for (int i = tid; i < halo; i += 256) hr[i] = acc[(int)((jr + i) & (ACC - 1))];
+  }
+}
+
+// Halo samples of every run boundary: previous run's partial sum + this run's. One
+// thread per sample; `chunks` workgroups per boundary, boundaries of all signals in turn.
+__global__ void __launch_bounds__(256) stft_halo_kernel(SignalArgs a, int halo, int chunks, size_t run_len) {
+  const size_t b = blockIdx.x / chunks;  // boundary b of signal sig: between runs r-1 and r
+  const int i = (int)(blockIdx.x - b * chunks) * 256 + threadIdx.x;
+  const size_t sig = b / (size_t)(a.tiles - 1);
+  const size_t r = b - sig * (a.tiles - 1) + 1;
+  const size_t jb = r * run_len;
+  if (i >= halo || jb + i >= a.len) return;
+  const float* pr = a.halo + ((sig * a.tiles + r) * 2 + 0) * (size_t)halo;
+  a.grad[sig * a.len + jb + i] = pr[i] + pr[halo + i];
 }
 
 template <int L, int E, int... Rs>
@@ -454,14 +511,27 @@ hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
   const size_t positions = (a.len + a.hop - 1) / a.hop;  // frame positions covering every sample
   const size_t per_signal = (capacity + a.signals - 1) / a.signals;
   size_t steps = (positions + FPS * per_signal - 1) / (FPS * per_signal);
+  // a run must cover the W-H samples it shares with its neighbour
+  const size_t halo = a.window > a.hop ? (size_t)(a.window - a.hop) : 0;
+  const size_t min_steps = (halo + (size_t)FPS * a.hop - 1) / ((size_t)FPS * a.hop);
   steps = steps < 4 ? 4 : (steps > 64 ? 64 : steps);
+  if (steps < min_steps) steps = min_steps;
   a.steps = (int)steps;
   a.tiles = (int)((positions + FPS * steps - 1) / (FPS * steps));
   const size_t grid = a.signals * (size_t)a.tiles;
   if (grid == 0) return hipSuccess;
   if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+  const bool halos = halo > 0 && a.tiles > 1;
+  if (halos && (!a.halo || grid * 2 * halo > a.halo_capacity)) return hipErrorInvalidValue;
   void* args[] = {&a};
-  return hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
+  e = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
+  if (e != hipSuccess || !halos) return e;
+  const int chunks = (int)((halo + 255) / 256);
+  const size_t hgrid = a.signals * (size_t)(a.tiles - 1) * chunks;
+  if (hgrid > 0x7fffffffull) return hipErrorInvalidValue;
+  const size_t run_len = (size_t)FPS * steps * (size_t)a.hop;
+  hipLaunchKernelGGL(stft_halo_kernel, dim3((unsigned)hgrid), dim3(256), 0, s, a, (int)halo, chunks, run_len);
+  return hipGetLastError();
 }
 
 // The smallest accumulator ring (2N, 4N or 8N samples) that holds one step's span.
@@ -476,6 +546,15 @@ hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s
 
 // Frames per adjoint step for each N (256 / threads per frame).
 int adjoint_fps(int n) { return n == 2048 ? 2 : 4; }
+
+size_t adjoint_scratch(int n, int window, int hop, size_t signals, size_t pcm_len) {
+  if (hop <= 0 || window <= hop) return 0;
+  // at least 4 steps per run: an upper bound on the runs the launcher makes
+  const size_t positions = (pcm_len + hop - 1) / hop;
+  const size_t per = (size_t)adjoint_fps(n) * 4;
+  const size_t runs = signals * ((positions + per - 1) / per);
+  return runs * 2 * (size_t)(window - hop);
+}
 
 bool signals_supported(int n, int window, int hop, bool adjoint) {
   if (n != 512 && n != 1024 && n != 2048) return false;

@@ -93,6 +93,7 @@ struct FftContext {
   StreamDispatcher dispatcher;
   DeviceBuffer d_awin, d_swin, d_pcm, d_planes, d_cplx;
   DeviceBuffer d_sink;  // N floats that kernels write skipped output planes to
+  DeviceBuffer d_halo;  // adjoint run-boundary partial sums
   bool windows_dirty = true;
 
   ~FftContext() {
@@ -102,6 +103,7 @@ struct FftContext {
     d_planes.release();
     d_cplx.release();
     d_sink.release();
+    d_halo.release();
     plan.reset();
     cache.clear();
     if (stream) {
@@ -685,8 +687,15 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
     a.g_mag = d_grad_real;
     a.g_phase = d_grad_imag;
     a.grad = d_grad_pcm;
+    const size_t halo_floats = fftfree::adjoint_scratch(c->N, a.window, a.hop, signals, pcm_len);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (halo_floats > 0) {
+      c->d_halo.reserve(halo_floats * sizeof(float));
+      a.halo = c->d_halo.as<float>();
+      a.halo_capacity = c->d_halo.bytes / sizeof(float);
+    }
     const int mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : 0;  // polar, magnitude, complex
-    fftfree::hip_check(fftfree::launch_stft_adjoint(a, mode, static_cast<hipStream_t>(stream)), "launch_stft_adjoint");
+    fftfree::hip_check(fftfree::launch_stft_adjoint(a, mode, s), "launch_stft_adjoint");
     return frames;
   } catch (const std::exception& e) {
     fftfree::report("fft_stft_adjoint_device", e.what());

@@ -68,7 +68,11 @@ struct SignalArgs {
   float* grad = nullptr;           // adjoint output: dL/dpcm, signals*len floats
   int tiles = 0;                   // set by the launchers: workgroups per signal
   int steps = 0;                   // adjoint: steps per workgroup
+  float* halo = nullptr;           // adjoint: boundary partial sums (2 slots of W-H per run)
+  size_t halo_capacity = 0;        // floats
 };
+// Device scratch (floats) the adjoint needs for `signals` signals of pcm_len samples.
+size_t adjoint_scratch(int n, int window, int hop, size_t signals, size_t pcm_len);
 
 // Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
 enum MetaKind : int {

@@ -46,7 +46,8 @@ def win_n(n, W, kind=1):
 
 
 CASES = [(512, 512, 128, 5000), (1024, 1024, 256, 9000), (2048, 2048, 512, 20000), (1024, 768, 200, 7001),
-         (2048, 2048, 512, 1500), (1024, 1024, 512, 9000), (1024, 1024, 1024, 9000), (512, 256, 700, 6000)]
+         (2048, 2048, 512, 1500), (1024, 1024, 512, 9000), (1024, 1024, 1024, 9000), (512, 256, 700, 6000),
+         (2048, 2048, 128, 50000), (512, 512, 64, 30000)]
 
 
 @pytest.mark.parametrize("case", CASES)
