@@ -13,7 +13,9 @@ of the reference CPU path on the same workload on this host's cores.
 
 Other workloads (--workload): c2c1024 (64 x 1024-point C2C, Stockham); c2c_large
 (forward + inverse round trip of 16 complex transforms of 2^22 points, or 2^--lg);
-stft_grad (differentiable STFT: 32 signals of 2^18 samples, window 2048, hop 512, polar
+ragged (512 requests, lengths log-uniform in [2^12, 2^20], window from {256..4096}, hop
+W/4, polar planes: one autotuned persistent launch per graph replay vs one plain launch
+per request); stft_grad (differentiable STFT: 32 signals of 2^18 samples, window 2048, hop 512, polar
 planes; forward kernel alone vs forward + adjoint kernel, and a whole torch autograd step
 of the loss sum (|X| - |Y|)^2); stft_host (the default workload through the reference's host-pointer entry
 fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); copy
@@ -39,7 +41,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "stft_grad", "copy"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "stft_grad", "ragged", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -261,6 +263,33 @@ def main():
                              "forward": {"achieved": fwd_bytes / (fwd_ms * 1e-3) / 1e9,
                                          "frac": fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                          "algorithmic_bytes_per_launch": fwd_bytes}}}
+    elif args.workload == "ragged":
+        from fftfree_amd.cffi_fft import RaggedBatch
+        from fftfree_amd.workloads import ragged_requests
+        reqs = ragged_requests(512, seed=5)
+        pcms = [torch.from_numpy(noise_plus_sines(L, seed=5000 + i)).cuda() for i, (L, W, H) in enumerate(reqs)]
+        rb = RaggedBatch(ff, [(pcm, W, H) for pcm, (L, W, H) in zip(pcms, reqs)])
+        t0 = time.perf_counter()
+        rb.autotune(3)
+        tune_s = time.perf_counter() - t0
+        frames = sum(rb.frames)
+        alg_bytes = sum(4 * L + 12 * f * (W // 2 + 1) for (L, W, H), f in zip(reqs, rb.frames))
+        graph_s, wall = timed(rb.execute, args.steps, args.warmup, torch)
+        plain_s, _ = timed(rb.execute_plain, args.steps, args.warmup, torch)
+        per, per_plain = graph_s / args.steps, plain_s / args.steps
+        line = {"metric": "ragged batched STFT: frames/s in one graph launch, speedup vs per-request launches",
+                "value": frames / per, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": per * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32", "data": "synthetic: noise + 8 sinusoids per request, seeds 5000..",
+                "config": {"workload": "512 requests, L log-uniform [2^12, 2^20], W in {256..4096}, hop W/4, "
+                                       "Hann, R2C half spectrum, polar planes, pad_mode 1",
+                           "frames": frames, "samples": sum(L for L, W, H in reqs),
+                           "plain_ms_per_step": per_plain * 1e3, "speedup_vs_plain": per_plain / per,
+                           "autotune_s": tune_s, "plan": rb.describe()},
+                "roofline": {"bound": "hbm", "achieved": alg_bytes / per / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": alg_bytes / per / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                             "algorithmic_bytes_per_launch": alg_bytes, "kernel": "stft_ragged_kernel"}}
+        rb.close()
     elif args.workload == "copy":
         nbytes = 512 << 20
         x = torch.ones(nbytes // 4, device="cuda")

@@ -20,6 +20,12 @@ c_int, c_float, c_size_t, c_void_p, c_char_p = ctypes.c_int, ctypes.c_float, cty
 f32p = ctypes.POINTER(ctypes.c_float)
 i32p = ctypes.POINTER(ctypes.c_int)
 
+class RaggedRequest(ctypes.Structure):
+    """fft_ragged_request (include/fft_cffi.h)."""
+    _fields_ = [("pcm", c_void_p), ("pcm_len", c_size_t), ("window", c_int), ("hop", c_int),
+                ("out_real", c_void_p), ("out_imag", c_void_p), ("out_mag", c_void_p), ("frames", c_size_t)]
+
+
 # name -> (restype, argtypes), in header order
 SIGNATURES = {
     "fft_pcm_to_channels": (None, [f32p, f32p, f32p, f32p, c_size_t, c_int]),
@@ -53,6 +59,12 @@ SIGNATURES = {
                                            c_int, c_void_p]),
     "fft_stft_adjoint_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
                                            c_int, c_void_p]),
+    "fft_ragged_create": (c_void_p, [ctypes.POINTER(RaggedRequest), c_size_t, c_int, c_int, c_int, c_int, c_int]),
+    "fft_ragged_autotune": (c_int, [c_void_p, c_int]),
+    "fft_ragged_execute": (c_int, [c_void_p, c_void_p]),
+    "fft_ragged_execute_plain": (c_int, [c_void_p, c_void_p]),
+    "fft_ragged_describe": (c_int, [c_void_p, c_char_p, c_size_t]),
+    "fft_ragged_free": (None, [c_void_p]),
     "fft_plan_metadata": (c_size_t, [c_void_p, c_int, f32p, c_size_t]),
     "fft_set_variant": (c_int, [c_void_p, c_int]),
     "fft_variant_count": (c_int, [c_void_p]),

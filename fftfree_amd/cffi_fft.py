@@ -230,3 +230,71 @@ class CffiFft:
         if got == 0:
             raise RuntimeError("fft_stft_adjoint_device failed")
         return out
+
+
+class RaggedBatch:
+    """A ragged batch over torch CUDA tensors (fft_ragged_*): requests are
+    (pcm, window, hop) with pcm a 1-D float32 CUDA tensor; outputs are allocated here as
+    three (frames, window/2+1) planes per request (None planes are skipped)."""
+
+    def __init__(self, ff: "CffiFft", requests, pad_mode: int = 1, reduce_magnitude: bool = False,
+                 store_polar: bool = True, apply_windows: bool = True, window_kind: int = 1,
+                 planes=(True, True, True)):
+        import torch
+        self.ff = ff
+        lib = ff.lib
+        n = len(requests)
+        arr = (_lib.RaggedRequest * n)()
+        self.pcm = []
+        for i, (pcm, window, hop) in enumerate(requests):
+            assert pcm.is_cuda and pcm.dtype == torch.float32 and pcm.is_contiguous()
+            self.pcm.append(pcm)
+            arr[i].pcm, arr[i].pcm_len, arr[i].window, arr[i].hop = pcm.data_ptr(), pcm.numel(), window, hop
+        # frame counts first (one context per geometry answers them on the host)
+        self.outputs = []
+        geometry = {}
+        for i, (pcm, window, hop) in enumerate(requests):
+            if (window, hop) not in geometry:
+                geometry[(window, hop)] = ff.make_plan(window, window=window, hop=hop, kernel=2, transform=1,
+                                                       half_spectrum=True, pad_mode=pad_mode)
+            frames = ff.frame_count(geometry[(window, hop)], pcm.numel(), 0)
+            outs = [torch.empty((frames, window // 2 + 1), device=pcm.device) if want else None for want in planes]
+            self.outputs.append(outs)
+            ptr = lambda t: t.data_ptr() if t is not None and t.numel() else None
+            arr[i].out_real, arr[i].out_imag, arr[i].out_mag = ptr(outs[0]), ptr(outs[1]), ptr(outs[2])
+        self._arr = arr
+        self.handle = lib.fft_ragged_create(arr, n, pad_mode, int(reduce_magnitude), int(store_polar),
+                                            int(apply_windows), window_kind)
+        if not self.handle:
+            raise ValueError("fft_ragged_create refused the batch")
+        self.frames = [int(arr[i].frames) for i in range(n)]
+        for outs, f in zip(self.outputs, self.frames):
+            assert all(o is None or o.shape[0] == f for o in outs)
+
+    def autotune(self, repeats: int = 3) -> None:
+        if not self.ff.lib.fft_ragged_autotune(self.handle, repeats):
+            raise RuntimeError("fft_ragged_autotune failed")
+
+    def execute(self) -> None:
+        if not self.ff.lib.fft_ragged_execute(self.handle, CffiFft._stream()):
+            raise RuntimeError("fft_ragged_execute failed")
+
+    def execute_plain(self) -> None:
+        if not self.ff.lib.fft_ragged_execute_plain(self.handle, CffiFft._stream()):
+            raise RuntimeError("fft_ragged_execute_plain failed")
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        self.ff.lib.fft_ragged_describe(self.handle, buf, 512)
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if self.handle:
+            self.ff.lib.fft_ragged_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

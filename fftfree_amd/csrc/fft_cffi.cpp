@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -374,6 +375,172 @@ const char* variant_desc(const FftContext* c, int idx, char* buf, size_t len) {
 
 using fftfree::FftContext;
 using fftfree::HipFailure;
+
+namespace fftfree {
+namespace {
+
+// A ragged batch: one context per distinct (window, hop) supplies windows, twiddles and
+// the plane sink; every request becomes a StftArgs block in device memory; the work list
+// (items of tiles, sorted by cost, split into 8 cost-balanced XCD chunks) and the graph
+// that holds the one persistent launch are rebuilt when the autotuner changes the
+// per-N tile shape or tiles per item.
+struct RaggedPlan {
+  std::vector<fft_ragged_request> reqs;
+  std::vector<void*> ctxs;           // FftContext handles
+  std::vector<int> req_ctx;
+  std::vector<StftArgs> args;
+  int mode = 0;
+  std::map<int, int> cfg_of_n;       // N -> ragged tile shape
+  std::map<int, int> tiles_of_n;     // N -> tiles per work item
+  DeviceBuffer d_args, d_items, d_chunks;
+  size_t n_items = 0;
+  int grid = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;      // private stream for capture and tuning
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~RaggedPlan() {
+    (void)hipSetDevice(device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+    d_args.release();
+    d_items.release();
+    d_chunks.release();
+    for (void* c : ctxs) fft_free(c);
+  }
+
+  // Items for the requests in `only` (all when empty) with the current choices; returns
+  // them with the chunk boundaries.
+  void build(const std::vector<size_t>& only, std::vector<RaggedItem>& items, std::vector<unsigned>& chunks) const {
+    items.clear();
+    std::vector<double> cost;
+    auto add = [&](size_t r) {
+      const int n = reqs[r].window;
+      const int cfg = cfg_of_n.at(n);
+      const int fpw = ragged_cfg(cfg)->fpw;
+      const unsigned per = (unsigned)tiles_of_n.at(n);
+      const size_t tiles = (reqs[r].frames + fpw - 1) / fpw;
+      const double tile_cost = (double)fpw * (4.0 * reqs[r].hop + 12.0 * (n / 2 + 1));
+      for (size_t t0 = 0; t0 < tiles; t0 += per) {
+        const unsigned cnt = (unsigned)std::min<size_t>(per, tiles - t0);
+        items.push_back(RaggedItem{(unsigned)r, (unsigned)cfg, (unsigned)t0, cnt});
+        cost.push_back(tile_cost * cnt);
+      }
+    };
+    if (only.empty())
+      for (size_t r = 0; r < reqs.size(); ++r) add(r);
+    else
+      for (size_t r : only) add(r);
+    // costliest first (stable, so the tiles of a request stay in order)
+    std::vector<size_t> order(items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<RaggedItem> sorted(items.size());
+    std::vector<double> sc(items.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      sorted[i] = items[order[i]];
+      sc[i] = cost[order[i]];
+    }
+    items.swap(sorted);
+    double total = 0;
+    for (double c : sc) total += c;
+    chunks.assign(9, 0);
+    double run = 0;
+    size_t i = 0;
+    for (int x = 1; x < 8; ++x) {
+      while (i < items.size() && run + sc[i] * 0.5 < total * x / 8.0) run += sc[i++];
+      chunks[x] = (unsigned)i;
+    }
+    chunks[8] = (unsigned)items.size();
+  }
+
+  void upload(const std::vector<RaggedItem>& items, const std::vector<unsigned>& chunks) {
+    d_items.reserve(std::max<size_t>(1, items.size()) * sizeof(RaggedItem));
+    if (!items.empty())
+      hip_check(hipMemcpy(d_items.ptr, items.data(), items.size() * sizeof(RaggedItem), hipMemcpyHostToDevice),
+                "upload items");
+    d_chunks.reserve(chunks.size() * sizeof(unsigned));
+    hip_check(hipMemcpy(d_chunks.ptr, chunks.data(), chunks.size() * sizeof(unsigned), hipMemcpyHostToDevice),
+              "upload chunks");
+    n_items = items.size();
+  }
+
+  void launch(hipStream_t s) const {
+    hip_check(launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(), mode,
+                                 grid, s),
+              "launch_stft_ragged");
+  }
+
+  // Rebuild the full work list and capture its launch in a graph.
+  void commit() {
+    std::vector<RaggedItem> items;
+    std::vector<unsigned> chunks;
+    build({}, items, chunks);
+    upload(items, chunks);
+    if (exec) hip_check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
+    if (graph) hip_check(hipGraphDestroy(graph), "hipGraphDestroy");
+    exec = nullptr;
+    graph = nullptr;
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t le = launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(),
+                                       mode, grid, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(stream, &g);
+    hip_check(le, "launch_stft_ragged (capture)");
+    hip_check(ee, "hipStreamEndCapture");
+    graph = g;
+    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+
+  // Time the persistent launch over the requests of one N with each tile shape for that
+  // N and each tiles-per-item choice; keep the fastest.
+  void tune(int repeats) {
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    for (auto& kv : cfg_of_n) {
+      const int n = kv.first;
+      std::vector<size_t> only;
+      for (size_t r = 0; r < reqs.size(); ++r)
+        if (reqs[r].window == n) only.push_back(r);
+      float best = 1e30f;
+      int best_cfg = kv.second, best_tiles = tiles_of_n[n];
+      for (int cfg = 0; cfg < ragged_cfg_count(); ++cfg) {
+        if (2 * ragged_cfg(cfg)->l != n) continue;
+        for (int per : {1, 2, 4, 8}) {
+          cfg_of_n[n] = cfg;
+          tiles_of_n[n] = per;
+          std::vector<RaggedItem> items;
+          std::vector<unsigned> chunks;
+          build(only, items, chunks);
+          upload(items, chunks);
+          launch(stream);  // warm
+          hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+          for (int i = 0; i < repeats; ++i) launch(stream);
+          hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float ms = 0;
+          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+          if (ms < best) {
+            best = ms;
+            best_cfg = cfg;
+            best_tiles = per;
+          }
+        }
+      }
+      cfg_of_n[n] = best_cfg;
+      tiles_of_n[n] = best_tiles;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+};
+
+}  // namespace
+}  // namespace fftfree
+
 
 extern "C" {
 
@@ -973,5 +1140,118 @@ int fft_variant_name(void* handle, int variant, char* buf, size_t buflen) {
   if (!handle || !buf || buflen == 0) return 0;
   return fftfree::variant_desc(static_cast<FftContext*>(handle), variant, buf, buflen) ? 1 : 0;
 }
+
+void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode, int reduce_magnitude,
+                        int store_polar, int apply_windows, int window_kind) {
+  if (!requests || count == 0) return nullptr;
+  std::unique_ptr<fftfree::RaggedPlan> p(new fftfree::RaggedPlan());
+  try {
+    std::map<std::pair<int, int>, int> ctx_of;
+    for (size_t i = 0; i < count; ++i) {
+      fft_ragged_request& q = requests[i];
+      const int n = q.window;
+      if (!q.pcm || q.pcm_len == 0 || q.hop <= 0 || (n != 256 && n != 512 && n != 1024 && n != 2048 && n != 4096))
+        return nullptr;
+      const auto key = std::make_pair(n, q.hop);
+      if (!ctx_of.count(key)) {
+        void* h = fft_init_full_v2((size_t)n, 1, 1, 0, 2, 0, nullptr, 0, pad_mode, n, q.hop, FFT_TRANSFORM_R2C,
+                                   reduce_magnitude, store_polar, 1, 0, 0, 0, 0, 1, apply_windows, 0, window_kind,
+                                   0.0f, 0.0f, window_kind, 0.0f, 0.0f, FFT_WINDOW_NORM_NONE, FFT_COLA_OFF);
+        if (!h) return nullptr;
+        ctx_of[key] = (int)p->ctxs.size();
+        p->ctxs.push_back(h);
+      }
+      p->req_ctx.push_back(ctx_of[key]);
+    }
+    p->reqs.assign(requests, requests + count);
+    for (size_t i = 0; i < count; ++i) {
+      auto* c = static_cast<FftContext*>(p->ctxs[p->req_ctx[i]]);
+      fftfree::ensure_gpu(c);
+      fft_ragged_request& q = p->reqs[i];
+      q.frames = fftfree::frame_count(c, q.pcm_len, 0, 0);
+      requests[i].frames = q.frames;
+      fftfree::StftArgs a = fftfree::stft_args(c, q.pcm, q.pcm_len);
+      a.frames = q.frames;
+      a.o_re = q.out_real;
+      a.o_im = q.out_imag;
+      a.o_mag = q.out_mag;
+      p->args.push_back(a);
+      p->cfg_of_n[q.window] = -1;
+    }
+    // defaults before tuning: the E=8 shape and 4 tiles per item
+    for (auto& kv : p->cfg_of_n) {
+      for (int cfg = 0; cfg < fftfree::ragged_cfg_count(); ++cfg)
+        if (2 * fftfree::ragged_cfg(cfg)->l == kv.first && kv.second < 0) kv.second = cfg;
+      p->tiles_of_n[kv.first] = 4;
+    }
+    p->mode = store_polar ? 3 : reduce_magnitude ? 2 : 0;
+    fftfree::hip_check(hipGetDevice(&p->device), "hipGetDevice");
+    fftfree::hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    fftfree::hip_check(fftfree::ragged_grid(p->mode, &p->grid), "ragged_grid");
+    if (p->grid < 8) p->grid = 8;
+    p->d_args.reserve(p->args.size() * sizeof(fftfree::StftArgs));
+    fftfree::hip_check(hipMemcpy(p->d_args.ptr, p->args.data(), p->args.size() * sizeof(fftfree::StftArgs),
+                                 hipMemcpyHostToDevice),
+                       "upload requests");
+    p->commit();
+    return p.release();
+  } catch (const std::exception& e) {
+    fftfree::report("fft_ragged_create", e.what());
+    return nullptr;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+int fft_ragged_autotune(void* ragged, int repeats) {
+  if (!ragged) return 0;
+  auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
+  try {
+    fftfree::hip_check(hipSetDevice(p->device), "hipSetDevice");
+    p->tune(repeats > 0 ? repeats : 3);
+    p->commit();
+    return 1;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_ragged_autotune", e.what());
+    return 0;
+  }
+}
+
+int fft_ragged_execute(void* ragged, void* stream) {
+  if (!ragged) return 0;
+  auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
+  return hipGraphLaunch(p->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? 1 : 0;
+}
+
+int fft_ragged_execute_plain(void* ragged, void* stream) {
+  if (!ragged) return 0;
+  auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
+  try {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (size_t i = 0; i < p->args.size(); ++i) {
+      auto* c = static_cast<FftContext*>(p->ctxs[p->req_ctx[i]]);
+      fftfree::hip_check(fftfree::launch_stft(p->args[i], 0, c->plan->stft_variant, s), "launch_stft");
+    }
+    return 1;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_ragged_execute_plain", e.what());
+    return 0;
+  }
+}
+
+int fft_ragged_describe(void* ragged, char* buf, size_t buflen) {
+  if (!ragged || !buf || buflen == 0) return 0;
+  auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
+  std::string out = "grid=" + std::to_string(p->grid) + " items=" + std::to_string(p->n_items);
+  for (auto& kv : p->cfg_of_n) {
+    const fftfree::VariantInfo* v = fftfree::ragged_cfg(kv.second);
+    out += " N" + std::to_string(kv.first) + ":" + v->radices + "/fpw" + std::to_string(v->fpw) + "/x" +
+           std::to_string(p->tiles_of_n[kv.first]);
+  }
+  std::snprintf(buf, buflen, "%s", out.c_str());
+  return 1;
+}
+
+void fft_ragged_free(void* ragged) { delete static_cast<fftfree::RaggedPlan*>(ragged); }
 
 }  // extern "C"

@@ -121,6 +121,24 @@ struct VariantInfo {
   int fpw;          // transforms per workgroup
   const char* radices;
 };
+
+// Ragged batches: one work item = `ntiles` consecutive tiles (FPW frames each) of
+// request `req`, run with tile shape `cfg` (ragged_cfg).
+struct RaggedItem {
+  unsigned req;
+  unsigned cfg;
+  unsigned tile;
+  unsigned ntiles;
+};
+int ragged_cfg_count();
+const VariantInfo* ragged_cfg(int id);  // l = N/2; 256 threads = fpw * l / e
+// Resident workgroups of the ragged kernel on the current device (CUs x occupancy).
+hipError_t ragged_grid(int mode, int* grid);
+// One persistent launch over every item: workgroups of XCD x (blockIdx % 8 == x) stride
+// through items[chunks[x] .. chunks[x+1]). Forward R2C, half spectrum, output mode
+// kOutComplex / kOutMagnitude / kOutPolar.
+hipError_t launch_stft_ragged(const StftArgs* reqs, const RaggedItem* items, const unsigned* chunks, int mode,
+                              int grid, hipStream_t s);
 int stft_variant_count(int n);
 const VariantInfo* stft_variant(int n, int idx);
 int c2c_variant_count(int n);

@@ -160,6 +160,39 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
                                const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,
                                void* stream);
 
+/* additive. One request of a ragged batch (device pointers): pcm_len samples, framed with
+ * window = N (a power of two from 256 to 4096) and hop; output planes of frames*(N/2+1)
+ * floats each, as fft_execute_batched writes them for a R2C half-spectrum context (any
+ * may be NULL). fft_ragged_create writes `frames`. */
+typedef struct fft_ragged_request {
+  const float* pcm;
+  size_t pcm_len;
+  int window;
+  int hop;
+  float* out_real;
+  float* out_imag;
+  float* out_mag;
+  size_t frames;
+} fft_ragged_request;
+
+/* additive. A ragged batch: every request gets the same flags (pad_mode as in
+ * fft_execute_batched, reduce_magnitude / store_polar output, analysis window kind when
+ * apply_windows). The buffers are fixed: fft_ragged_execute reads and writes the ones
+ * given here, so refill them between executions. Returns NULL on bad requests. */
+void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode, int reduce_magnitude,
+                        int store_polar, int apply_windows, int window_kind);
+/* additive. Times each tile shape (radix pattern and frames per workgroup) and tiles per
+ * work item for every N of the batch on the batch itself (writing its outputs), keeps
+ * the fastest, and rebuilds the work list and graph. Returns 1 on success. */
+int fft_ragged_autotune(void* ragged, int repeats);
+/* additive. The whole batch as one graph launch (one persistent kernel) on `stream`. */
+int fft_ragged_execute(void* ragged, void* stream);
+/* additive. The same outputs with one plain launch per request (the unbatched baseline). */
+int fft_ragged_execute_plain(void* ragged, void* stream);
+/* additive. Text summary of the grid, item count and per-N choices. */
+int fft_ragged_describe(void* ragged, char* buf, size_t buflen);
+void fft_ragged_free(void* ragged);
+
 /* additive. Index metadata of the context's plan (MetadataRequest kinds of the C++ API,
  * eigen_fft.hpp:1496-1501), written by a GPU index kernel as complex<float> pairs into
  * the host buffer `out` (capacity in complex entries). Returns entries written; 0 if the

@@ -1,0 +1,126 @@
+"""Ragged batched STFT (fft_ragged_*): one persistent launch in a graph over requests
+with different lengths, windows and hops, against per-request plain launches and the
+FP64 oracle. Tolerance: relative L2 <= 2e-7*log2(N) (fp64.tolerance); frame counts exact."""
+import numpy as np
+import pytest
+
+from fftfree_amd import CffiFft
+from fftfree_amd.cffi_fft import RaggedBatch
+from fftfree_amd.workloads import noise_plus_sines, ragged_requests
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ff():
+    f = CffiFft()
+    yield f
+    f.close()
+
+
+def make_batch(ff, count, seed, lo=12, hi=16, **kw):
+    import torch
+    reqs = ragged_requests(count, seed=seed, lo=lo, hi=hi)
+    pcms = [torch.from_numpy(noise_plus_sines(L, seed=seed * 1000 + i)).cuda() for i, (L, W, H) in enumerate(reqs)]
+    return reqs, pcms, RaggedBatch(ff, [(p, W, H) for p, (L, W, H) in zip(pcms, reqs)], **kw)
+
+
+def oracle_planes(pcm, W, H, frames, mode):
+    win = fp64.build_window(1, W)
+    flags = {"polar": (False, True), "magnitude": (True, False), "complex": (False, False)}[mode]
+    return fp64.stft_planes(pcm, W, W, H, frames, win, False, 1, flags[0], flags[1], True)
+
+
+def check_planes(outs, want, mode, n):
+    re, im, mg = (o.cpu().numpy() for o in outs)
+    ore, oim, omg = want
+    tol = fp64.tolerance(n)
+    if mode == "polar":
+        assert fp64.rel_l2(re * np.exp(1j * im), ore * np.exp(1j * oim)) <= tol
+    else:
+        assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= tol
+    assert fp64.rel_l2(mg, omg) <= tol
+
+
+@pytest.mark.parametrize("mode", ["polar", "magnitude", "complex"])
+def test_ragged_graph_matches_oracle_and_plain(ff, mode):
+    import torch
+    kw = dict(store_polar=mode == "polar", reduce_magnitude=mode == "magnitude")
+    reqs, pcms, rb = make_batch(ff, 40, seed=3, **kw)
+    print(f"ragged mode={mode}: {rb.describe()}", flush=True)
+    for (L, W, H), f in zip(reqs, rb.frames):
+        assert f == fp64.frame_count(L, W, W, H, 0, 1)
+    rb.execute()
+    torch.cuda.synchronize()
+    got = [[o.clone() for o in outs] for outs in rb.outputs]
+    for i in range(0, 40, 7):
+        L, W, H = reqs[i]
+        check_planes(got[i], oracle_planes(pcms[i].cpu().numpy(), W, H, rb.frames[i], mode), mode, W)
+    rb.execute_plain()
+    torch.cuda.synchronize()
+    for i in range(40):
+        a = [o.cpu().numpy() for o in got[i]]
+        b = [o.cpu().numpy() for o in rb.outputs[i]]
+        tol = fp64.tolerance(reqs[i][1])
+        if mode == "polar":  # the phase of near-zero bins is noise: compare values
+            assert fp64.rel_l2(a[0] * np.exp(1j * a[1]), b[0] * np.exp(1j * b[1])) <= tol
+        else:
+            assert fp64.rel_l2(a[0] + 1j * a[1], b[0] + 1j * b[1]) <= tol
+        assert fp64.rel_l2(a[2], b[2]) <= tol
+    rb.close()
+
+
+def test_ragged_autotune_and_graph_replay(ff):
+    import torch
+    reqs, pcms, rb = make_batch(ff, 24, seed=4)
+    rb.autotune(2)
+    print(f"tuned: {rb.describe()}", flush=True)
+    rb.execute()
+    torch.cuda.synchronize()
+    first = [[o.clone() for o in outs] for outs in rb.outputs]
+    rb.execute()
+    torch.cuda.synchronize()
+    for a, b in zip(first, rb.outputs):
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)  # replays are deterministic
+    # the graph reads the live buffers: halve every input and replay
+    for p in pcms:
+        p.mul_(0.5)
+    rb.execute()
+    torch.cuda.synchronize()
+    for a, b in zip(first, rb.outputs):
+        assert fp64.rel_l2(b[0].cpu().numpy(), 0.5 * a[0].cpu().numpy()) <= 1e-6  # |X| halves
+    for i in range(0, 24, 5):
+        L, W, H = reqs[i]
+        check_planes(rb.outputs[i], oracle_planes(pcms[i].cpu().numpy(), W, H, rb.frames[i], "polar"), "polar", W)
+    rb.close()
+
+
+def test_ragged_config5_batch(ff):
+    """512 requests, lengths log-uniform in [2^12, 2^20], window from {256..4096}, hop
+    W/4, R2C half spectrum, polar, pad_mode 1: exact frame counts for all, FP64 oracle on
+    sampled frames of a spread of requests."""
+    import torch
+    reqs, pcms, rb = make_batch(ff, 512, seed=5, lo=12, hi=20)
+    rb.execute()
+    torch.cuda.synchronize()
+    for (L, W, H), f in zip(reqs, rb.frames):
+        assert f == fp64.frame_count(L, W, W, H, 0, 1)
+    rng = np.random.default_rng(0)
+    for i in rng.choice(512, 12, replace=False):
+        L, W, H = reqs[i]
+        pcm = pcms[i].cpu().numpy()
+        pick = np.sort(rng.choice(rb.frames[i], min(8, rb.frames[i]), replace=False))
+        win = fp64.build_window(1, W)
+        frames = np.zeros((len(pick), W))
+        for k, f in enumerate(pick):
+            seg = pcm[f * H:f * H + W].astype(np.float64)
+            frames[k, :seg.size] = seg * win[:seg.size]
+        X = np.fft.rfft(frames, axis=-1)
+        X[:, 0] = X[:, 0].real
+        X[:, -1] = X[:, -1].real
+        mag = rb.outputs[i][0].cpu().numpy()[pick]
+        ph = rb.outputs[i][1].cpu().numpy()[pick]
+        assert fp64.rel_l2(mag * np.exp(1j * ph), X) <= fp64.tolerance(W), i
+    rb.close()
