@@ -18,6 +18,11 @@
 #include "fft_kernels.hpp"
 #include "fft_stft_tile.hpp"
 
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+
 namespace fftfree {
 using namespace dev;
 
@@ -50,8 +55,8 @@ __global__ void __launch_bounds__(FPW*(L / E)) stft_r2c_kernel(StftArgs a) {
 // samples and planes. The frames after the last whole tile (a few dozen) go to
 // workgroup 0, which is dispatched first: it walks them P at a time with per-frame
 // bounds, zero fill and direct plane stores, while the tiles run.
-template <int L, int E, int K, int MODE, int HALF, int DIR, int... Rs>
-__global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
+template <int L, int E, int K, int PL, int MODE, int HALF, int DIR, int... Rs>
+__global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
   constexpr int NT = 256;
@@ -68,16 +73,21 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   constexpr int RUN = P * BINS;        // floats per plane per iteration
   constexpr int N4 = RUN / 4;
   constexpr int C4 = (N4 + NT - 1) / NT;
-  constexpr int SPAN = (P - 1) * (N / 4) + N;  // samples of P frames, hop <= N/4
+  // PL = 0: each iteration's samples (P frames) are loaded a few iterations ahead into a
+  // double buffer; PL = 1: the samples of the whole tile (P*K frames) are loaded once, at
+  // the start, so no load (and no wait for one) follows the first store.
+  constexpr int SPAN = (PL ? P * K - 1 : P - 1) * (N / 4) + N;  // hop <= N/4
   constexpr int SPAN4 = SPAN / 4;
-  constexpr int SR = (SPAN4 + NT - 1) / NT;    // 16-byte loads per thread per iteration
+  constexpr int SR = (SPAN4 + NT - 1) / NT;    // 16-byte loads per thread per load phase
+  constexpr int NSPAN = PL ? 1 : 2;
+  constexpr int NSTAGE = PL ? 1 : 2;
   static_assert(T <= 64 && (64 % T) == 0, "one transform per wave or less");
   static_assert(JF * T == L / 2, "T must divide L/2");
   static_assert(RUN % 4 == 0 && SPAN % 4 == 0, "runs must be whole 16-byte words");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
   __shared__ float2 lds[P][LP];
-  __shared__ __attribute__((aligned(16))) float stage[2][STAGE];
-  __shared__ __attribute__((aligned(16))) float span[2][SPAN];
+  __shared__ __attribute__((aligned(16))) float stage[NSTAGE][STAGE];
+  __shared__ __attribute__((aligned(16))) float span[NSPAN][SPAN];
 
   const int tid = threadIdx.x;
   const int slot = tid / T;
@@ -85,7 +95,7 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   float2* buf = &lds[slot][0];
   const int has_rest = a.frames > a.lead ? 1 : 0;
   const size_t tile = (size_t)xcd_tile((int)blockIdx.x - has_rest, (int)gridDim.x - has_rest);
-  const size_t f0 = tile * (size_t)(K * P);
+  size_t f0 = tile * (size_t)(K * P);  // first frame of the tile in hand
   const int hop = a.hop;
   const float* __restrict__ win = a.win;
   const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win);
@@ -105,7 +115,9 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   for (int j = 0; j < JF; ++j) wsp[j] = load_tw<DIR>(a.tw, t + j * T);
   const float2 wmid = load_tw<DIR>(a.tw, L / 2);
 
-  if (has_rest && blockIdx.x == 0) {
+  // frames after the last whole tile: P at a time with per-frame bounds, zero fill and
+  // direct plane stores
+  auto do_rest = [&]() {
     for (size_t fb = a.lead; fb < a.frames; fb += P) {
       const size_t f = fb + slot;
       const bool active = f < a.frames;
@@ -170,13 +182,12 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
       }
       wave_lds_fence();
     }
-    return;
-  }
+  };
 
   // iteration it covers frames f0 + it*P .. +P-1, i.e. samples [s, s + (P-1)*hop + N)
   // of the PCM (the host guarantees 16-byte alignment and that they lie inside it);
   // SR 16-byte loads per thread, spare lanes re-read the last word
-  const int span4 = ((P - 1) * hop + N) / 4;
+  const int span4 = ((PL ? P * K - 1 : P - 1) * hop + N) / 4;
   auto load_span = [&](int it, f32x4* dst) {
     const f32x4* __restrict__ src =
         reinterpret_cast<const f32x4*>(a.pcm + (f0 + (size_t)it * P + a.frame_offset) * (size_t)hop);
@@ -190,21 +201,21 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
 #pragma unroll
     for (int c = 0; c < SR; ++c) {
       const int idx = tid + c * NT;
-      if (idx < span4) reinterpret_cast<f32x4*>(&span[it & 1][0])[idx] = val[c];
+      if (idx < span4) reinterpret_cast<f32x4*>(&span[it & (NSPAN - 1)][0])[idx] = val[c];
     }
   };
 
-  f32x4 nxt[SR];
-  load_span(0, nxt);
-  stash_span(0, nxt);
-  __syncthreads();
-  if (K > 1) load_span(1, nxt);
+  // Spans are requested PF iterations ahead of use. A wait for a load also waits for
+  // every store issued before it, so the distance lets each wave keep PF iterations of
+  // stores in flight instead of draining them at every iteration.
+  constexpr int PF = PL ? 1 : (K - 1 < 3 ? (K - 1 > 0 ? K - 1 : 1) : 3);
+  f32x4 nxt[PF][SR];
 
   // Store round c (c < C4) of iteration it's staged run: the P frames of an iteration
   // are one contiguous, 16-byte-aligned run per plane; spare lanes of the last round
   // rewrite the run's final word, so every wave issues the same stores.
   auto store_round = [&](int it, int c) {
-    const float* sA = &stage[it & 1][0];
+    const float* sA = &stage[it & (NSTAGE - 1)][0];
     const float* sB = sA + P * ROW;
     const size_t base = (f0 + (size_t)it * P) * (size_t)BINS;
     int idx = tid + c * NT;
@@ -240,11 +251,23 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
   // rounds of the previous iteration spread over the NP passes of the current one
   constexpr int PER_PASS = (C4 + NP - 1) / NP;
 
+  // The K iterations of the tile at f0; its samples are already in the span buffer when
+  // PL == 2 (the persistent loop stashes them).
+  auto run_tile = [&]() {
+  if constexpr (PL != 2) {
+    load_span(0, nxt[0]);
+    stash_span(0, nxt[0]);
+    __syncthreads();
+  }
+  if constexpr (!PL) {
+#pragma unroll
+    for (int j = 1; j <= PF && j < K; ++j) load_span(j, nxt[j % PF]);
+  }
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     // this slot's frame from the staged span, times the window, first-pass order
     float2 v[E];
-    const float2* fs = reinterpret_cast<const float2*>(&span[i & 1][slot * hop]);
+    const float2* fs = reinterpret_cast<const float2*>(PL ? &span[0][(i * P + slot) * hop] : &span[i & 1][slot * hop]);
 #pragma unroll
     for (int q = 0; q < E / R0; ++q)
 #pragma unroll
@@ -261,11 +284,15 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
       }
     });
     store_natural<L, E, Rl>(v, t, buf);
-    wave_lds_fence();
+    if constexpr (PL) {
+      __syncthreads();  // the one staging buffer: every store round of i-1 has read it
+    } else {
+      wave_lds_fence();
+    }
 
     // split into this iteration's staging buffer (double-buffered across iterations)
-    float* rowA = &stage[i & 1][0] + slot * ROW;
-    float* rowB = &stage[i & 1][0] + P * ROW + slot * ROW;
+    float* rowA = &stage[i & (NSTAGE - 1)][0] + slot * ROW;
+    float* rowB = &stage[i & (NSTAGE - 1)][0] + P * ROW + slot * ROW;
     auto put = [&](int k, float2 X) {
       if (DIR > 0) X = c_scale(X, scale);
       float A, B;
@@ -298,15 +325,52 @@ __global__ void __launch_bounds__(256) stft_r2c_mf_kernel(StftArgs a) {
       put(L / 2, c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x))));
     }
     wave_lds_fence();
-    // next iteration's samples: stash what was prefetched, request the one after
-    if (i + 1 < K) {
-      stash_span(i + 1, nxt);
-      if (i + 2 < K) load_span(i + 2, nxt);
+    // next iteration's samples: stash what was prefetched, request the one PF later
+    if constexpr (!PL) {
+      if (i + 1 < K) {
+        stash_span(i + 1, nxt[(i + 1) % PF]);
+        if (i + 1 + PF < K) load_span(i + 1 + PF, nxt[(i + 1 + PF) % PF]);
+      }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int c = 0; c < C4; ++c) store_round(K - 1, c);
+  };
+
+  if constexpr (PL == 2) {
+    // Persistent: workgroup of rank r (consecutive ranks share an XCD) runs tiles
+    // [r*T/G, (r+1)*T/G) and requests each tile's samples while it runs the one before,
+    // so the wait for them covers only stores a whole tile old. The rank the host picks
+    // (a tile short) also runs the frames after the last tile.
+    const size_t G = gridDim.x;
+    const size_t T = a.lead / (size_t)(K * P);
+    const size_t r = (size_t)xcd_tile(blockIdx.x, gridDim.x);
+    const size_t tb = r * T / G, te = (r + 1) * T / G;
+    f32x4 ahead[SR];
+    if (tb < te) {
+      f0 = tb * (size_t)(K * P);
+      load_span(0, ahead);
+      stash_span(0, ahead);
+    }
+    __syncthreads();
+    for (size_t j = tb; j < te; ++j) {
+      f0 = (j + 1) * (size_t)(K * P);
+      if (j + 1 < te) load_span(0, ahead);
+      f0 = j * (size_t)(K * P);
+      run_tile();
+      __syncthreads();  // every frame of tile j has been read from the span
+      if (j + 1 < te) stash_span(0, ahead);
+      __syncthreads();
+    }
+    if (r == (size_t)a.rest_rank && a.frames > a.lead) do_rest();
+  } else {
+    if (has_rest && blockIdx.x == 0) {
+      do_rest();
+      return;
+    }
+    run_tile();
+  }
 }
 
 // Pre-transform hook on load (eigen_fft.hpp:2484-2533): mode 1 clears imag (R2C);
@@ -538,9 +602,27 @@ hipError_t stft_launch_impl(const StftArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Workgroups of kernel k that the current device holds at once (CUs x occupancy at 256
+// threads), cached per device.
+size_t resident_slots(const void* k) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({k, dev});
+  if (it != cache.end()) return it->second;
+  int cus = 0, occ = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 256, 0) != hipSuccess || occ <= 0) occ = 1;
+  const size_t slots = (size_t)cus * (size_t)occ;
+  cache[{k, dev}] = slots;
+  return slots;
+}
+
 // Multi-frame launch: whole tiles of regular frames plus a short remainder go to one
 // stft_r2c_mf_kernel launch; anything it cannot take goes to stft_r2c_kernel.
-template <int L, int E, int K, int DIR, int... Rs>
+template <int L, int E, int K, int PL, int DIR, int... Rs>
 hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
   constexpr int N = 2 * L;
   constexpr int P = 256 / (L / E);
@@ -562,16 +644,32 @@ hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
     StftArgs r = a;
     r.frames = take;
     r.lead = nreg;
-    const size_t grid = nreg / ((size_t)P * K) + (take > nreg ? 1 : 0);
+    size_t grid = nreg / ((size_t)P * K) + (take > nreg ? 1 : 0);
+    const bool half = a.bins == L + 1;
+    if constexpr (PL == 2) {
+      // one resident round: as many workgroups as the chip holds at once, or fewer
+      const size_t tiles = nreg / ((size_t)P * K);
+      const void* k = half ? reinterpret_cast<const void*>(&stft_r2c_mf_kernel<L, E, K, PL, kOutMagnitude, 1, DIR, Rs...>)
+                           : reinterpret_cast<const void*>(&stft_r2c_mf_kernel<L, E, K, PL, kOutMagnitude, 0, DIR, Rs...>);
+      const size_t slots = resident_slots(k);
+      grid = tiles < slots ? tiles : slots;
+      if (grid == 0) grid = 1;
+      // the rest rides with a rank that runs the smaller tile count
+      r.rest_rank = 0;
+      for (size_t q = 0; q < grid; ++q)
+        if ((q + 1) * tiles / grid - q * tiles / grid == tiles / grid) {
+          r.rest_rank = (int)q;
+          break;
+        }
+    }
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid), blk(256);
-    const bool half = a.bins == L + 1;
 #define FFTFREE_MF_GO(MODE)                                                                    \
   do {                                                                                         \
     if (half)                                                                                  \
-      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, MODE, 1, DIR, Rs...>), g, blk, 0, s, r); \
+      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, PL, MODE, 1, DIR, Rs...>), g, blk, 0, s, r); \
     else                                                                                       \
-      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, MODE, 0, DIR, Rs...>), g, blk, 0, s, r); \
+      hipLaunchKernelGGL((stft_r2c_mf_kernel<L, E, K, PL, MODE, 0, DIR, Rs...>), g, blk, 0, s, r); \
   } while (0)
     switch (a.out_mode) {
       case kOutComplex: FFTFREE_MF_GO(kOutComplex); break;
@@ -621,8 +719,14 @@ struct C2CEntry {
   {2 * (L), {L, E, FPW, RS}, &stft_launch_impl<L, E, FPW, IP, -1, __VA_ARGS__>,           \
    &stft_launch_impl<L, E, FPW, IP, 1, __VA_ARGS__>}
 #define FFTFREE_STFT_MF(L, E, K, RS, ...)                                                \
-  {2 * (L), {L, E, K, RS}, &stft_mf_launch_impl<L, E, K, -1, __VA_ARGS__>,                \
-   &stft_mf_launch_impl<L, E, K, 1, __VA_ARGS__>}
+  {2 * (L), {L, E, K, RS}, &stft_mf_launch_impl<L, E, K, 0, -1, __VA_ARGS__>,             \
+   &stft_mf_launch_impl<L, E, K, 0, 1, __VA_ARGS__>}
+#define FFTFREE_STFT_PL(L, E, K, RS, ...)                                                \
+  {2 * (L), {L, E, K, RS}, &stft_mf_launch_impl<L, E, K, 1, -1, __VA_ARGS__>,             \
+   &stft_mf_launch_impl<L, E, K, 1, 1, __VA_ARGS__>}
+#define FFTFREE_STFT_PP(L, E, K, RS, ...)                                                \
+  {2 * (L), {L, E, K, RS}, &stft_mf_launch_impl<L, E, K, 2, -1, __VA_ARGS__>,             \
+   &stft_mf_launch_impl<L, E, K, 2, 1, __VA_ARGS__>}
 #define FFTFREE_C2C(L, E, FPW, RS, ...)                                                   \
   {L, {L, E, FPW, RS}, &c2c_launch_impl<L, E, FPW, -1, __VA_ARGS__>,                     \
    &c2c_launch_impl<L, E, FPW, 1, __VA_ARGS__>}
@@ -636,9 +740,14 @@ const StftEntry kStft[] = {
     FFTFREE_STFT(128, 16, 32, 1, "16x8 inplace", 16, 8),
     FFTFREE_STFT_MF(256, 16, 4, "16x16 multiframe k=4", 16, 16),
     FFTFREE_STFT(256, 16, 16, 1, "16x16 inplace", 16, 16),
+    FFTFREE_STFT_PL(512, 8, 6, "8x8x8 tile-preload k=6", 8, 8, 8),
+    FFTFREE_STFT_PL(512, 8, 5, "8x8x8 tile-preload k=5", 8, 8, 8),
+    FFTFREE_STFT_PL(512, 8, 4, "8x8x8 tile-preload k=4", 8, 8, 8),
+    FFTFREE_STFT_PP(512, 8, 4, "8x8x8 persistent k=4", 8, 8, 8),
+    FFTFREE_STFT_MF(512, 8, 11, "8x8x8 multiframe k=11", 8, 8, 8),
+    FFTFREE_STFT_MF(512, 8, 6, "8x8x8 multiframe k=6", 8, 8, 8),
     FFTFREE_STFT_MF(512, 8, 8, "8x8x8 multiframe k=8", 8, 8, 8),
     FFTFREE_STFT_MF(512, 8, 4, "8x8x8 multiframe k=4", 8, 8, 8),
-    FFTFREE_STFT_MF(512, 8, 16, "8x8x8 multiframe k=16", 8, 8, 8),
     FFTFREE_STFT(512, 8, 4, 1, "8x8x8 inplace", 8, 8, 8),
     FFTFREE_STFT(512, 8, 4, 0, "8x8x8 pingpong", 8, 8, 8),
     FFTFREE_STFT(512, 8, 8, 1, "8x8x8 inplace", 8, 8, 8),
@@ -668,6 +777,8 @@ const C2CEntry kC2C[] = {
 #undef FFTFREE_STFT
 #undef FFTFREE_C2C
 #undef FFTFREE_STFT_MF
+#undef FFTFREE_STFT_PL
+#undef FFTFREE_STFT_PP
 
 template <class Entry, size_t K>
 const Entry* find_variant(const Entry (&tab)[K], int n, int idx) {
@@ -704,6 +815,37 @@ const VariantInfo* c2c_variant(int n, int idx) {
   return e ? &e->info : nullptr;
 }
 
+// Default kernel variant for an STFT: among the leading table entries of N that share
+// the first entry's kind (same description up to "k="), the tile size K whose whole
+// tiles fill the resident workgroup slots best: fewest rounds x K, ties to the larger K.
+// Tiles are P = 256/(L/E) frames by K; the multi-frame kernels hold 3 workgroups per CU.
+int default_stft_variant(const StftArgs& a) {
+  const int count = count_variants(kStft, a.n);
+  const StftEntry* first = find_variant(kStft, a.n, 0);
+  if (!first || count < 2) return 0;
+  const char* k0 = std::strstr(first->info.radices, "k=");
+  if (!k0) return 0;
+  const size_t stem = (size_t)(k0 - first->info.radices);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  const double slots = 3.0 * (cus > 0 ? cus : 256);
+  int best = 0;
+  double best_cost = 1e300;
+  for (int i = 0; i < count; ++i) {
+    const StftEntry* e = find_variant(kStft, a.n, i);
+    if (std::strncmp(e->info.radices, first->info.radices, stem + 2) != 0) break;
+    const int P = 256 / (e->info.l / e->info.e);
+    const double tiles = std::floor((double)a.frames / (double)(P * e->info.fpw));
+    const double cost = std::ceil(tiles / slots) * e->info.fpw;
+    if (cost < best_cost || (cost == best_cost && e->info.fpw > find_variant(kStft, a.n, best)->info.fpw)) {
+      best_cost = cost;
+      best = i;
+    }
+  }
+  return best;
+}
+
 hipError_t launch_stft(const StftArgs& a, int inverse, int variant, hipStream_t s) {
   if (a.frames == 0) return hipSuccess;
   if (a.n < 16) {
@@ -716,7 +858,8 @@ hipError_t launch_stft(const StftArgs& a, int inverse, int variant, hipStream_t 
       hipLaunchKernelGGL(stft_tiny_kernel<-1>, dim3((unsigned)grid), dim3(kTinyBlock), 0, s, a);
     return hipGetLastError();
   }
-  const StftEntry* e = find_variant(kStft, a.n, variant < 0 ? 0 : variant);
+  if (variant < 0) variant = default_stft_variant(a);
+  const StftEntry* e = find_variant(kStft, a.n, variant);
   if (!e) return hipErrorInvalidValue;
   return inverse ? e->inv(a, s) : e->fwd(a, s);
 }

@@ -29,6 +29,7 @@ struct StftArgs {
   size_t frame_offset = 0;     // first frame index (ragged / chunked callers)
   float* sink = nullptr;       // N floats of scratch that skipped planes are written to
   size_t lead = 0;             // multi-frame kernel: frames covered by whole tiles
+  int rest_rank = 0;           // persistent multi-frame kernel: the rank that runs the rest
 };
 
 // Complex batched transform (fft_inplace_batched semantics, eigen_fft.hpp:3152-3175):
