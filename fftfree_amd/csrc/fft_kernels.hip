@@ -244,9 +244,10 @@ __global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
     }
     // a skipped plane lands in the sink (N floats, never read)
     const int sidx = idx & (N / 4 - 1);
-    reinterpret_cast<f32x4*>(a.o_re ? a.o_re + base : a.sink)[a.o_re ? idx : sidx] = o0;
-    reinterpret_cast<f32x4*>(a.o_im ? a.o_im + base : a.sink)[a.o_im ? idx : sidx] = o1;
-    reinterpret_cast<f32x4*>(a.o_mag ? a.o_mag + base : a.sink)[a.o_mag ? idx : sidx] = o2;
+    // planes are written once and not read back here: streaming (non-temporal) stores
+    __builtin_nontemporal_store(o0, reinterpret_cast<f32x4*>(a.o_re ? a.o_re + base : a.sink) + (a.o_re ? idx : sidx));
+    __builtin_nontemporal_store(o1, reinterpret_cast<f32x4*>(a.o_im ? a.o_im + base : a.sink) + (a.o_im ? idx : sidx));
+    __builtin_nontemporal_store(o2, reinterpret_cast<f32x4*>(a.o_mag ? a.o_mag + base : a.sink) + (a.o_mag ? idx : sidx));
   };
   // rounds of the previous iteration spread over the NP passes of the current one
   constexpr int PER_PASS = (C4 + NP - 1) / NP;
@@ -355,12 +356,14 @@ __global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
     }
     __syncthreads();
     for (size_t j = tb; j < te; ++j) {
-      f0 = (j + 1) * (size_t)(K * P);
-      if (j + 1 < te) load_span(0, ahead);
+      // unconditional request and stash (the last tile re-requests itself), so that no
+      // load is still pending on the loop's back edge
+      f0 = (j + 1 < te ? j + 1 : j) * (size_t)(K * P);
+      load_span(0, ahead);
       f0 = j * (size_t)(K * P);
       run_tile();
       __syncthreads();  // every frame of tile j has been read from the span
-      if (j + 1 < te) stash_span(0, ahead);
+      stash_span(0, ahead);
       __syncthreads();
     }
     if (r == (size_t)a.rest_rank && a.frames > a.lead) do_rest();

@@ -51,11 +51,11 @@ __device__ __forceinline__ void store_plane_run(float* __restrict__ dst, const f
         o.z = value(4 * i + 2);
         o.w = value(4 * i + 3);
       }
-      reinterpret_cast<f32x4*>(dst)[i] = o;
+      __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(dst) + i);  // written once: streaming
     }
-    for (int i = (n4 << 2) + tid; i < count; i += nthreads) dst[i] = value(i);
+    for (int i = (n4 << 2) + tid; i < count; i += nthreads) __builtin_nontemporal_store(value(i), dst + i);
   } else {
-    for (int i = tid; i < count; i += nthreads) dst[i] = value(i);
+    for (int i = tid; i < count; i += nthreads) __builtin_nontemporal_store(value(i), dst + i);
   }
 }
 
