@@ -90,6 +90,14 @@ __device__ __forceinline__ void fetch_frame(float2* v, int t, const float* __res
     cnt = (int)(rem < (size_t)window ? rem : (size_t)window);
   }
   const float* __restrict__ src = x + (valid ? start : 0);
+  if (cnt == 2 * L && ((((uintptr_t)src) & 7) == 0)) {  // whole frame: paired loads
+    const float2* __restrict__ s2 = reinterpret_cast<const float2*>(src);
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = s2[t + q * T + r * (L / R0)];
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < E / R0; ++q)
 #pragma unroll
