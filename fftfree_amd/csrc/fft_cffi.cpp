@@ -390,11 +390,12 @@ struct RaggedPlan {
   std::vector<int> req_ctx;
   std::vector<StftArgs> args;
   int mode = 0;
+  int family = 0;                    // ragged kernel: 0 = E=8 shapes, 1 = E=16 shapes
+  int grids[2] = {0, 0};             // resident workgroups of each kernel
   std::map<int, int> cfg_of_n;       // N -> ragged tile shape
   std::map<int, int> tiles_of_n;     // N -> tiles per work item
   DeviceBuffer d_args, d_items, d_chunks;
   size_t n_items = 0;
-  int grid = 0;
   int device = 0;
   hipStream_t stream = nullptr;      // private stream for capture and tuning
   hipGraph_t graph = nullptr;
@@ -469,7 +470,7 @@ struct RaggedPlan {
 
   void launch(hipStream_t s) const {
     hip_check(launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(), mode,
-                                 grid, s),
+                                 family, grids[family], s),
               "launch_stft_ragged");
   }
 
@@ -485,7 +486,7 @@ struct RaggedPlan {
     graph = nullptr;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t le = launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(),
-                                       mode, grid, stream);
+                                       mode, family, grids[family], stream);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(stream, &g);
     hip_check(le, "launch_stft_ragged (capture)");
@@ -494,45 +495,58 @@ struct RaggedPlan {
     hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }
 
-  // Time the persistent launch over the requests of one N with each tile shape for that
-  // N and each tiles-per-item choice; keep the fastest.
+  // Per kernel family: for every N, time the persistent launch over that N's requests
+  // with the family's tile shape and each tiles-per-item choice, keep the fastest; then
+  // time the whole batch per family and keep the faster family.
+  float time_launches(const std::vector<size_t>& only, int repeats, hipEvent_t e0, hipEvent_t e1) {
+    std::vector<RaggedItem> items;
+    std::vector<unsigned> chunks;
+    build(only, items, chunks);
+    upload(items, chunks);
+    launch(stream);  // warm
+    hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+    for (int i = 0; i < repeats; ++i) launch(stream);
+    hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    return ms;
+  }
+
   void tune(int repeats) {
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
-    for (auto& kv : cfg_of_n) {
-      const int n = kv.first;
-      std::vector<size_t> only;
-      for (size_t r = 0; r < reqs.size(); ++r)
-        if (reqs[r].window == n) only.push_back(r);
-      float best = 1e30f;
-      int best_cfg = kv.second, best_tiles = tiles_of_n[n];
-      for (int cfg = 0; cfg < ragged_cfg_count(); ++cfg) {
-        if (2 * ragged_cfg(cfg)->l != n) continue;
+    std::map<int, int> best_cfg[2], best_tiles[2];
+    float family_ms[2] = {0, 0};
+    for (int fam = 0; fam < 2; ++fam) {
+      family = fam;
+      for (auto& kv : cfg_of_n) {
+        const int n = kv.first;
+        int shape = -1;
+        for (int cfg = 0; cfg < ragged_cfg_count(); ++cfg)
+          if (2 * ragged_cfg(cfg)->l == n && ragged_cfg_family(cfg) == fam) shape = cfg;
+        cfg_of_n[n] = shape;
+        std::vector<size_t> only;
+        for (size_t r = 0; r < reqs.size(); ++r)
+          if (reqs[r].window == n) only.push_back(r);
+        float best = 1e30f;
         for (int per : {1, 2, 4, 8}) {
-          cfg_of_n[n] = cfg;
           tiles_of_n[n] = per;
-          std::vector<RaggedItem> items;
-          std::vector<unsigned> chunks;
-          build(only, items, chunks);
-          upload(items, chunks);
-          launch(stream);  // warm
-          hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-          for (int i = 0; i < repeats; ++i) launch(stream);
-          hip_check(hipEventRecord(e1, stream), "hipEventRecord");
-          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-          float ms = 0;
-          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+          const float ms = time_launches(only, repeats, e0, e1);
           if (ms < best) {
             best = ms;
-            best_cfg = cfg;
-            best_tiles = per;
+            best_tiles[fam][n] = per;
           }
         }
+        tiles_of_n[n] = best_tiles[fam][n];
+        best_cfg[fam][n] = shape;
       }
-      cfg_of_n[n] = best_cfg;
-      tiles_of_n[n] = best_tiles;
+      family_ms[fam] = time_launches({}, repeats, e0, e1);
     }
+    family = family_ms[1] < family_ms[0] ? 1 : 0;
+    cfg_of_n = best_cfg[family];
+    tiles_of_n = best_tiles[family];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
@@ -1178,17 +1192,19 @@ void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode
       p->args.push_back(a);
       p->cfg_of_n[q.window] = -1;
     }
-    // defaults before tuning: the E=8 shape and 4 tiles per item
+    // defaults before tuning: the E=8 family and 2 tiles per item
     for (auto& kv : p->cfg_of_n) {
       for (int cfg = 0; cfg < fftfree::ragged_cfg_count(); ++cfg)
-        if (2 * fftfree::ragged_cfg(cfg)->l == kv.first && kv.second < 0) kv.second = cfg;
-      p->tiles_of_n[kv.first] = 4;
+        if (2 * fftfree::ragged_cfg(cfg)->l == kv.first && fftfree::ragged_cfg_family(cfg) == 0) kv.second = cfg;
+      p->tiles_of_n[kv.first] = 2;
     }
     p->mode = store_polar ? 3 : reduce_magnitude ? 2 : 0;
     fftfree::hip_check(hipGetDevice(&p->device), "hipGetDevice");
     fftfree::hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
-    fftfree::hip_check(fftfree::ragged_grid(p->mode, &p->grid), "ragged_grid");
-    if (p->grid < 8) p->grid = 8;
+    for (int fam = 0; fam < 2; ++fam) {
+      fftfree::hip_check(fftfree::ragged_grid(p->mode, fam, &p->grids[fam]), "ragged_grid");
+      if (p->grids[fam] < 8) p->grids[fam] = 8;
+    }
     p->d_args.reserve(p->args.size() * sizeof(fftfree::StftArgs));
     fftfree::hip_check(hipMemcpy(p->d_args.ptr, p->args.data(), p->args.size() * sizeof(fftfree::StftArgs),
                                  hipMemcpyHostToDevice),
@@ -1242,7 +1258,8 @@ int fft_ragged_execute_plain(void* ragged, void* stream) {
 int fft_ragged_describe(void* ragged, char* buf, size_t buflen) {
   if (!ragged || !buf || buflen == 0) return 0;
   auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
-  std::string out = "grid=" + std::to_string(p->grid) + " items=" + std::to_string(p->n_items);
+  std::string out = "kernel=" + std::string(p->family ? "E16" : "E8") + " grid=" + std::to_string(p->grids[p->family]) +
+                    " items=" + std::to_string(p->n_items);
   for (auto& kv : p->cfg_of_n) {
     const fftfree::VariantInfo* v = fftfree::ragged_cfg(kv.second);
     out += " N" + std::to_string(kv.first) + ":" + v->radices + "/fpw" + std::to_string(v->fpw) + "/x" +

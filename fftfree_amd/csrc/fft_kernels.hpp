@@ -133,13 +133,14 @@ struct RaggedItem {
 };
 int ragged_cfg_count();
 const VariantInfo* ragged_cfg(int id);  // l = N/2; 256 threads = fpw * l / e
-// Resident workgroups of the ragged kernel on the current device (CUs x occupancy).
-hipError_t ragged_grid(int mode, int* grid);
+int ragged_cfg_family(int id);          // the ragged kernel (0: E=8 shapes, 1: E=16) that runs it
+// Resident workgroups of a ragged kernel on the current device (CUs x occupancy).
+hipError_t ragged_grid(int mode, int family, int* grid);
 // One persistent launch over every item: workgroups of XCD x (blockIdx % 8 == x) stride
 // through items[chunks[x] .. chunks[x+1]). Forward R2C, half spectrum, output mode
 // kOutComplex / kOutMagnitude / kOutPolar.
 hipError_t launch_stft_ragged(const StftArgs* reqs, const RaggedItem* items, const unsigned* chunks, int mode,
-                              int grid, hipStream_t s);
+                              int family, int grid, hipStream_t s);
 int stft_variant_count(int n);
 const VariantInfo* stft_variant(int n, int idx);
 int c2c_variant_count(int n);

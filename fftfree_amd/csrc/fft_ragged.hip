@@ -58,7 +58,10 @@ __device__ __noinline__ void ragged_tile(const StftArgs& a, size_t tile, float2*
   StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::run(a, tile, lds, nullptr);
 }
 
-template <int MODE>
+// Two kernels: family 0 holds the E=8 shapes, family 1 the E=16 ones. A kernel's register
+// allocation is that of its largest shape, so keeping the lean shapes apart lets them
+// run at a higher occupancy; the autotuner picks the family for the batch.
+template <int MODE, int FAM>
 __global__ void __launch_bounds__(256) stft_ragged_kernel(const StftArgs* __restrict__ reqs,
                                                           const RaggedItem* __restrict__ items,
                                                           const unsigned* __restrict__ chunks) {
@@ -71,9 +74,10 @@ __global__ void __launch_bounds__(256) stft_ragged_kernel(const StftArgs* __rest
     for (unsigned k = 0; k < w.ntiles; ++k) {
       const size_t tile = (size_t)w.tile + k;
       switch (w.cfg) {
-#define FFTFREE_RAGGED_CASE(ID, L, E, FPW, NAME, ...)             \
-  case ID:                                                        \
-    ragged_tile<MODE, ID, L, E, FPW, __VA_ARGS__>(a, tile, lds); \
+#define FFTFREE_RAGGED_CASE(ID, L, E, FPW, NAME, ...)                        \
+  case ID:                                                                   \
+    if constexpr ((E == 16) == (FAM == 1))                                    \
+      ragged_tile<MODE, ID, L, E, FPW, __VA_ARGS__>(a, tile, lds);            \
     break;
         FFTFREE_RAGGED_CFGS(FFTFREE_RAGGED_CASE)
 #undef FFTFREE_RAGGED_CASE
@@ -86,15 +90,16 @@ __global__ void __launch_bounds__(256) stft_ragged_kernel(const StftArgs* __rest
 }
 
 template <int MODE>
-const void* ragged_kernel_ptr() {
-  return reinterpret_cast<const void*>(&stft_ragged_kernel<MODE>);
+const void* ragged_kernel_ptr(int family) {
+  return family ? reinterpret_cast<const void*>(&stft_ragged_kernel<MODE, 1>)
+                : reinterpret_cast<const void*>(&stft_ragged_kernel<MODE, 0>);
 }
 
-const void* ragged_kernel(int mode) {
+const void* ragged_kernel(int mode, int family) {
   switch (mode) {
-    case kOutComplex: return ragged_kernel_ptr<kOutComplex>();
-    case kOutMagnitude: return ragged_kernel_ptr<kOutMagnitude>();
-    case kOutPolar: return ragged_kernel_ptr<kOutPolar>();
+    case kOutComplex: return ragged_kernel_ptr<kOutComplex>(family);
+    case kOutMagnitude: return ragged_kernel_ptr<kOutMagnitude>(family);
+    case kOutPolar: return ragged_kernel_ptr<kOutPolar>(family);
     default: return nullptr;
   }
 }
@@ -103,9 +108,10 @@ const void* ragged_kernel(int mode) {
 
 int ragged_cfg_count() { return kRaggedCfgCount; }
 const VariantInfo* ragged_cfg(int id) { return id >= 0 && id < kRaggedCfgCount ? &kRaggedCfgs[id] : nullptr; }
+int ragged_cfg_family(int id) { return id >= 0 && id < kRaggedCfgCount && kRaggedCfgs[id].e == 16 ? 1 : 0; }
 
-hipError_t ragged_grid(int mode, int* grid) {
-  const void* k = ragged_kernel(mode);
+hipError_t ragged_grid(int mode, int family, int* grid) {
+  const void* k = ragged_kernel(mode, family);
   if (!k) return hipErrorInvalidValue;
   int dev = 0, cus = 0, occ = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -117,8 +123,8 @@ hipError_t ragged_grid(int mode, int* grid) {
 }
 
 hipError_t launch_stft_ragged(const StftArgs* reqs, const RaggedItem* items, const unsigned* chunks, int mode,
-                              int grid, hipStream_t s) {
-  const void* k = ragged_kernel(mode);
+                              int family, int grid, hipStream_t s) {
+  const void* k = ragged_kernel(mode, family);
   if (!k || grid <= 0) return hipErrorInvalidValue;
   void* args[] = {&reqs, &items, &chunks};
   return hipLaunchKernel(k, dim3((unsigned)grid), dim3(256), args, 0, s);
