@@ -18,7 +18,11 @@ W/4, polar planes: one autotuned persistent launch per graph replay vs one plain
 per request); stft_grad (differentiable STFT: 32 signals of 2^18 samples, window 2048, hop 512, polar
 planes; forward kernel alone vs forward + adjoint kernel, and a whole torch autograd step
 of the loss sum (|X| - |Y|)^2); stft_host (the default workload through the reference's host-pointer entry
-fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); copy
+fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); istft (the
+inverse of the default workload through fft_execute_complex_batched_device: 65533 frames x
+513 complex bins -> Hermitian rebuild, 1024-point inverse, Hann synthesis window,
+overlap-add with COLA normalisation into 65533*1024 output samples, zeros past the
+overlap-add included, as the reference writes them); copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
 WRITE_SIZE counters).
 """
@@ -41,7 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "c2c1024", "c2c_large", "stft_grad", "ragged", "copy"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "c2c1024", "c2c_large", "stft_grad", "ragged",
+                                                     "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -157,6 +162,43 @@ def main():
         }
         if not args.no_cpu:
             line["cpu_baseline"] = cpu_baseline_stft(pcm, n, W, H, args.cpu_seconds)
+    elif args.workload == "istft":
+        n, W, H, L = 1024, 1024, 256, 1 << 24
+        fwd = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, half_spectrum=True, apply_windows=True,
+                           analysis_window=1, pad_mode=0)
+        inv = ff.make_plan(n, window=W, hop=H, inverse=1, kernel=2, transform=2, half_spectrum=True,
+                           apply_windows=True, apply_ola=True, analysis_window=1, synthesis_window=1, cola=1)
+        pcm = noise_plus_sines(L, seed=2)
+        frames = ff.frame_count(fwd, L, 0)
+        bins = n // 2 + 1
+        d_pcm = torch.from_numpy(pcm).cuda()
+        re = torch.empty((frames, bins), device="cuda")
+        im = torch.empty((frames, bins), device="cuda")
+        ff.stft_device(fwd, d_pcm, re, im, None, pad_mode=0)
+        out = torch.empty(frames * n, device="cuda")
+
+        def step():
+            ff.complex_batched_device(inv, re, im, out)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per_launch = gpu_s / args.steps
+        span = (frames - 1) * H + n
+        # interior samples (every frame present): the round trip returns the signal
+        inner = slice(n, span - n)
+        rt_err = float(torch.linalg.vector_norm(out[inner] - d_pcm[inner]) / torch.linalg.vector_norm(d_pcm[inner]))
+        alg_bytes = frames * (8 * bins + 4 * n)
+        achieved = alg_bytes / per_launch / 1e9
+        line = {"metric": "inverse STFT frames/s (fft_execute_complex_batched, overlap-add + COLA), N=1024",
+                "value": frames * args.steps / gpu_s, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": per_launch * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic: planes of the forward STFT of 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
+                "config": {"workload": "inverse C2R half spectrum, Hann synthesis 1024, hop 256, overlap-add with "
+                                       "COLA normalisation, 65533 frames x 513 bins -> 65533*1024 samples",
+                           "n": n, "hop": H, "frames": frames, "interior_roundtrip_rel_l2": rt_err},
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("istft_n1024"),
+                             "algorithmic_bytes_per_launch": alg_bytes, "kernel": "istft_kernel"}}
     elif args.workload == "c2c_large":
         n, B = 1 << args.lg, 16
         fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)

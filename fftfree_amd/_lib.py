@@ -55,6 +55,8 @@ SIGNATURES = {
                                               c_size_t, c_void_p]),
     "fft_execute_c2c_batched": (c_size_t, [c_void_p, f32p, f32p, c_size_t]),
     "fft_execute_c2c_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "fft_execute_complex_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
+                                                      c_void_p]),
     "fft_stft_signals_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
                                            c_int, c_void_p]),
     "fft_stft_adjoint_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,

@@ -204,6 +204,26 @@ class CffiFft:
             raise RuntimeError("fft_execute_c2c_batched_device failed")
         return out
 
+    def complex_batched_device(self, plan: FftPlan, in_re, in_im, out=None, max_frames: int = 0):
+        """fft_execute_complex_batched_device: in_re / in_im are (frames, bins) float32 CUDA
+        tensors; returns the (frames * n,) float32 output (allocated when out is None)."""
+        import torch
+        assert in_re.shape == in_im.shape and in_re.dim() == 2 and in_re.shape[1] == plan.bins
+        assert in_re.is_contiguous() and in_im.is_contiguous()
+        frames = int(in_re.shape[0])
+        if max_frames:
+            frames = min(frames, max_frames)
+        if out is None:
+            out = torch.empty(frames * plan.n, dtype=torch.float32, device=in_re.device)
+        assert out.numel() >= frames * plan.n and out.is_contiguous()
+        got = int(self.lib.fft_execute_complex_batched_device(plan.handle, ctypes.c_void_p(in_re.data_ptr()),
+                                                              ctypes.c_void_p(in_im.data_ptr()), frames,
+                                                              ctypes.c_void_p(out.data_ptr()), max_frames,
+                                                              self._stream()))
+        if got == 0:
+            raise RuntimeError("fft_execute_complex_batched_device failed")
+        return out
+
     def stft_signals_device(self, plan: FftPlan, pcm, out_re=None, out_im=None, out_mag=None,
                             pad_mode: int = 0) -> int:
         """fft_stft_signals_device: pcm is a (signals, length) float32 CUDA tensor; planes

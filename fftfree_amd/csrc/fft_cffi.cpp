@@ -934,80 +934,100 @@ size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t
   }
 }
 
+namespace fftfree {
+namespace {
+
+// The device form of fft_execute_complex_batched (fft_cffi.cpp:1042-1157).
+size_t complex_batched(FftContext* c, const float* d_re, const float* d_im, size_t frames, float* d_out,
+                       hipStream_t s) {
+  IstftArgs a;
+  a.in_re = d_re;
+  a.in_im = d_im;
+  a.frames = frames;
+  a.n = c->N;
+  a.bins = (int)bins_of(c);
+  a.hop = c->hop > 0 ? c->hop : (c->window > 0 ? c->window : c->N);  // fft_cffi.cpp:1120
+  a.tw = c->plan->W;
+  a.swin = (c->windows_enabled && !c->synthesis_win.empty()) ? c->d_swin.as<float>() : nullptr;
+  a.awin = (c->windows_enabled && !c->analysis_win.empty()) ? c->d_awin.as<float>() : nullptr;
+  a.ola = c->apply_ola ? 1 : 0;
+  a.cola = c->cola_mode == FFT_COLA_NORMALIZE ? 1 : 0;
+  a.out = d_out;
+  // An inverse C2R half-spectrum plan without a polar/magnitude post hook computes
+  // x = Re IDFT of the Hermitian rebuild on either kernel: the fused kernel.
+  const bool fused = c->inverse && c->transform == FFT_TRANSFORM_C2R && c->half_spectrum && !c->store_polar &&
+                     !c->reduce_magnitude && istft_fused_supported(c->N, a.hop, a.ola);
+  if (fused) {
+    hip_check(launch_istft(a, s), "launch_istft");
+    return frames;
+  }
+  const size_t bytes = frames * (size_t)c->N * sizeof(float2);
+  c->d_cplx.reserve(bytes);
+  const int hermitian = (c->half_spectrum && c->transform == FFT_TRANSFORM_C2R) ? 1 : 0;
+  hip_check(launch_complex_columns(a, hermitian, c->d_cplx.as<float2>(), s), "launch_complex_columns");
+  AxisLayout l;
+  l.base = c->d_cplx.as<float2>();
+  l.axis_size = c->N;
+  l.batch_size = (int64_t)frames;
+  l.axis_stride = 1;
+  l.batch_stride = c->N;
+  c->dispatcher.set_stream(s);
+  try {
+    fft_apply_axis(*c->plan, l);
+  } catch (...) {
+    c->dispatcher.set_stream(c->stream);
+    throw;
+  }
+  c->dispatcher.set_stream(c->stream);
+  hip_check(launch_overlap(a, c->d_cplx.as<float2>(), s), "launch_overlap");
+  return frames;
+}
+
+}  // namespace
+}  // namespace fftfree
+
+size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
+                                          size_t frames, float* d_out_pcm, size_t max_frames, void* stream) {
+  if (!handle || !d_in_real || !d_in_imag || !d_out_pcm || frames == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (c->N <= 0) return 0;
+  if (max_frames != 0 && frames > max_frames) frames = max_frames;
+  try {
+    fftfree::ensure_gpu(c);
+    return fftfree::complex_batched(c, d_in_real, d_in_imag, frames, d_out_pcm, static_cast<hipStream_t>(stream));
+  } catch (const std::exception& e) {
+    fftfree::report("fft_execute_complex_batched_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
 size_t fft_execute_complex_batched(void* handle, const float* in_real, const float* in_imag, size_t frames,
                                    float* out_pcm, int pad_mode, int enable_backup, size_t max_frames) {
-  (void)pad_mode;
-  (void)enable_backup;
+  (void)pad_mode;       // not read by the reference either
+  (void)enable_backup;  // selects the reference's retry path, same output
   if (!handle || !in_real || !in_imag || !out_pcm || frames == 0) return 0;
   auto* c = static_cast<FftContext*>(handle);
+  if (c->N <= 0) return 0;
   if (max_frames != 0 && frames > max_frames) frames = max_frames;
   try {
     fftfree::ensure_gpu(c);
     const size_t N = (size_t)c->N;
     const size_t bins = fftfree::bins_of(c);
-    // Column rebuild exactly as fft_cffi.cpp:1062-1098 (Hermitian mirror for a C2R half
-    // spectrum, zero fill otherwise), then the plan kernel with its hooks.
-    std::vector<std::complex<float>> buf(frames * N);
-    for (size_t f = 0; f < frames; ++f) {
-      std::complex<float>* col = buf.data() + f * N;
-      for (size_t i = 0; i < bins; ++i) col[i] = std::complex<float>(in_real[f * bins + i], in_imag[f * bins + i]);
-      if (c->half_spectrum && c->transform == FFT_TRANSFORM_C2R) {
-        col[0].imag(0.0f);
-        const size_t ny = N / 2;
-        if (ny < bins) col[ny].imag(0.0f);
-        const size_t maxk = (bins > ny + 1) ? ny : (bins - 1);
-        for (size_t k = 1; k < maxk; ++k) col[N - k] = std::conj(col[k]);
-      }
-    }
-    const size_t bytes = frames * N * sizeof(float2);
-    c->d_cplx.reserve(bytes);
-    fftfree::hip_check(hipMemcpyAsync(c->d_cplx.ptr, buf.data(), bytes, hipMemcpyHostToDevice, c->stream), "H2D");
-    fftfree::AxisLayout l;
-    l.base = c->d_cplx.as<float2>();
-    l.axis_size = c->N;
-    l.batch_size = (int64_t)frames;
-    l.batch_stride = c->N;
-    fftfree::fft_apply_axis(*c->plan, l);
-    fftfree::hip_check(hipMemcpyAsync(buf.data(), c->d_cplx.ptr, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+    const size_t in_bytes = frames * bins * sizeof(float);
+    const size_t out_bytes = frames * N * sizeof(float);
+    // staging: the two input planes, then the output
+    const size_t in_off = (2 * in_bytes + 255) & ~size_t(255);
+    c->d_planes.reserve(in_off + out_bytes);
+    float* d_re = c->d_planes.as<float>();
+    float* d_im = d_re + frames * bins;
+    float* d_out = reinterpret_cast<float*>(c->d_planes.as<char>() + in_off);
+    fftfree::hip_check(hipMemcpyAsync(d_re, in_real, in_bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    fftfree::hip_check(hipMemcpyAsync(d_im, in_imag, in_bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    fftfree::complex_batched(c, d_re, d_im, frames, d_out, c->stream);
+    fftfree::hip_check(hipMemcpyAsync(out_pcm, d_out, out_bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
     fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
-    const bool win = c->windows_enabled && !c->synthesis_win.empty();
-    if (c->apply_ola) {
-      // fft_cffi.cpp:1119-1145
-      const int H = c->hop > 0 ? c->hop : (c->window > 0 ? c->window : c->N);
-      const size_t total = frames * N;
-      std::fill(out_pcm, out_pcm + total, 0.0f);
-      std::vector<float> norm;
-      if (c->cola_mode == FFT_COLA_NORMALIZE) norm.assign(total, 0.0f);
-      for (size_t f = 0; f < frames; ++f) {
-        const size_t start = f * (size_t)H;
-        for (size_t i = 0; i < N; ++i) {
-          float s = buf[f * N + i].real();
-          if (win) s *= c->synthesis_win[i];
-          const size_t t = start + i;
-          if (t < total) {
-            out_pcm[t] += s;
-            if (!norm.empty()) {
-              float a = 1.0f, b = 1.0f;
-              if (c->windows_enabled) {
-                if (i < c->analysis_win.size()) a = c->analysis_win[i];
-                if (i < c->synthesis_win.size()) b = c->synthesis_win[i];
-              }
-              norm[t] += a * b;
-            }
-          }
-        }
-      }
-      if (!norm.empty())
-        for (size_t t = 0; t < total; ++t)
-          if (norm[t] > 1e-12f) out_pcm[t] /= norm[t];
-    } else {
-      for (size_t f = 0; f < frames; ++f)
-        for (size_t i = 0; i < N; ++i) {
-          float s = buf[f * N + i].real();
-          if (win) s *= c->synthesis_win[i];
-          out_pcm[f * N + i] = s;
-        }
-    }
     return frames;
   } catch (const std::exception& e) {
     fftfree::report("fft_execute_complex_batched", e.what());

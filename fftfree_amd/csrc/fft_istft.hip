@@ -1,0 +1,421 @@
+// fft_istft.hip -- fft_execute_complex_batched on the device: frame-major complex bins
+// in, one transform per frame, real part times the synthesis window out, either frame by
+// frame or overlap-added (with optional COLA normalisation).
+//
+// Reference semantics (fft_cffi.cpp:1042-1157):
+//  * column rebuild (1062-1098): bins 0..bins-1 from the planes; for a C2R half-spectrum
+//    context, imag(X_0) = imag(X_{N/2}) = 0 and X_{N-k} = conj X_k for 0 < k < N/2; else
+//    zeros above the given bins;
+//  * the context's plan with its hooks (pre/post eigen_fft.hpp:2484-2596; inverse 1/N);
+//  * s = Re(x_f[i]) * ws[i] (ws only when windows are on), then either out[f*N + i] = s,
+//    or out = 0 over frames*N samples and out[f*H + i] += s for f*H + i < frames*N, in
+//    ascending frame order; with COLA normalisation, norm[t] += wa[i]*ws[i] (1 when
+//    windows are off, cola_window_weight 210-226) alongside, and out[t] /= norm[t]
+//    wherever norm[t] > 1e-12.
+//
+// Fused kernel (istft_kernel) for an inverse C2R half-spectrum plan with N = 2L in
+// {512, 1024, 2048} and H <= N: x = IDFT_N(X) is real, so with e_p = e^{+2 pi i p/N}
+//   2 Z_p = (X_p + conj X_{L-p}) + i e_p (X_p - conj X_{L-p})   (0 < p < L),
+//   2 Z_0 = (Re X_0 + Re X_L) + i (Re X_0 - Re X_L),   2 Z_{L/2} = 2 conj X_{L/2},
+// and the unscaled L-point inverse y = IDFT(2Z) gives x[2m] = Re y_m / N,
+// x[2m+1] = Im y_m / N. The inverse runs as conj(DFT(conj 2Z)) on the forward engine.
+// Only Re X_0 and Re X_L enter, which is the rebuild's imag(X_0) = imag(X_{N/2}) = 0.
+//
+// A workgroup owns a run of frame positions and overlap-adds its frames, in ascending
+// order, in an LDS ring; a sample is written once, when no later frame touches it. The
+// ceil(N/H) - 1 frames before a run that reach into it are transformed again by that
+// run (warm-up steps whose samples are not written), so runs need no exchange and the
+// whole result, COLA division included, comes from one launch. The samples past the
+// last frame, up to frames*N, are zero-filled by the same launch.
+#include "fft_device.hpp"
+#include "fft_kernels.hpp"
+
+namespace fftfree {
+using namespace dev;
+
+namespace {
+
+constexpr int kFps = 4;  // frames per step: one 64-lane wave per frame
+
+enum IstftMode : int { kFrameMajor = 0, kOverlap = 1, kOverlapCola = 2 };
+
+// COLA weight of frame tap n (fft_cffi.cpp:210-226).
+__device__ __forceinline__ float cola_weight(const IstftArgs& a, int n) {
+  const float wa = a.awin ? a.awin[n] : 1.0f;
+  const float ws = a.swin ? a.swin[n] : 1.0f;
+  return wa * ws;
+}
+
+// Stores zeros over out[from, to): float4 where aligned, split over the grid.
+__device__ __forceinline__ void zero_tail(float* __restrict__ out, size_t from, size_t to, int rank, int ranks) {
+  if (from >= to) return;
+  float* base = out + from;
+  const size_t len = to - from;
+  size_t head = ((16 - ((uintptr_t)base & 15)) & 15) >> 2;
+  if (head > len) head = len;
+  if (rank == 0 && threadIdx.x < head) base[threadIdx.x] = 0.0f;
+  const size_t n4 = (len - head) >> 2;
+  const size_t per = (n4 + ranks - 1) / ranks;
+  const size_t b = (size_t)rank * per;
+  const size_t e = b + per < n4 ? b + per : n4;
+  f32x4* z4 = reinterpret_cast<f32x4*>(base + head);
+  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) __builtin_nontemporal_store(zero, &z4[i]);
+  const size_t rest = (len - head) & 3;
+  if (rank == ranks - 1 && threadIdx.x < rest) base[head + 4 * n4 + threadIdx.x] = 0.0f;
+}
+
+template <int L, int E, int MODE, int ACCM, int... Rs>
+__global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
+  constexpr int T = L / E;
+  constexpr int N = 2 * L;
+  constexpr int NP = sizeof...(Rs);
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int LP = padded_len<L>();
+  constexpr int JF = (L / 2) / T;
+  constexpr int BINS = L + 1;
+  constexpr int ACC = MODE == kFrameMajor ? 2 : ACCM * N;  // ring; the host checks (FPS-1)*H + N <= ACC
+  static_assert(T == 64 && JF * T == L / 2, "one frame per wave");
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  __shared__ float2 fbuf[kFps][LP];
+  __shared__ float acc[ACC];
+  __shared__ float wl[N];                                  // synthesis taps / N
+  __shared__ float nh[MODE == kOverlapCola ? N : 1];      // COLA sums per sample residue mod H
+
+  const int tid = threadIdx.x;
+  const int slot = tid >> 6;
+  const int t = tid & 63;
+  float2* buf = &fbuf[slot][0];
+  const int rank = xcd_tile(blockIdx.x, gridDim.x);
+  const long long frames = (long long)a.frames;
+  const int H = a.hop;
+  const long long f0 = (long long)rank * kFps * a.steps;
+
+  float2 twr[E * NP];
+  StockhamPlan<L, E, -1, Rs...>::load_twiddles(twr, t, a.tw, 2);
+  float2 ep[JF];  // e^{+2 pi i p/N}
+#pragma unroll
+  for (int j = 0; j < JF; ++j) ep[j] = load_tw<1>(a.tw, t + j * T);
+  for (int i = tid; i < N; i += 256) wl[i] = (a.swin ? a.swin[i] : 1.0f) * (1.0f / (float)N);
+  if constexpr (MODE == kOverlapCola) {
+    // norm of a sample whose frames are all present: taps r + kH, frames ascending
+    // (taps descending), as the reference accumulates
+    for (int r = tid; r < H; r += 256) {
+      float s = 0.0f;
+      for (int n = r + ((N - 1 - r) / H) * H; n >= 0; n -= H) s += cola_weight(a, n);
+      nh[r] = s;
+    }
+  }
+  if constexpr (MODE != kFrameMajor)
+    for (int i = tid; i < ACC; i += 256) acc[i] = 0.0f;
+  __syncthreads();
+
+  // bins of the frame a step ahead, in registers
+  float xr[2 * JF], xi[2 * JF], mr = 0.0f, mi = 0.0f;
+  auto fetch = [&](long long f) {
+    const bool valid = f >= 0 && f < frames;
+    const float* __restrict__ re = a.in_re + (valid ? (size_t)f * BINS : 0);
+    const float* __restrict__ im = a.in_im + (valid ? (size_t)f * BINS : 0);
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      xr[2 * j] = valid ? re[p] : 0.0f;
+      xi[2 * j] = valid ? im[p] : 0.0f;
+      xr[2 * j + 1] = valid ? re[L - p] : 0.0f;
+      xi[2 * j + 1] = valid ? im[L - p] : 0.0f;
+    }
+    mr = (valid && t == 0) ? re[L / 2] : 0.0f;
+    mi = (valid && t == 0) ? im[L / 2] : 0.0f;
+  };
+
+  const long long first = (MODE != kFrameMajor && f0 > 0) ? -(long long)a.warm : 0;
+  fetch(f0 + first * kFps + slot);
+  for (long long step = first; step < a.steps; ++step) {
+    const long long fbase = f0 + step * kFps;
+    const long long f = fbase + slot;
+    // conj(2Z) into buf; thread t owns the pairs (p, L-p), lane 0 also L/2
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      const float2 Xp = make_float2(xr[2 * j], xi[2 * j]);
+      const float2 Xq = make_float2(xr[2 * j + 1], xi[2 * j + 1]);
+      if (p == 0) {
+        buf[pidx(0)] = make_float2(Xp.x + Xq.x, Xq.x - Xp.x);
+      } else {
+        const float2 sv = make_float2(Xp.x + Xq.x, Xp.y - Xq.y);  // X_p + conj X_q
+        const float2 dv = make_float2(Xp.x - Xq.x, Xp.y + Xq.y);  // X_p - conj X_q
+        const float2 ed = c_mul(ep[j], dv);
+        buf[pidx(p)] = make_float2(sv.x - ed.y, -(sv.y + ed.x));
+        buf[pidx(L - p)] = make_float2(sv.x + ed.y, sv.y - ed.x);
+      }
+    }
+    if (t == 0) buf[pidx(L / 2)] = make_float2(2.0f * mr, 2.0f * mi);
+    wave_lds_fence();
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
+    wave_lds_fence();
+    if (step + 1 < a.steps) fetch(f + kFps);  // in flight during the transform
+    WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
+    // v[q*Rl + r] = DFT(conj 2Z)[m], m = t + q*T + r*L/Rl; y_m = conj of it
+
+    if constexpr (MODE == kFrameMajor) {
+      if (f < frames) {
+        float2* __restrict__ o = reinterpret_cast<float2*>(a.out + (size_t)f * N);
+        const float2* wl2 = reinterpret_cast<const float2*>(wl);
+#pragma unroll
+        for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+          for (int r = 0; r < Rl; ++r) {
+            const int m = t + q * T + r * (L / Rl);
+            const float2 w = wl2[m];
+            o[m] = make_float2(v[q * Rl + r].x * w.x, -v[q * Rl + r].y * w.y);
+          }
+      }
+      wave_lds_fence();  // buf is rewritten by the next step
+    } else {
+#pragma unroll
+      for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+        for (int r = 0; r < Rl; ++r) {
+          const float2 y = v[q * Rl + r];
+          buf[pidx(t + q * T + r * (L / Rl))] = make_float2(y.x, -y.y);
+        }
+      __syncthreads();
+
+      // Overlap-add of the step's frames over the samples they touch, ascending. The
+      // first FPS*H samples get no later frame: they are final (written unless this is
+      // a warm-up step) and their ring entries cleared; the rest stay in the ring.
+      const long long j0 = fbase * H;
+      const long long room = (long long)a.ola_len - j0;
+      const int span = (kFps - 1) * H + N;
+      const int uhi = room < span ? (int)room : span;
+      const int fin = kFps * H;
+      const int rb = (int)(j0 & (ACC - 1));
+      bool vg[kFps];
+#pragma unroll
+      for (int g = 0; g < kFps; ++g) vg[g] = fbase + g >= 0 && fbase + g < frames;
+      const bool write = step >= 0;
+      // every frame touching the step's final samples exists: the periodic norm applies
+      const bool inner = fbase >= (long long)a.kd && fbase + kFps <= frames;
+      float* __restrict__ out = a.out + j0;
+      auto norm_of = [&](int u) -> float {
+        if (inner) return nh[u % H];
+        float s = 0.0f;
+        for (int d = -a.kd; d < kFps; ++d) {  // frames ascending
+          const int n = u - d * H;
+          const long long fr = fbase + d;
+          if (n >= 0 && n < N && fr >= 0 && fr < frames) s += cola_weight(a, n);
+        }
+        return s;
+      };
+      auto finish = [&](float sum, int u) -> float {
+        if constexpr (MODE == kOverlapCola) {
+          const float nrm = norm_of(u);
+          return nrm > 1e-12f ? sum / nrm : sum;
+        } else {
+          return sum;
+        }
+      };
+      if ((H & 1) == 0) {
+        // even hop: sample pairs (u, u+1), u even, sit in one float2 of every frame
+        // buffer and of the ring (j0 and rb are even)
+        float2* acc2 = reinterpret_cast<float2*>(acc);
+        const float2* wl2 = reinterpret_cast<const float2*>(wl);
+        for (int u = 2 * tid; u < uhi; u += 512) {
+          const int ri = ((rb + u) & (ACC - 1)) >> 1;
+          float2 sum = acc2[ri];
+#pragma unroll
+          for (int g = 0; g < kFps; ++g) {
+            const int n = u - g * H;
+            if (vg[g] && n >= 0 && n < N) {
+              const float2 c = fbuf[g][pidx(n >> 1)];
+              const float2 w = wl2[n >> 1];
+              sum.x = __builtin_fmaf(c.x, w.x, sum.x);
+              sum.y = __builtin_fmaf(c.y, w.y, sum.y);
+            }
+          }
+          if (u < fin) {
+            if (write) {
+              if (u + 1 < uhi) {
+                const float2 o = make_float2(finish(sum.x, u), finish(sum.y, u + 1));
+                if ((((uintptr_t)(out + u)) & 7) == 0) {
+                  *reinterpret_cast<float2*>(out + u) = o;
+                } else {
+                  out[u] = o.x;
+                  out[u + 1] = o.y;
+                }
+              } else {
+                out[u] = finish(sum.x, u);
+              }
+            }
+            acc2[ri] = make_float2(0.0f, 0.0f);
+          } else {
+            acc2[ri] = sum;
+          }
+        }
+      } else {
+        for (int u = tid; u < uhi; u += 256) {
+          const int ri = (rb + u) & (ACC - 1);
+          float sum = acc[ri];
+#pragma unroll
+          for (int g = 0; g < kFps; ++g) {
+            const int n = u - g * H;
+            if (vg[g] && n >= 0 && n < N) {
+              const float2 c = fbuf[g][pidx(n >> 1)];
+              sum = __builtin_fmaf((n & 1) ? c.y : c.x, wl[n], sum);
+            }
+          }
+          if (u < fin) {
+            if (write) out[u] = finish(sum, u);
+            acc[ri] = 0.0f;
+          } else {
+            acc[ri] = sum;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if constexpr (MODE != kFrameMajor) zero_tail(a.out, a.ola_len, (size_t)a.frames * N, rank, gridDim.x);
+}
+
+// General path, step 1: the column rebuild of fft_cffi.cpp:1062-1098, one thread per
+// column entry (frames x N complex values).
+__global__ void __launch_bounds__(256) complex_columns_kernel(IstftArgs a, int hermitian, float2* __restrict__ cols) {
+  const size_t total = a.frames * (size_t)a.n;
+  const size_t n = (size_t)a.n;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const size_t f = idx / n;
+    const size_t i = idx - f * n;
+    const size_t row = f * (size_t)a.bins;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (hermitian) {
+      const size_t ny = n / 2;
+      if (i == 0 || i == ny) {
+        z = make_float2(a.in_re[row + i], 0.0f);
+      } else if (i < ny) {
+        z = make_float2(a.in_re[row + i], a.in_im[row + i]);
+      } else {
+        z = make_float2(a.in_re[row + n - i], -a.in_im[row + n - i]);
+      }
+    } else if (i < (size_t)a.bins) {
+      z = make_float2(a.in_re[row + i], a.in_im[row + i]);
+    }
+    cols[idx] = z;
+  }
+}
+
+// General path, step 3: real parts of the transformed columns to the output, one thread
+// per output sample (a gather, so the frame order of every sum is fixed).
+__global__ void __launch_bounds__(256) overlap_kernel(IstftArgs a, const float2* __restrict__ cols) {
+  const size_t n = (size_t)a.n;
+  const size_t total = a.frames * n;
+  const size_t H = (size_t)a.hop;
+  for (size_t s = (size_t)blockIdx.x * 256 + threadIdx.x; s < total; s += (size_t)gridDim.x * 256) {
+    if (!a.ola) {
+      const size_t i = s & (n - 1);
+      const float x = cols[s].x;
+      a.out[s] = a.swin ? x * a.swin[i] : x;
+      continue;
+    }
+    const size_t flo = s >= n ? (s - n) / H + 1 : 0;
+    size_t fhi = s / H;
+    if (fhi > a.frames - 1) fhi = a.frames - 1;
+    float sum = 0.0f, nrm = 0.0f;
+    for (size_t f = flo; f <= fhi; ++f) {
+      const size_t i = s - f * H;
+      const float x = cols[f * n + i].x;
+      sum += a.swin ? x * a.swin[i] : x;
+      if (a.cola) nrm += cola_weight(a, (int)i);
+    }
+    a.out[s] = (a.cola && nrm > 1e-12f) ? sum / nrm : sum;
+  }
+}
+
+template <int L, int E, int MODE, int ACCM, int... Rs>
+hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
+  const void* kern = reinterpret_cast<const void*>(&istft_kernel<L, E, MODE, ACCM, Rs...>);
+  int dev = 0, cus = 0, occ = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0);
+  if (e != hipSuccess) return e;
+  const size_t capacity = (size_t)(cus > 0 ? cus : 1) * (size_t)(occ > 0 ? occ : 1);
+  // frame positions the runs cover: every frame, or every hop of the overlap-added output
+  const size_t positions = MODE == kFrameMajor ? a.frames : (a.ola_len + a.hop - 1) / a.hop;
+  size_t steps = (positions + kFps * capacity - 1) / (kFps * capacity);
+  const size_t lo = MODE == kFrameMajor ? 1 : (size_t)(2 * a.warm > 4 ? 2 * a.warm : 4);
+  steps = steps < lo ? lo : (steps > 256 ? 256 : steps);
+  a.steps = (int)steps;
+  const size_t grid = (positions + kFps * steps - 1) / (kFps * steps);
+  if (grid == 0) return hipSuccess;
+  if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+  a.tiles = (int)grid;
+  void* args[] = {&a};
+  return hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
+}
+
+template <int L, int E, int... Rs>
+hipError_t istft_launch(const IstftArgs& a, hipStream_t s) {
+  if (!a.ola) return istft_launch_mode<L, E, kFrameMajor, 2, Rs...>(a, s);
+  const int span = (kFps - 1) * a.hop + 2 * L;
+  if (a.cola) {
+    if (span <= 4 * L) return istft_launch_mode<L, E, kOverlapCola, 2, Rs...>(a, s);
+    return istft_launch_mode<L, E, kOverlapCola, 4, Rs...>(a, s);
+  }
+  if (span <= 4 * L) return istft_launch_mode<L, E, kOverlap, 2, Rs...>(a, s);
+  return istft_launch_mode<L, E, kOverlap, 4, Rs...>(a, s);
+}
+
+}  // namespace
+
+bool istft_fused_supported(int n, int hop, int ola) {
+  if (n != 512 && n != 1024 && n != 2048) return false;
+  return !ola || (hop >= 1 && hop <= n);
+}
+
+size_t istft_ola_len(size_t frames, int n, int hop) {
+  if (frames == 0) return 0;
+  const size_t total = frames * (size_t)n;
+  const size_t span = (frames - 1) * (size_t)hop + (size_t)n;
+  return span < total ? span : total;
+}
+
+hipError_t launch_istft(IstftArgs a, hipStream_t s) {
+  if (!istft_fused_supported(a.n, a.hop, a.ola) || a.frames == 0 || !a.out) return hipErrorInvalidValue;
+  if (a.ola) {
+    a.ola_len = istft_ola_len(a.frames, a.n, a.hop);
+    a.kd = (a.n + a.hop - 1) / a.hop;              // frames that touch one sample, at most
+    a.warm = (a.kd - 1 + kFps - 1) / kFps;          // warm-up steps before a run
+  }
+  switch (a.n) {
+    case 512: return istft_launch<256, 4, 4, 4, 4, 4>(a, s);
+    case 1024: return istft_launch<512, 8, 8, 8, 8>(a, s);
+    default: return istft_launch<1024, 16, 16, 16, 4>(a, s);
+  }
+}
+
+hipError_t launch_complex_columns(const IstftArgs& a, int hermitian, float2* cols, hipStream_t s) {
+  const size_t total = a.frames * (size_t)a.n;
+  if (total == 0) return hipSuccess;
+  size_t grid = (total + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(complex_columns_kernel, dim3((unsigned)grid), dim3(256), 0, s, a, hermitian, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlap(IstftArgs a, const float2* cols, hipStream_t s) {
+  const size_t total = a.frames * (size_t)a.n;
+  if (total == 0) return hipSuccess;
+  if (a.ola && a.hop <= 0) return hipErrorInvalidValue;
+  size_t grid = (total + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(overlap_kernel, dim3((unsigned)grid), dim3(256), 0, s, a, cols);
+  return hipGetLastError();
+}
+
+}  // namespace fftfree

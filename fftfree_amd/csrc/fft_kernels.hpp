@@ -75,6 +75,37 @@ struct SignalArgs {
 // Device scratch (floats) the adjoint needs for `signals` signals of pcm_len samples.
 size_t adjoint_scratch(int n, int window, int hop, size_t signals, size_t pcm_len);
 
+// Transform of frame-major complex bins with synthesis window and overlap-add
+// (fft_execute_complex_batched semantics, fft_cffi.cpp:1042-1157).
+struct IstftArgs {
+  const float* in_re = nullptr;  // frames x bins
+  const float* in_im = nullptr;
+  size_t frames = 0;
+  int n = 0;
+  int bins = 0;
+  int hop = 0;                   // overlap-add hop
+  const float2* tw = nullptr;    // forward table exp(-2 pi i m / N), m < N
+  const float* swin = nullptr;   // N synthesis taps, or nullptr when windows are off
+  const float* awin = nullptr;   // N analysis taps, or nullptr; COLA weight = awin*swin
+  int ola = 0;                   // 0: N samples per frame, frame-major; 1: overlap-add
+  int cola = 0;                  // with ola: divide by the summed COLA weights
+  float* out = nullptr;          // frames*N floats
+  size_t ola_len = 0;            // set by the launcher: samples the overlap-add reaches
+  int kd = 0;                    // set by the launcher: ceil(N / hop)
+  int warm = 0;                  // set by the launcher: warm-up steps per run
+  int tiles = 0;                 // set by the launcher: workgroups
+  int steps = 0;                 // set by the launcher: steps per workgroup
+};
+// Fused kernel: an inverse C2R half-spectrum plan, N in {512, 1024, 2048}, and (with
+// overlap-add) 1 <= hop <= N.
+bool istft_fused_supported(int n, int hop, int ola);
+size_t istft_ola_len(size_t frames, int n, int hop);
+hipError_t launch_istft(IstftArgs a, hipStream_t s);
+// General path: column rebuild into `cols` (frames x N complex), the plan's transform,
+// then launch_overlap for the output.
+hipError_t launch_complex_columns(const IstftArgs& a, int hermitian, float2* cols, hipStream_t s);
+hipError_t launch_overlap(IstftArgs a, const float2* cols, hipStream_t s);
+
 // Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
 enum MetaKind : int {
   kMetaTwiddle = 0,

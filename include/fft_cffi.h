@@ -140,6 +140,16 @@ size_t fft_execute_batched_device(void* handle, const float* d_pcm, size_t pcm_l
 size_t fft_execute_c2c_batched(void* handle, const float* in, float* out, size_t batch);
 size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream);
 
+/* additive. fft_execute_complex_batched (fft_cffi.cpp:1042-1157) on device-resident
+ * buffers: d_in_real / d_in_imag hold frames*bins floats (frame-major), d_out_pcm
+ * frames*N floats, written in full as the reference writes them (overlap-added output
+ * followed by zeros when the context has apply_ola). pad_mode and enable_backup of the
+ * host entry have no effect there either and are not taken. Returns frames, 0 on error.
+ * Enqueued on `stream`; a context whose flags the fused kernel does not cover uses a
+ * device scratch of frames*N complex values owned by the context. */
+size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
+                                          size_t frames, float* d_out_pcm, size_t max_frames, void* stream);
+
 /* additive. Batched STFT of `signals` signals of pcm_len samples each (signal s at
  * d_pcm + s*pcm_len), for a forward R2C half-spectrum context with N in {512, 1024,
  * 2048}: every signal is framed with the context's window/hop/pad rule (frames per

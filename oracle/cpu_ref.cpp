@@ -16,7 +16,7 @@
 //   radix-2 DIT scatter/in-place      butterfly_kernel.hpp:77-98, 128-233
 //   window builder + normalisation    fft_cffi.cpp:163-208
 //   frame count + framing + output    fft_cffi.cpp:701-825
-//   complex-input batched inverse     fft_cffi.cpp:1042-1157
+//   complex-input batched inverse     fft_cffi.cpp:1042-1157 (output stage 1119-1157, 210-226)
 //   metadata writers (index maps)     eigen_fft.hpp:1529-1888 as called at 2672-2682,
 //                                     2981-3011, 3239-3244, 3478-3505
 // The reference's WorkerPool (eigen_fft.hpp:818-1419) is restated as a plain
@@ -443,6 +443,53 @@ size_t ref_execute_complex_batched(int N, int inverse, int kernel, int transform
     }
   });
   return frames;
+}
+
+// Output stage of fft_execute_complex_batched (fft_cffi.cpp:1119-1157) from the real
+// parts of the transformed columns (frames x N, frame-major). With windows enabled, s =
+// re * synthesis[i]; apply_ola = 0: out[f*N + i] = s. apply_ola = 1: out is zeroed over
+// frames*N samples and out[f*H + i] += s for f*H + i < frames*N, frames ascending; with
+// cola_mode 1 (FFT_COLA_NORMALIZE) norm[t] accumulates cola_window_weight (analysis[i] *
+// synthesis[i] with windows enabled, else 1; fft_cffi.cpp:210-226) and out[t] /= norm[t]
+// where norm[t] > 1e-12. NULL taps stand for the reference's empty window vectors.
+void ref_complex_output(int N, size_t frames, const float* cols_real, int windows_enabled, const float* analysis,
+                        const float* synthesis, int apply_ola, int hop, int cola_mode, float* out_pcm) {
+  const size_t n = size_t(N);
+  const bool win = windows_enabled && synthesis;
+  auto weight = [&](size_t i) -> float {
+    if (!windows_enabled) return 1.0f;
+    const float a = analysis ? analysis[i] : 1.0f;
+    const float b = synthesis ? synthesis[i] : 1.0f;
+    return a * b;
+  };
+  if (!apply_ola) {
+    for (size_t f = 0; f < frames; ++f)
+      for (size_t i = 0; i < n; ++i) {
+        float v = cols_real[f * n + i];
+        if (win) v *= synthesis[i];
+        out_pcm[f * n + i] = v;
+      }
+    return;
+  }
+  const size_t total = frames * n;
+  for (size_t t = 0; t < total; ++t) out_pcm[t] = 0.0f;
+  std::vector<float> norm;
+  if (cola_mode == 1) norm.assign(total, 0.0f);
+  for (size_t f = 0; f < frames; ++f) {
+    const size_t start = f * size_t(hop);
+    for (size_t i = 0; i < n; ++i) {
+      float v = cols_real[f * n + i];
+      if (win) v *= synthesis[i];
+      const size_t t = start + i;
+      if (t < total) {
+        out_pcm[t] += v;
+        if (!norm.empty()) norm[t] += weight(i);
+      }
+    }
+  }
+  if (!norm.empty())
+    for (size_t t = 0; t < total; ++t)
+      if (norm[t] > 1e-12f) out_pcm[t] /= norm[t];
 }
 
 // Metadata of one column as the reference writers record it. kind follows

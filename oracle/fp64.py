@@ -274,3 +274,100 @@ def stft_adjoint64(pcm, N, W, H, frames, win, mode, ga, gb=None, ola_window=True
         if c > 0:
             g[f * H:f * H + c] += r[f, :c] * (w[:c] if ola_window else 1.0)
     return g
+
+
+def complex_columns64(in_real, in_imag, N, transform, half_spectrum):
+    """Column rebuild of fft_execute_complex_batched (fft_cffi.cpp:1062-1098) in float64
+    from the float32 planes (frames x bins): for a C2R half spectrum imag(X_0) =
+    imag(X_{N/2}) = 0 and X_{N-k} = conj X_k (0 < k < N/2); otherwise zeros above the
+    given bins."""
+    re = np.asarray(in_real, dtype=np.float64)
+    im = np.asarray(in_imag, dtype=np.float64)
+    frames, bins = re.shape
+    cols = np.zeros((frames, N), dtype=np.complex128)
+    cols[:, :bins] = re + 1j * im
+    if half_spectrum and transform == C2R:
+        ny = N // 2
+        cols[:, 0] = cols[:, 0].real
+        if ny < bins:
+            cols[:, ny] = cols[:, ny].real
+        maxk = ny if bins > ny + 1 else bins - 1
+        if maxk > 1:
+            cols[:, N - maxk + 1:] = np.conj(cols[:, 1:maxk][:, ::-1])
+    return cols
+
+
+def complex_batched64(in_real, in_imag, N, inverse=True, kernel=0, transform=C2R, reduce_magnitude=False,
+                      store_polar=False, half_spectrum=True, windows_enabled=False, analysis=None, synthesis=None,
+                      apply_ola=False, hop=0, cola_mode=0):
+    """FP64 oracle of fft_execute_complex_batched (fft_cffi.cpp:1042-1157): column rebuild,
+    the plan's pre hook (Cooley-Tukey only, kernel != 2; eigen_fft.hpp:2484-2533), the
+    transform (inverse 1/N), the post hook's effect on the real parts
+    (eigen_fft.hpp:2535-2596), then the output stage (1119-1157): synthesis window,
+    overlap-add over frames*N samples, COLA division where the weight sum > 1e-12
+    (weights fft_cffi.cpp:210-226). Returns frames*N floats (float64), shaped (frames, N)
+    without overlap-add."""
+    cols = complex_columns64(in_real, in_imag, N, transform, half_spectrum)
+    frames = cols.shape[0]
+    if kernel != 2:
+        if transform == R2C:
+            cols = cols.real.astype(np.complex128)
+        elif transform == C2R and half_spectrum:
+            ny = N // 2
+            if store_polar:
+                mag = cols[:, :ny + 1].real.copy()
+                ph = cols[:, :ny + 1].imag.copy()
+                cols[:, :ny + 1] = mag * np.cos(ph) + 1j * mag * np.sin(ph)
+            cols[:, 0] = cols[:, 0].real
+            cols[:, ny] = cols[:, ny].real
+            if ny > 1:
+                cols[:, N - ny + 1:] = np.conj(cols[:, 1:ny][:, ::-1])
+    X = fft64(cols, inverse)
+    if transform == R2C and half_spectrum:
+        X[:, N // 2 + 1:] = 0
+    upper = N // 2 + 1 if (transform == R2C and half_spectrum) else N
+    re = X.real.copy()
+    if store_polar or reduce_magnitude:
+        re[:, :upper] = np.abs(X[:, :upper])
+    win = windows_enabled and synthesis is not None
+    syn = np.asarray(synthesis, dtype=np.float64) if synthesis is not None else None
+    if win:
+        re = re * syn[None, :]
+    if not apply_ola:
+        return re
+    total = frames * N
+    out = np.zeros(total)
+    norm = np.zeros(total) if cola_mode == 1 else None
+    if norm is not None:
+        if windows_enabled:
+            a = np.asarray(analysis, dtype=np.float64) if analysis is not None else np.ones(N)
+            b = syn if syn is not None else np.ones(N)
+            wgt = a * b
+        else:
+            wgt = np.ones(N)
+    for f in range(frames):
+        s = f * hop
+        if s >= total:
+            break
+        c = min(N, total - s)
+        out[s:s + c] += re[f, :c]
+        if norm is not None:
+            norm[s:s + c] += wgt[:c]
+    if norm is not None:
+        m = norm > 1e-12
+        out[m] /= norm[m]
+    return out
+
+
+def cola_norm64(frames, N, hop, weights):
+    """Summed COLA weights per output sample (frames*N samples) in float64."""
+    total = frames * N
+    norm = np.zeros(total)
+    w = np.asarray(weights, dtype=np.float64)
+    for f in range(frames):
+        s = f * hop
+        if s >= total:
+            break
+        c = min(N, total - s)
+        norm[s:s + c] += w[:c]
+    return norm

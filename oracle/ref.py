@@ -45,6 +45,9 @@ def lib():
         L.ref_execute_complex_batched.argtypes = ([ctypes.c_int] * 7 + [_f32p, _f32p, _f32p, _sz, _f32p, _sz,
                                                                         ctypes.c_int])
         L.ref_execute_complex_batched.restype = _sz
+        L.ref_complex_output.argtypes = [ctypes.c_int, _sz, _f32p, ctypes.c_int, _f32p, _f32p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, _f32p]
+        L.ref_complex_output.restype = None
         L.ref_metadata.argtypes = [ctypes.c_int] * 6 + [_f32p, _sz]
         L.ref_metadata.restype = _sz
         _lib = L
@@ -111,16 +114,28 @@ def execute_batched(pcm: np.ndarray, N, window=0, hop=0, ctx_pad=0, pad_mode=0, 
 
 
 def execute_complex_batched(in_real, in_imag, N, inverse=True, kernel=0, transform=2, reduce_magnitude=False,
-                            store_polar=False, half_spectrum=True, synth_taps=None, max_frames=0, threads=None):
+                            store_polar=False, half_spectrum=True, synth_taps=None, max_frames=0, threads=None,
+                            windows_enabled=None, analysis_taps=None, apply_ola=False, hop=0, cola_mode=0):
+    """Reference fft_execute_complex_batched (fft_cffi.cpp:1042-1157): returns (frames,
+    out) with out of frames*N floats, shaped (frames, N) without overlap-add.
+    `windows_enabled` defaults to "synth_taps given"; taps are N floats."""
     in_real = np.ascontiguousarray(in_real, dtype=np.float32)
     in_imag = np.ascontiguousarray(in_imag, dtype=np.float32)
     frames = in_real.shape[0]
-    out = np.zeros((frames, N), np.float32)
-    taps = None if synth_taps is None else np.ascontiguousarray(synth_taps, dtype=np.float32)
+    if max_frames:
+        frames = min(frames, max_frames)
+    cols = np.zeros((frames, N), np.float32)
     got = lib().ref_execute_complex_batched(N, int(inverse), kernel, transform, int(reduce_magnitude),
-                                            int(store_polar), int(half_spectrum), _p(taps), _p(in_real),
-                                            _p(in_imag), frames, _p(out), max_frames, threads or default_threads())
-    return int(got), out[:got]
+                                            int(store_polar), int(half_spectrum), None, _p(in_real),
+                                            _p(in_imag), frames, _p(cols), max_frames, threads or default_threads())
+    if windows_enabled is None:
+        windows_enabled = synth_taps is not None
+    syn = None if synth_taps is None else np.ascontiguousarray(synth_taps, dtype=np.float32)
+    ana = None if analysis_taps is None else np.ascontiguousarray(analysis_taps, dtype=np.float32)
+    out = np.zeros((got, N), np.float32)
+    lib().ref_complex_output(N, got, _p(cols[:got]), int(bool(windows_enabled)), _p(ana), _p(syn), int(bool(apply_ola)),
+                             int(hop), int(cola_mode), _p(out))
+    return int(got), (out.reshape(-1) if apply_ola else out)
 
 
 def metadata(N: int, kind: int, kernel: int = 0, inverse=False, transform=0, half_spectrum=False) -> np.ndarray:

@@ -299,3 +299,118 @@ def test_adjoint_oracle_agrees_with_direct_magnitude_gradient():
     direct = fp64.stft_magnitude_grad(pcm, y, N, W, H, frames, win)
     fast = fp64.stft_adjoint64(pcm, N, W, H, frames, win, "magnitude", 2 * (A - y))
     assert np.max(np.abs(direct - fast)) <= 1e-9 * np.max(np.abs(direct))
+
+
+# ---- fft_execute_complex_batched: rebuild, hooks, overlap-add, COLA -------------------
+
+def _planes64(x, N, H, win):
+    """R2C half-spectrum planes of W = N frames, all in float64 (no float32 rounding)."""
+    frames = fp64.frame_count(x.size, N, N, H, 1, 1)
+    fr = np.stack([np.asarray(x[f * H:f * H + N], np.float64) for f in range(frames)])
+    X = np.fft.rfft(fr * np.asarray(win, np.float64), axis=-1)
+    return frames, X.real.copy(), X.imag.copy()
+
+
+@pytest.mark.parametrize("N", [16, 64, 256])
+@pytest.mark.parametrize("hop_div", [1, 2, 4])
+@pytest.mark.parametrize("kinds", [(fp64.WINDOW_HANN, fp64.WINDOW_HANN), (fp64.WINDOW_HAMMING, fp64.WINDOW_RECT),
+                                   (fp64.WINDOW_RECT, fp64.WINDOW_BLACKMAN)])
+def test_complex_oracle_reconstructs_the_signal(N, hop_div, kinds):
+    """Analysis STFT then the inverse with overlap-add: each frame returns wa*x, so the
+    output is x * sum(wa*ws) without COLA and x itself with it (where the sum is not
+    tiny), and frame-major output is wa*ws*frame. Pins the Hermitian rebuild, the 1/N
+    scale, the hop and the frame order of the sums."""
+    H = N // hop_div
+    wa = fp64.build_window(kinds[0], N)
+    ws = fp64.build_window(kinds[1], N)
+    x = np.random.default_rng(N + H).standard_normal(N * 6).astype(np.float64)
+    frames, re, im = _planes64(x, N, H, wa)
+    L = (frames - 1) * H + N
+    norm = fp64.cola_norm64(frames, N, H, wa.astype(np.float64) * ws)
+    kw = dict(windows_enabled=True, analysis=wa, synthesis=ws, hop=H)
+    raw = fp64.complex_batched64(re, im, N, apply_ola=True, cola_mode=0, **kw)
+    assert np.max(np.abs(raw[:L] - x[:L] * norm[:L])) <= 1e-10
+    assert not raw[L:].any()
+    cola = fp64.complex_batched64(re, im, N, apply_ola=True, cola_mode=1, **kw)
+    m = norm[:L] > 1e-3 * norm.max()
+    assert m.sum() > L // 2
+    assert np.max(np.abs(cola[:L][m] - x[:L][m])) <= 1e-9
+    fm = fp64.complex_batched64(re, im, N, apply_ola=False, **kw)
+    framed = np.stack([x[f * H:f * H + N] for f in range(frames)])
+    assert np.max(np.abs(fm - framed * wa * ws)) <= 1e-10
+
+
+def test_complex_oracle_reconstruction_check_has_teeth():
+    """Planted mistakes (a hop off by one, a mirror without the conjugate) must fail the
+    reconstruction check above."""
+    N, H = 64, 16
+    w = fp64.build_window(fp64.WINDOW_HANN, N)
+    x = np.random.default_rng(5).standard_normal(N * 6)
+    frames, re, im = _planes64(x, N, H, w)
+    norm = fp64.cola_norm64(frames, N, H, w.astype(np.float64) ** 2)
+    L = (frames - 1) * H + N
+    m = norm[:L] > 1e-3 * norm.max()
+    bad_hop = fp64.complex_batched64(re, im, N, windows_enabled=True, analysis=w, synthesis=w, hop=H + 1,
+                                     apply_ola=True, cola_mode=1)
+    assert np.max(np.abs(bad_hop[:L][m] - x[:L][m])) > 1e-2
+    cols = fp64.complex_columns64(re, im, N, fp64.C2R, True)
+    cols[:, N // 2 + 1:] = cols[:, 1:N // 2][:, ::-1]  # mirror without conj
+    y = fp64.fft64(cols, True).real
+    assert np.max(np.abs(y[0, :N] - x[:N] * w)) > 1e-2
+
+
+@pytest.mark.parametrize("N", [8, 32])
+def test_complex_oracle_closed_forms(N):
+    """One bin k0 set: a C2R half-spectrum plan gives (2/N) cos(2 pi k0 n/N + phi), a
+    complex plan on the zero-filled half spectrum (1/N) e^{i...}; with store_polar the
+    Cooley-Tukey pre hook reads (mag, phase) (eigen_fft.hpp:2511-2519), Stockham does not,
+    and the polar post hook leaves |x| in the real part; DC and Nyquist imaginary parts
+    are dropped by the rebuild."""
+    k0, phi = 3, 0.7
+    n = np.arange(N)
+    th = 2 * np.pi * k0 * n / N
+    re = np.zeros((1, N // 2 + 1))
+    im = np.zeros((1, N // 2 + 1))
+    re[0, k0], im[0, k0] = np.cos(phi), np.sin(phi)
+    c2r = fp64.complex_batched64(re, im, N, transform=fp64.C2R)
+    assert np.allclose(c2r[0], (2 / N) * np.cos(th + phi), atol=1e-14)
+    c2c = fp64.complex_batched64(re, im, N, transform=fp64.C2C)
+    assert np.allclose(c2c[0], (1 / N) * np.cos(th + phi), atol=1e-14)
+    pre, pim = np.zeros_like(re), np.zeros_like(im)
+    pre[0, k0], pim[0, k0] = 1.0, phi
+    ct = fp64.complex_batched64(pre, pim, N, kernel=0, transform=fp64.C2R, store_polar=True)
+    assert np.allclose(ct[0], np.abs((2 / N) * np.cos(th + phi)), atol=1e-14)
+    st = fp64.complex_batched64(pre, pim, N, kernel=2, transform=fp64.C2R, store_polar=True)
+    assert np.allclose(st[0], np.abs((2 / N) * (np.cos(th) - phi * np.sin(th))), atol=1e-14)
+    dc = np.zeros_like(re)
+    dci = np.zeros_like(im)
+    dc[0, 0], dci[0, 0], dc[0, -1], dci[0, -1] = 1.0, 5.0, 1.0, -3.0
+    out = fp64.complex_batched64(dc, dci, N, transform=fp64.C2R)
+    assert np.allclose(out[0], (1 + (-1.0) ** n) / N, atol=1e-14)
+
+
+@pytest.mark.parametrize("kernel", [0, 2])
+@pytest.mark.parametrize("mode", [(0, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0)])
+def test_cpu_ref_complex_batched_vs_fp64(kernel, mode):
+    """The float restatement (rebuild, plan, output stage) against the FP64 oracle:
+    (apply_ola, cola, windows) combinations. Samples whose COLA weight sum is tiny are
+    ill-conditioned (division by ~1e-10 at a Hann edge) whatever the precision, so the
+    normalised comparison covers samples with a sum above 1e-3 of its maximum."""
+    ola, cola, windows = mode
+    N, H = 256, 64
+    w = fp64.build_window(fp64.WINDOW_HANN, N)
+    x = np.random.default_rng(9).standard_normal(N * 12).astype(np.float32)
+    frames, re, im = _planes64(x, N, H, w)
+    re, im = re.astype(np.float32), im.astype(np.float32)
+    taps = w if windows else None
+    got, out = ref.execute_complex_batched(re, im, N, kernel=kernel, synth_taps=taps, analysis_taps=taps,
+                                           apply_ola=bool(ola), hop=H, cola_mode=cola)
+    exp = fp64.complex_batched64(re, im, N, kernel=kernel, windows_enabled=bool(windows), analysis=taps,
+                                 synthesis=taps, apply_ola=bool(ola), hop=H, cola_mode=cola)
+    assert got == frames
+    out, exp = np.asarray(out).ravel(), np.asarray(exp).ravel()
+    m = np.ones(out.size, bool)
+    if ola and cola:
+        norm = fp64.cola_norm64(frames, N, H, (w.astype(np.float64) ** 2) if windows else np.ones(N))
+        m = norm > 1e-3 * norm.max()
+    assert fp64.rel_l2(out[m], exp[m]) <= fp64.tolerance(N)
