@@ -36,6 +36,14 @@ using namespace dev;
 namespace {
 
 constexpr int kFps = 4;  // frames per step: one 64-lane wave per frame
+#ifndef FFTFREE_ISTFT_FILL
+#define FFTFREE_ISTFT_FILL 100
+#endif
+// Workgroups per 100 resident slots of the chip: more means shorter runs (more warm-up
+// frames transformed twice), fewer means less latency hiding.
+constexpr int kIstftFill = FFTFREE_ISTFT_FILL;
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 enum IstftMode : int { kFrameMajor = 0, kOverlap = 1, kOverlapCola = 2 };
 
@@ -46,22 +54,37 @@ __device__ __forceinline__ float cola_weight(const IstftArgs& a, int n) {
   return wa * ws;
 }
 
-// Stores zeros over out[from, to): float4 where aligned, split over the grid.
-__device__ __forceinline__ void zero_tail(float* __restrict__ out, size_t from, size_t to, int rank, int ranks) {
+// The float4-aligned part of out[from, to) that rank `rank` of `ranks` zero-fills, in
+// float4 units from `base4`; the unaligned head and tail floats go to the first and last
+// rank (zero_edges).
+struct ZeroSpan {
+  f32x4* base4 = nullptr;
+  size_t b = 0, e = 0;
+};
+__device__ __forceinline__ ZeroSpan zero_span(float* __restrict__ out, size_t from, size_t to, int rank, int ranks) {
+  ZeroSpan z;
+  if (from >= to) return z;
+  float* base = out + from;
+  const size_t len = to - from;
+  size_t head = ((16 - ((uintptr_t)base & 15)) & 15) >> 2;
+  if (head > len) head = len;
+  const size_t n4 = (len - head) >> 2;
+  const size_t per = (n4 + ranks - 1) / ranks;
+  z.base4 = reinterpret_cast<f32x4*>(base + head);
+  z.b = (size_t)rank * per;
+  z.e = z.b + per < n4 ? z.b + per : n4;
+  if (z.b > z.e) z.b = z.e;
+  return z;
+}
+__device__ __forceinline__ void zero_edges(float* __restrict__ out, size_t from, size_t to, int rank, int ranks) {
   if (from >= to) return;
   float* base = out + from;
   const size_t len = to - from;
   size_t head = ((16 - ((uintptr_t)base & 15)) & 15) >> 2;
   if (head > len) head = len;
-  if (rank == 0 && threadIdx.x < head) base[threadIdx.x] = 0.0f;
   const size_t n4 = (len - head) >> 2;
-  const size_t per = (n4 + ranks - 1) / ranks;
-  const size_t b = (size_t)rank * per;
-  const size_t e = b + per < n4 ? b + per : n4;
-  f32x4* z4 = reinterpret_cast<f32x4*>(base + head);
-  const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (size_t i = b + threadIdx.x; i < e; i += blockDim.x) __builtin_nontemporal_store(zero, &z4[i]);
   const size_t rest = (len - head) & 3;
+  if (rank == 0 && threadIdx.x < head) base[threadIdx.x] = 0.0f;
   if (rank == ranks - 1 && threadIdx.x < rest) base[head + 4 * n4 + threadIdx.x] = 0.0f;
 }
 
@@ -75,13 +98,16 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
   constexpr int LP = padded_len<L>();
   constexpr int JF = (L / 2) / T;
   constexpr int BINS = L + 1;
-  constexpr int ACC = MODE == kFrameMajor ? 2 : ACCM * N;  // ring; the host checks (FPS-1)*H + N <= ACC
+  constexpr bool OLA = MODE != kFrameMajor;
+  constexpr bool COLA = MODE == kOverlapCola;
+  constexpr int ACC = OLA ? ACCM * N : 4;  // ring; the host checks (FPS-1)*H + N <= ACC
   static_assert(T == 64 && JF * T == L / 2, "one frame per wave");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
   __shared__ float2 fbuf[kFps][LP];
-  __shared__ float acc[ACC];
-  __shared__ float wl[N];                                  // synthesis taps / N
-  __shared__ float nh[MODE == kOverlapCola ? N : 1];      // COLA sums per sample residue mod H
+  __shared__ __attribute__((aligned(16))) float acc[ACC];
+  __shared__ float wl[N];                // synthesis taps / N
+  __shared__ float nhr[COLA ? N : 1];    // 1 / COLA sum per sample residue mod H (1 where <= 1e-12)
+  __shared__ float cw[COLA ? N : 1];     // COLA weights
 
   const int tid = threadIdx.x;
   const int slot = tid >> 6;
@@ -91,6 +117,7 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
   const long long frames = (long long)a.frames;
   const int H = a.hop;
   const long long f0 = (long long)rank * kFps * a.steps;
+  const bool vec4 = (((uintptr_t)a.out) & 15) == 0 && (H & 3) == 0;
 
   float2 twr[E * NP];
   StockhamPlan<L, E, -1, Rs...>::load_twiddles(twr, t, a.tw, 2);
@@ -98,48 +125,101 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
 #pragma unroll
   for (int j = 0; j < JF; ++j) ep[j] = load_tw<1>(a.tw, t + j * T);
   for (int i = tid; i < N; i += 256) wl[i] = (a.swin ? a.swin[i] : 1.0f) * (1.0f / (float)N);
-  if constexpr (MODE == kOverlapCola) {
-    // norm of a sample whose frames are all present: taps r + kH, frames ascending
-    // (taps descending), as the reference accumulates
-    for (int r = tid; r < H; r += 256) {
-      float s = 0.0f;
-      for (int n = r + ((N - 1 - r) / H) * H; n >= 0; n -= H) s += cola_weight(a, n);
-      nh[r] = s;
-    }
-  }
-  if constexpr (MODE != kFrameMajor)
+  if constexpr (COLA)
+    for (int i = tid; i < N; i += 256) cw[i] = cola_weight(a, i);
+  if constexpr (OLA)
     for (int i = tid; i < ACC; i += 256) acc[i] = 0.0f;
   __syncthreads();
+  if constexpr (COLA) {
+    // sum of a sample whose frames are all present: taps r + kH, frames ascending (taps
+    // descending), as the reference accumulates
+    for (int r = tid; r < H; r += 256) {
+      float sum = 0.0f;
+      for (int n = r + ((N - 1 - r) / H) * H; n >= 0; n -= H) sum += cw[n];
+      nhr[r] = sum > 1e-12f ? 1.0f / sum : 1.0f;
+    }
+    __syncthreads();
+  }
 
-  // bins of the frame a step ahead, in registers
-  float xr[2 * JF], xi[2 * JF], mr = 0.0f, mi = 0.0f;
+  // bins of the frame a step ahead, in registers (the index is clamped so the loads are
+  // unconditional; frames outside [0, frames) count as zero where the values are used)
+  float xr[2 * JF], xi[2 * JF], mr, mi;
+  bool xvalid = false;
   auto fetch = [&](long long f) {
-    const bool valid = f >= 0 && f < frames;
-    const float* __restrict__ re = a.in_re + (valid ? (size_t)f * BINS : 0);
-    const float* __restrict__ im = a.in_im + (valid ? (size_t)f * BINS : 0);
+    xvalid = f >= 0 && f < frames;
+    const long long fc = f < 0 ? 0 : (f >= frames ? frames - 1 : f);
+    const float* __restrict__ re = a.in_re + (size_t)fc * BINS;
+    const float* __restrict__ im = a.in_im + (size_t)fc * BINS;
 #pragma unroll
     for (int j = 0; j < JF; ++j) {
       const int p = t + j * T;
-      xr[2 * j] = valid ? re[p] : 0.0f;
-      xi[2 * j] = valid ? im[p] : 0.0f;
-      xr[2 * j + 1] = valid ? re[L - p] : 0.0f;
-      xi[2 * j + 1] = valid ? im[L - p] : 0.0f;
+      xr[2 * j] = re[p];
+      xi[2 * j] = im[p];
+      xr[2 * j + 1] = re[L - p];
+      xi[2 * j + 1] = im[L - p];
     }
-    mr = (valid && t == 0) ? re[L / 2] : 0.0f;
-    mi = (valid && t == 0) ? im[L / 2] : 0.0f;
+    mr = re[L / 2];
+    mi = im[L / 2];
   };
 
-  const long long first = (MODE != kFrameMajor && f0 > 0) ? -(long long)a.warm : 0;
+  const long long first = (OLA && f0 > 0) ? -(long long)a.warm : 0;
+  // the zero tail past the overlap-add: this rank's share, spread over its steps
+  ZeroSpan zs;
+  size_t zper = 0;
+  if constexpr (OLA) {
+    zs = zero_span(a.out, a.ola_len, a.frames * (size_t)N, rank, gridDim.x);
+    const size_t nsteps = (size_t)(a.steps - first);
+    zper = (zs.e - zs.b + nsteps - 1) / nsteps;
+    zero_edges(a.out, a.ola_len, a.frames * (size_t)N, rank, gridDim.x);
+  }
+  const f32x4 zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
+
+  // Final samples of the previous step wait in the ring, normalised; they are stored at
+  // the start of the next step, ahead of its loads, so that no wait for those loads
+  // also waits for fresh stores.
+  long long pj0 = 0;
+  int pfin = 0;
+  bool pwrite = false;
+  auto flush = [&]() {
+    if (pfin <= 0) return;
+    const int prb = (int)(pj0 & (ACC - 1));
+    float* __restrict__ o = a.out + pj0;
+    if (vec4) {
+      const int n4 = pfin & ~3;
+      for (int u = 4 * tid; u < n4; u += 1024) {
+        f32x4* r = reinterpret_cast<f32x4*>(&acc[(prb + u) & (ACC - 1)]);
+        const f32x4 v = *r;
+        *r = zero4;
+        if (pwrite) __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(o + u));
+      }
+      if (tid < pfin - n4) {
+        const int u = n4 + tid;
+        float& r = acc[(prb + u) & (ACC - 1)];
+        if (pwrite) o[u] = r;
+        r = 0.0f;
+      }
+    } else {
+      for (int u = tid; u < pfin; u += 256) {
+        float& r = acc[(prb + u) & (ACC - 1)];
+        if (pwrite) o[u] = r;
+        r = 0.0f;
+      }
+    }
+  };
+  float2 pv[OLA ? 1 : E];  // frame-major: the previous step's frame, stored a step late
   fetch(f0 + first * kFps + slot);
   for (long long step = first; step < a.steps; ++step) {
     const long long fbase = f0 + step * kFps;
     const long long f = fbase + slot;
-    // conj(2Z) into buf; thread t owns the pairs (p, L-p), lane 0 also L/2
+    // conj(2Z) into buf; thread t owns the pairs (p, L-p), lane 0 also L/2. Frame-major
+    // output keeps a clamped frame's values: the wave then rewrites the last frame's
+    // samples with the same values, so its stores need no condition.
+    const float keep = (xvalid || !OLA) ? 1.0f : 0.0f;
 #pragma unroll
     for (int j = 0; j < JF; ++j) {
       const int p = t + j * T;
-      const float2 Xp = make_float2(xr[2 * j], xi[2 * j]);
-      const float2 Xq = make_float2(xr[2 * j + 1], xi[2 * j + 1]);
+      const float2 Xp = make_float2(xr[2 * j] * keep, xi[2 * j] * keep);
+      const float2 Xq = make_float2(xr[2 * j + 1] * keep, xi[2 * j + 1] * keep);
       if (p == 0) {
         buf[pidx(0)] = make_float2(Xp.x + Xq.x, Xq.x - Xp.x);
       } else {
@@ -150,7 +230,36 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
         buf[pidx(L - p)] = make_float2(sv.x + ed.y, sv.y - ed.x);
       }
     }
-    if (t == 0) buf[pidx(L / 2)] = make_float2(2.0f * mr, 2.0f * mi);
+    if (t == 0) buf[pidx(L / 2)] = make_float2(2.0f * keep * mr, 2.0f * keep * mi);
+    // The next step's loads, then this step's stores: the wait for those loads (at the
+    // next step's conj(2Z)) also covers every store pending at that point, so the stores
+    // are issued a whole step ahead of it.
+    fetch(f + kFps);  // unconditional, so the waits stay counted
+    if constexpr (OLA) {
+      // the previous step's final samples and a share of the zero tail
+      flush();
+      const size_t k = (size_t)(step - first);
+      const size_t zb = zs.b + k * zper;
+      const size_t ze = zb + zper < zs.e ? zb + zper : zs.e;
+      for (size_t i = zb + tid; i < ze; i += 256) __builtin_nontemporal_store(zero4, &zs.base4[i]);
+    } else {
+      // the previous step's frame (a clamped frame index: see above)
+      if (step > first) {
+        const long long pf = f - kFps < frames ? f - kFps : frames - 1;
+        f32x2* __restrict__ o = reinterpret_cast<f32x2*>(a.out + (size_t)pf * N);
+        const float2* wl2 = reinterpret_cast<const float2*>(wl);
+#pragma unroll
+        for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+          for (int r = 0; r < Rl; ++r) {
+            const int m = t + q * T + r * (L / Rl);
+            const float2 y = pv[q * Rl + r];
+            const float2 w = wl2[m];
+            const f32x2 val = {y.x * w.x, -y.y * w.y};
+            __builtin_nontemporal_store(val, &o[m]);
+          }
+      }
+    }
     wave_lds_fence();
     float2 v[E];
 #pragma unroll
@@ -158,23 +267,12 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
     wave_lds_fence();
-    if (step + 1 < a.steps) fetch(f + kFps);  // in flight during the transform
     WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
-    // v[q*Rl + r] = DFT(conj 2Z)[m], m = t + q*T + r*L/Rl; y_m = conj of it
-
-    if constexpr (MODE == kFrameMajor) {
-      if (f < frames) {
-        float2* __restrict__ o = reinterpret_cast<float2*>(a.out + (size_t)f * N);
-        const float2* wl2 = reinterpret_cast<const float2*>(wl);
+    // v[q*Rl + r] = DFT(conj 2Z)[m], m = t + q*T + r*L/Rl; y = conj of it:
+    // x[2m] = Re y_m / N, x[2m+1] = Im y_m / N (the 1/N is in wl)
+    if constexpr (!OLA) {
 #pragma unroll
-        for (int q = 0; q < E / Rl; ++q)
-#pragma unroll
-          for (int r = 0; r < Rl; ++r) {
-            const int m = t + q * T + r * (L / Rl);
-            const float2 w = wl2[m];
-            o[m] = make_float2(v[q * Rl + r].x * w.x, -v[q * Rl + r].y * w.y);
-          }
-      }
+      for (int e = 0; e < E; ++e) pv[e] = v[e];
       wave_lds_fence();  // buf is rewritten by the next step
     } else {
 #pragma unroll
@@ -187,34 +285,32 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
       __syncthreads();
 
       // Overlap-add of the step's frames over the samples they touch, ascending. The
-      // first FPS*H samples get no later frame: they are final (written unless this is
-      // a warm-up step) and their ring entries cleared; the rest stay in the ring.
+      // first FPS*H samples get no later frame: they are final and stay in the ring,
+      // normalised, until the next step stores them (unless this is a warm-up step).
       const long long j0 = fbase * H;
       const long long room = (long long)a.ola_len - j0;
       const int span = (kFps - 1) * H + N;
-      const int uhi = room < span ? (int)room : span;
+      const int uhi = room < span ? (int)(room > 0 ? room : 0) : span;
       const int fin = kFps * H;
       const int rb = (int)(j0 & (ACC - 1));
       bool vg[kFps];
 #pragma unroll
       for (int g = 0; g < kFps; ++g) vg[g] = fbase + g >= 0 && fbase + g < frames;
-      const bool write = step >= 0;
-      // every frame touching the step's final samples exists: the periodic norm applies
+      // every frame touching the step's final samples exists: the periodic sum applies
       const bool inner = fbase >= (long long)a.kd && fbase + kFps <= frames;
-      float* __restrict__ out = a.out + j0;
-      auto norm_of = [&](int u) -> float {
-        if (inner) return nh[u % H];
-        float s = 0.0f;
+      auto edge_norm = [&](int u) -> float {
+        float sum = 0.0f;
         for (int d = -a.kd; d < kFps; ++d) {  // frames ascending
           const int n = u - d * H;
           const long long fr = fbase + d;
-          if (n >= 0 && n < N && fr >= 0 && fr < frames) s += cola_weight(a, n);
+          if (n >= 0 && n < N && fr >= 0 && fr < frames) sum += cw[n];
         }
-        return s;
+        return sum;
       };
-      auto finish = [&](float sum, int u) -> float {
-        if constexpr (MODE == kOverlapCola) {
-          const float nrm = norm_of(u);
+      auto finish = [&](float sum, int u, int r) -> float {
+        if constexpr (COLA) {
+          if (inner) return sum * nhr[r];
+          const float nrm = edge_norm(u);
           return nrm > 1e-12f ? sum / nrm : sum;
         } else {
           return sum;
@@ -225,6 +321,8 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
         // buffer and of the ring (j0 and rb are even)
         float2* acc2 = reinterpret_cast<float2*>(acc);
         const float2* wl2 = reinterpret_cast<const float2*>(wl);
+        int r = COLA ? (2 * tid) % H : 0;  // residue of u mod H
+        const int dr = COLA ? 512 % H : 0;
         for (int u = 2 * tid; u < uhi; u += 512) {
           const int ri = ((rb + u) & (ACC - 1)) >> 1;
           float2 sum = acc2[ri];
@@ -239,25 +337,18 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
             }
           }
           if (u < fin) {
-            if (write) {
-              if (u + 1 < uhi) {
-                const float2 o = make_float2(finish(sum.x, u), finish(sum.y, u + 1));
-                if ((((uintptr_t)(out + u)) & 7) == 0) {
-                  *reinterpret_cast<float2*>(out + u) = o;
-                } else {
-                  out[u] = o.x;
-                  out[u + 1] = o.y;
-                }
-              } else {
-                out[u] = finish(sum.x, u);
-              }
-            }
-            acc2[ri] = make_float2(0.0f, 0.0f);
-          } else {
-            acc2[ri] = sum;
+            const int r1 = r + 1 == H ? 0 : r + 1;
+            sum = make_float2(finish(sum.x, u, r), finish(sum.y, u + 1, r1));
+          }
+          acc2[ri] = sum;
+          if (COLA) {
+            r += dr;
+            r = r >= H ? r - H : r;
           }
         }
       } else {
+        int r = COLA ? tid % H : 0;
+        const int dr = COLA ? 256 % H : 0;
         for (int u = tid; u < uhi; u += 256) {
           const int ri = (rb + u) & (ACC - 1);
           float sum = acc[ri];
@@ -269,18 +360,37 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
               sum = __builtin_fmaf((n & 1) ? c.y : c.x, wl[n], sum);
             }
           }
-          if (u < fin) {
-            if (write) out[u] = finish(sum, u);
-            acc[ri] = 0.0f;
-          } else {
-            acc[ri] = sum;
+          acc[ri] = u < fin ? finish(sum, u, r) : sum;
+          if (COLA) {
+            r += dr;
+            r = r >= H ? r - H : r;
           }
         }
       }
+      pj0 = j0;
+      pfin = uhi < fin ? uhi : fin;
+      pwrite = step >= 0;
       __syncthreads();
     }
   }
-  if constexpr (MODE != kFrameMajor) zero_tail(a.out, a.ola_len, (size_t)a.frames * N, rank, gridDim.x);
+  if constexpr (OLA) {
+    flush();
+  } else if (a.steps > first) {
+    const long long fl = f0 + (a.steps - 1) * kFps + slot;
+    const long long pf = fl < frames ? fl : frames - 1;
+    f32x2* __restrict__ o = reinterpret_cast<f32x2*>(a.out + (size_t)pf * N);
+    const float2* wl2 = reinterpret_cast<const float2*>(wl);
+#pragma unroll
+    for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+      for (int r = 0; r < Rl; ++r) {
+        const int m = t + q * T + r * (L / Rl);
+        const float2 y = pv[q * Rl + r];
+        const float2 w = wl2[m];
+        const f32x2 val = {y.x * w.x, -y.y * w.y};
+        __builtin_nontemporal_store(val, &o[m]);
+      }
+  }
 }
 
 // General path, step 1: the column rebuild of fft_cffi.cpp:1062-1098, one thread per
@@ -347,7 +457,8 @@ hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
   const size_t capacity = (size_t)(cus > 0 ? cus : 1) * (size_t)(occ > 0 ? occ : 1);
   // frame positions the runs cover: every frame, or every hop of the overlap-added output
   const size_t positions = MODE == kFrameMajor ? a.frames : (a.ola_len + a.hop - 1) / a.hop;
-  size_t steps = (positions + kFps * capacity - 1) / (kFps * capacity);
+  const size_t slots = (capacity * kIstftFill + 99) / 100;
+  size_t steps = (positions + kFps * slots - 1) / (kFps * slots);
   const size_t lo = MODE == kFrameMajor ? 1 : (size_t)(2 * a.warm > 4 ? 2 * a.warm : 4);
   steps = steps < lo ? lo : (steps > 256 ? 256 : steps);
   a.steps = (int)steps;
