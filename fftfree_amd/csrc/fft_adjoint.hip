@@ -287,30 +287,54 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
     const long long f = f0 + slot;
     fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(f) ? (size_t)(f * hop) : 0, W, frame_valid(f));
   }
+  // Final samples of the previous step wait in the ring until the next step has used its
+  // gradients; they are stored then, a whole transform ahead of the next wait for loads
+  // (with stores pending, a wait for a load is a wait for every access in flight).
+  long long pj0 = 0;
+  int pcnt = 0, plh = 0;
+  auto flush = [&]() {
+    if (pcnt <= 0) return;
+    const int prb = (int)(pj0 & (ACC - 1));
+    float* __restrict__ out = gx + pj0;
+    float* __restrict__ hl = a.halo + ((size_t)(sig * a.tiles + tix) * 2 + 1) * halo + (pj0 - f0 * hop);
+    for (int u = tid; u < pcnt; u += 256) {
+      const int ri = (prb + u) & (ACC - 1);
+      if (u < plh)
+        hl[u] = acc[ri];
+      else
+        out[u] = acc[ri];
+      acc[ri] = 0.0f;
+    }
+  };
   for (long long step = 0; step < steps; ++step) {
     const long long fbase = f0 + step * FPS;
     const long long f = fbase + slot;
     const bool valid = frame_valid(f);  // invalid groups run on zeros (barriers stay uniform)
-
-    // upstream gradients of this frame's bins, requested before the transform
-    const size_t row = valid ? (size_t)f * BINS : 0;
-    float gAp[JF], gAq[JF], gBp[JF], gBq[JF];
-#pragma unroll
-    for (int j = 0; j < JF; ++j) {
-      const int p = t + j * T;
-      gAp[j] = (ga && valid) ? ga[row + p] : 0.0f;
-      gAq[j] = (ga && valid) ? ga[row + L - p] : 0.0f;
-      gBp[j] = (gb && valid) ? gb[row + p] : 0.0f;
-      gBq[j] = (gb && valid) ? gb[row + L - p] : 0.0f;
-    }
-    const float gAm = (t == 0 && ga && valid) ? ga[row + L / 2] : 0.0f;
-    const float gBm = (t == 0 && gb && valid) ? gb[row + L / 2] : 0.0f;
 
     // 1. r_f into buf
     float2 v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = nxt[e];
     window_frame<L, E, R0>(v, t, wl);
+    // Wait here for the samples (and the stores issued with them, a transform ago), so
+    // that the gradient loads below are not part of that wait: s_waitcnt vmcnt(0)
+    // expcnt(7) lgkmcnt(15).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __builtin_amdgcn_sched_barrier(0);
+    // upstream gradients of this frame's bins, requested before the transform (a clamped
+    // row: an invalid frame's spectrum is zero, or its r_f is never added)
+    const size_t row = (size_t)(valid ? f : (f < 0 ? 0 : frames - 1)) * BINS;
+    float gAp[JF], gAq[JF], gBp[JF], gBq[JF];
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      gAp[j] = ga ? ga[row + p] : 0.0f;
+      gAq[j] = ga ? ga[row + L - p] : 0.0f;
+      gBp[j] = gb ? gb[row + p] : 0.0f;
+      gBq[j] = gb ? gb[row + L - p] : 0.0f;
+    }
+    const float gAm = ga ? ga[row + L / 2] : 0.0f;
+    const float gBm = gb ? gb[row + L / 2] : 0.0f;
     float2* fb;
     StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
     store_natural<L, E, Rl>(v, t, buf);
@@ -356,6 +380,10 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
       const float2 Xm = c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x)));
       buf[pidx(L / 2)] = grad_spectrum<MODE>(Xm, gAm, gBm, false);
     }
+    // every gradient load has been used (lane 0 of a frame also used the middle bin's):
+    // waiting here costs nothing and keeps the stores below out of later waits
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    flush();  // the previous step's final samples (ring entries no one reads before the next barrier)
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E / R0; ++q)
@@ -431,26 +459,17 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
     }
     __syncthreads();
 
-    // 3. samples [fbase*H, (fbase+FPS)*H) have all of this run's frames: write them
-    //    (the first `halo` samples of a run after the first: to the left halo slot)
-    //    and clear their ring entries
+    // 3. samples [fbase*H, (fbase+FPS)*H) have all of this run's frames: they are stored
+    //    by the next step's flush (the first `halo` samples of a run after the first go
+    //    to the left halo slot), which also clears their ring entries
     {
       const long long room = len - j0;
-      const int uhi = room < FPS * (int)hop ? (int)room : FPS * (int)hop;
-      const int lh = tix > 0 ? (int)(f0 * hop + halo - j0) : 0;  // u below lh: halo
-      float* __restrict__ out = gx + j0;
-      float* __restrict__ hl = a.halo + ((size_t)(sig * a.tiles + tix) * 2 + 1) * halo + (j0 - f0 * hop);
-      for (int u = tid; u < uhi; u += 256) {
-        const int ri = (rb + u) & (ACC - 1);
-        if (u < lh)
-          hl[u] = acc[ri];
-        else
-          out[u] = acc[ri];
-        acc[ri] = 0.0f;
-      }
+      pj0 = j0;
+      pcnt = room < FPS * (int)hop ? (int)(room > 0 ? room : 0) : FPS * (int)hop;
+      plh = tix > 0 ? (int)(f0 * hop + halo - j0) : 0;  // u below plh: halo
     }
-    __syncthreads();
   }
+  flush();
 
   // right halo: the samples past the run that its last frames touch
   const long long jr = fend * hop;
