@@ -17,6 +17,7 @@ from pathlib import Path
 LIB_PATH = Path(os.environ.get("FFTFREE_CFFI_LIB", Path(__file__).resolve().parent / "libfft_cffi.so"))
 
 c_int, c_float, c_size_t, c_void_p, c_char_p = ctypes.c_int, ctypes.c_float, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p
+c_uint = ctypes.c_uint
 f32p = ctypes.POINTER(ctypes.c_float)
 i32p = ctypes.POINTER(ctypes.c_int)
 
@@ -57,6 +58,9 @@ SIGNATURES = {
     "fft_execute_c2c_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fft_execute_complex_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
                                                       c_void_p]),
+    "fft_griffin_lim": (c_int, [c_void_p, c_void_p, f32p, c_size_t, c_int, c_int, c_int, c_int, c_uint, f32p, f32p]),
+    "fft_griffin_lim_device": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_int, c_int, c_int, c_int, c_uint,
+                                       c_void_p, c_void_p, c_void_p]),
     "fft_stft_signals_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
                                            c_int, c_void_p]),
     "fft_stft_adjoint_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,

@@ -224,6 +224,34 @@ class CffiFft:
             raise RuntimeError("fft_execute_complex_batched_device failed")
         return out
 
+    def griffin_lim(self, fwd: FftPlan, inv: FftPlan, mag: np.ndarray, hop: int = 0, iterations: int = 32,
+                    pad_mode: int = 1, seed: int = 0):
+        """fft_griffin_lim (the reference's phase reconstruction) on a (frames, bins) float32
+        magnitude array; returns (real, imag) of the same shape."""
+        mag = np.ascontiguousarray(mag, dtype=np.float32)
+        re = np.zeros_like(mag)
+        im = np.zeros_like(mag)
+        ok = int(self.lib.fft_griffin_lim(fwd.handle, inv.handle, _fp(mag), mag.shape[0], hop, int(inv.half),
+                                          iterations, pad_mode, seed & 0xFFFFFFFF, _fp(re), _fp(im)))
+        if ok != 1:
+            raise RuntimeError("fft_griffin_lim failed")
+        return re, im
+
+    def griffin_lim_device(self, fwd: FftPlan, inv: FftPlan, mag, out_re=None, out_im=None, hop: int = 0,
+                           iterations: int = 32, pad_mode: int = 1, seed: int = 0):
+        """fft_griffin_lim_device on a (frames, bins) float32 CUDA tensor; returns (real, imag)."""
+        import torch
+        assert mag.dim() == 2 and mag.is_contiguous()
+        out_re = torch.empty_like(mag) if out_re is None else out_re
+        out_im = torch.empty_like(mag) if out_im is None else out_im
+        vp = lambda t: ctypes.c_void_p(t.data_ptr())
+        ok = int(self.lib.fft_griffin_lim_device(fwd.handle, inv.handle, vp(mag), mag.shape[0], hop, int(inv.half),
+                                                 iterations, pad_mode, seed & 0xFFFFFFFF, vp(out_re), vp(out_im),
+                                                 self._stream()))
+        if ok != 1:
+            raise RuntimeError("fft_griffin_lim_device failed")
+        return out_re, out_im
+
     def stft_signals_device(self, plan: FftPlan, pcm, out_re=None, out_im=None, out_mag=None,
                             pad_mode: int = 0) -> int:
         """fft_stft_signals_device: pcm is a (signals, length) float32 CUDA tensor; planes

@@ -481,16 +481,16 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
 }
 
 // Halo samples of every run boundary: previous run's partial sum + this run's. One
-// thread per sample; `chunks` workgroups per boundary, boundaries of all signals in turn.
-__global__ void __launch_bounds__(256) stft_halo_kernel(SignalArgs a, int halo, int chunks, size_t run_len) {
-  const size_t b = blockIdx.x / chunks;  // boundary b of signal sig: between runs r-1 and r
-  const int i = (int)(blockIdx.x - b * chunks) * 256 + threadIdx.x;
+// workgroup per boundary, boundaries of all signals in turn.
+__global__ void __launch_bounds__(256) stft_halo_kernel(SignalArgs a, int halo, size_t run_len) {
+  const size_t b = blockIdx.x;  // boundary b of signal sig: between runs r-1 and r
   const size_t sig = b / (size_t)(a.tiles - 1);
   const size_t r = b - sig * (a.tiles - 1) + 1;
   const size_t jb = r * run_len;
-  if (i >= halo || jb + i >= a.len) return;
   const float* pr = a.halo + ((sig * a.tiles + r) * 2 + 0) * (size_t)halo;
-  a.grad[sig * a.len + jb + i] = pr[i] + pr[halo + i];
+  float* __restrict__ g = a.grad + sig * a.len + jb;
+  const int n = jb >= a.len ? 0 : (jb + halo <= a.len ? halo : (int)(a.len - jb));
+  for (int i = threadIdx.x; i < n; i += 256) g[i] = pr[i] + pr[halo + i];
 }
 
 template <int L, int E, int... Rs>
@@ -553,11 +553,10 @@ hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
   void* args[] = {&a};
   e = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
   if (e != hipSuccess || !halos) return e;
-  const int chunks = (int)((halo + 255) / 256);
-  const size_t hgrid = a.signals * (size_t)(a.tiles - 1) * chunks;
+  const size_t hgrid = a.signals * (size_t)(a.tiles - 1);
   if (hgrid > 0x7fffffffull) return hipErrorInvalidValue;
   const size_t run_len = (size_t)FPS * steps * (size_t)a.hop;
-  hipLaunchKernelGGL(stft_halo_kernel, dim3((unsigned)hgrid), dim3(256), 0, s, a, (int)halo, chunks, run_len);
+  hipLaunchKernelGGL(stft_halo_kernel, dim3((unsigned)hgrid), dim3(256), 0, s, a, (int)halo, run_len);
   return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <random>
 #include <string>
 #include <thread>
 #include <vector>
@@ -95,6 +96,7 @@ struct FftContext {
   DeviceBuffer d_awin, d_swin, d_pcm, d_planes, d_cplx;
   DeviceBuffer d_sink;  // N floats that kernels write skipped output planes to
   DeviceBuffer d_halo;  // adjoint run-boundary partial sums
+  DeviceBuffer d_gl;    // Griffin-Lim working set (on the inverse context)
   bool windows_dirty = true;
 
   ~FftContext() {
@@ -105,6 +107,7 @@ struct FftContext {
     d_cplx.release();
     d_sink.release();
     d_halo.release();
+    d_gl.release();
     plan.reset();
     cache.clear();
     if (stream) {
@@ -983,8 +986,155 @@ size_t complex_batched(FftContext* c, const float* d_re, const float* d_im, size
   return frames;
 }
 
+// Griffin-Lim (phase_infer.hpp:69-204, called from fft_cffi.cpp:2176-2248) with every
+// buffer on the device. Per iteration: the inverse context's transform of the current
+// spectrum, overlap-add averaged over the frames per sample, the forward context's STFT,
+// and the phase update X = |X| (cos, sin)(atan2(Im, Re)) of the frames the forward
+// produced. When the inverse context is what the fused inverse kernel runs, that kernel
+// does the transform and the averaged overlap-add, with the phase update applied as it
+// loads the bins, and the update is written out once at the end.
+bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t frames, int hop, int half,
+                 int iterations, int pad_mode, unsigned seed, float* d_re, float* d_im, hipStream_t s) {
+  const int N = iv->N;
+  if (N <= 0 || fw->N != N) return false;  // the reference would write past its buffers
+  const size_t bins = half ? (size_t)(N / 2 + 1) : (size_t)N;
+  if (bins != bins_of(iv) || bins != bins_of(fw)) return false;
+  const size_t total = frames * bins;
+  const size_t ola_len = (frames - 1) * (size_t)hop + (size_t)N;
+  const int iters = iterations > 0 ? iterations : 32;
+  const size_t fwd_frames = frame_count(fw, ola_len, pad_mode, frames);
+  if (fwd_frames == 0) return false;
+  const size_t uf = std::min(fwd_frames, frames);
+  ensure_gpu(fw);
+  const bool fused = iv->inverse && iv->transform == FFT_TRANSFORM_C2R && iv->half_spectrum && !iv->store_polar &&
+                     !iv->reduce_magnitude && !iv->apply_ola && istft_fused_supported(N, hop, 1);
+  // working set: start words, the overlap-added signal, the forward planes, and (general
+  // path) the frame-major inverse output
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t o_words = 0, o_ola = up(total * 4), o_tre = o_ola + up(ola_len * 4), o_tim = o_tre + up(total * 4),
+               o_fr = o_tim + up(total * 4), bytes = o_fr + (fused ? 0 : up(frames * (size_t)N * 4));
+  iv->d_gl.reserve(bytes);
+  char* base = iv->d_gl.as<char>();
+  uint32_t* d_words = reinterpret_cast<uint32_t*>(base + o_words);
+  float* d_ola = reinterpret_cast<float*>(base + o_ola);
+  float* d_tre = reinterpret_cast<float*>(base + o_tre);
+  float* d_tim = reinterpret_cast<float*>(base + o_tim);
+  float* d_fr = reinterpret_cast<float*>(base + o_fr);
+
+  // the start: mt19937 is sequential, so its words come from the host (phase_infer.hpp:124-134)
+  {
+    std::mt19937 rng(seed ? seed : 0x47d7bfadu);
+    std::vector<uint32_t> w(total);
+    for (size_t i = 0; i < total; ++i) w[i] = (uint32_t)rng();
+    hip_check(hipMemcpyAsync(d_words, w.data(), total * 4, hipMemcpyHostToDevice, s), "H2D words");
+    hip_check(launch_gl_start(d_mag, d_words, total, d_re, d_im, s), "launch_gl_start");
+    hip_check(hipStreamSynchronize(s), "sync");  // w goes out of scope
+  }
+  const StftArgs fa = stft_args(fw, d_ola, ola_len);
+  for (int it = 0; it < iters; ++it) {
+    if (fused) {
+      IstftArgs a;
+      a.in_re = d_re;
+      a.in_im = d_im;
+      a.frames = frames;
+      a.n = N;
+      a.bins = (int)bins;
+      a.hop = hop;
+      a.tw = iv->plan->W;
+      a.swin = (iv->windows_enabled && !iv->synthesis_win.empty()) ? iv->d_swin.as<float>() : nullptr;
+      a.ola = 1;
+      a.cola = 1;
+      a.unit_cola = 1;
+      a.out = d_ola;
+      a.out_len = ola_len;
+      a.gl_mag = d_mag;
+      a.gl_re = d_tre;
+      a.gl_im = d_tim;
+      a.gl_frames = it > 0 ? uf : 0;
+      hip_check(launch_istft(a, s), "launch_istft (Griffin-Lim)");
+    } else {
+      if (it > 0)
+        hip_check(launch_gl_update(d_mag, d_tre, d_tim, uf, (int)bins, half, (N & 1) == 0, d_re, d_im, s),
+                  "launch_gl_update");
+      complex_batched(iv, d_re, d_im, frames, d_fr, s);
+      hip_check(launch_gl_overlap(d_fr, frames, N, hop, d_ola, ola_len, s), "launch_gl_overlap");
+    }
+    run_stft_frames(fw, fa, 0, uf, d_tre, d_tim, nullptr, s);
+  }
+  hip_check(launch_gl_update(d_mag, d_tre, d_tim, uf, (int)bins, half, (N & 1) == 0, d_re, d_im, s),
+            "launch_gl_update");
+  return true;
+}
+
+// Argument checks and hop choice of fft_griffin_lim (fft_cffi.cpp:2176-2248).
+int griffin_lim_args(void* ctx_forward, void* ctx_inverse, const void* in_mag, size_t frames, int* hop,
+                     int half_spectrum, const void* out_real, const void* out_imag) {
+  if (!ctx_forward || !ctx_inverse || !in_mag || !out_real || !out_imag || frames == 0) return 0;
+  if (in_mag == out_real || in_mag == out_imag || out_real == out_imag) return 0;
+  auto* fw = static_cast<FftContext*>(ctx_forward);
+  auto* iv = static_cast<FftContext*>(ctx_inverse);
+  const int N = iv->N > 0 ? iv->N : fw->N;
+  if (N <= 0) return 0;
+  const bool half = half_spectrum != 0;
+  if (half != (iv->half_spectrum != 0) || half != (fw->half_spectrum != 0)) return 0;
+  if (*hop <= 0) *hop = iv->hop > 0 ? iv->hop : (fw->hop > 0 ? fw->hop : N);
+  return 1;
+}
+
 }  // namespace
 }  // namespace fftfree
+
+int fft_griffin_lim_device(void* ctx_forward, void* ctx_inverse, const float* d_in_mag, size_t frames, int hop,
+                           int half_spectrum, int iterations, int pad_mode, unsigned int seed, float* d_out_real,
+                           float* d_out_imag, void* stream) {
+  if (!fftfree::griffin_lim_args(ctx_forward, ctx_inverse, d_in_mag, frames, &hop, half_spectrum, d_out_real,
+                                 d_out_imag))
+    return 0;
+  try {
+    auto* iv = static_cast<FftContext*>(ctx_inverse);
+    fftfree::ensure_gpu(iv);
+    return fftfree::griffin_lim(static_cast<FftContext*>(ctx_forward), iv, d_in_mag, frames, hop, half_spectrum != 0,
+                                iterations, pad_mode, seed, d_out_real, d_out_imag, static_cast<hipStream_t>(stream))
+               ? 1
+               : 0;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_griffin_lim_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+int fft_griffin_lim(void* ctx_forward, void* ctx_inverse, const float* in_mag, size_t frames, int hop,
+                    int half_spectrum, int iterations, int pad_mode, unsigned int seed, float* out_real,
+                    float* out_imag) {
+  if (!fftfree::griffin_lim_args(ctx_forward, ctx_inverse, in_mag, frames, &hop, half_spectrum, out_real, out_imag))
+    return 0;
+  try {
+    auto* iv = static_cast<FftContext*>(ctx_inverse);
+    fftfree::ensure_gpu(iv);
+    const size_t bins = fftfree::bins_of(iv);
+    const size_t bytes = frames * bins * sizeof(float);
+    const size_t off = (bytes + 255) & ~size_t(255);
+    iv->d_planes.reserve(3 * off);
+    float* d_mag = iv->d_planes.as<float>();
+    float* d_re = reinterpret_cast<float*>(iv->d_planes.as<char>() + off);
+    float* d_im = reinterpret_cast<float*>(iv->d_planes.as<char>() + 2 * off);
+    fftfree::hip_check(hipMemcpyAsync(d_mag, in_mag, bytes, hipMemcpyHostToDevice, iv->stream), "H2D");
+    if (!fftfree::griffin_lim(static_cast<FftContext*>(ctx_forward), iv, d_mag, frames, hop, half_spectrum != 0,
+                              iterations, pad_mode, seed, d_re, d_im, iv->stream))
+      return 0;
+    fftfree::hip_check(hipMemcpyAsync(out_real, d_re, bytes, hipMemcpyDeviceToHost, iv->stream), "D2H");
+    fftfree::hip_check(hipMemcpyAsync(out_imag, d_im, bytes, hipMemcpyDeviceToHost, iv->stream), "D2H");
+    fftfree::hip_check(hipStreamSynchronize(iv->stream), "sync");
+    return 1;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_griffin_lim", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
 
 size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
                                           size_t frames, float* d_out_pcm, size_t max_frames, void* stream) {

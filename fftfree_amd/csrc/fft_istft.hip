@@ -49,9 +49,22 @@ enum IstftMode : int { kFrameMajor = 0, kOverlap = 1, kOverlapCola = 2 };
 
 // COLA weight of frame tap n (fft_cffi.cpp:210-226).
 __device__ __forceinline__ float cola_weight(const IstftArgs& a, int n) {
+  if (a.unit_cola) return 1.0f;
   const float wa = a.awin ? a.awin[n] : 1.0f;
   const float ws = a.swin ? a.swin[n] : 1.0f;
   return wa * ws;
+}
+
+// m * (cos, sin)(atan2(y, x)), the Griffin-Lim phase update (phase_infer.hpp:187-191),
+// as m * (x, y) / |(x, y)|; for x = y = 0 the values atan2 gives for signed zeros
+// (angle 0 or +-pi; sin(pi) in float is -8.742278e-8).
+__device__ __forceinline__ float2 gl_dir(float m, float x, float y) {
+  if (x == 0.0f && y == 0.0f) {
+    if (!__builtin_signbit(x)) return make_float2(m, __builtin_copysignf(0.0f, y) * m);
+    return make_float2(-m, (__builtin_signbit(y) ? 8.742278e-8f : -8.742278e-8f) * m);
+  }
+  const float inv = 1.0f / hypotf(x, y);
+  return make_float2(m * (x * inv), m * (y * inv));
 }
 
 // The float4-aligned part of out[from, to) that rank `rank` of `ranks` zero-fills, in
@@ -88,7 +101,7 @@ __device__ __forceinline__ void zero_edges(float* __restrict__ out, size_t from,
   if (rank == ranks - 1 && threadIdx.x < rest) base[head + 4 * n4 + threadIdx.x] = 0.0f;
 }
 
-template <int L, int E, int MODE, int ACCM, int... Rs>
+template <int L, int E, int MODE, int ACCM, bool GLM, int... Rs>
 __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
@@ -144,12 +157,14 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
   // bins of the frame a step ahead, in registers (the index is clamped so the loads are
   // unconditional; frames outside [0, frames) count as zero where the values are used)
   float xr[2 * JF], xi[2 * JF], mr, mi;
-  bool xvalid = false;
+  float xm[GLM ? 2 * JF : 1], mm = 0.0f;  // Griffin-Lim: |X| of the same bins
+  bool xvalid = false, xupd = false;
   auto fetch = [&](long long f) {
     xvalid = f >= 0 && f < frames;
     const long long fc = f < 0 ? 0 : (f >= frames ? frames - 1 : f);
-    const float* __restrict__ re = a.in_re + (size_t)fc * BINS;
-    const float* __restrict__ im = a.in_im + (size_t)fc * BINS;
+    xupd = GLM && (size_t)fc < a.gl_frames;
+    const float* __restrict__ re = (xupd ? a.gl_re : a.in_re) + (size_t)fc * BINS;
+    const float* __restrict__ im = (xupd ? a.gl_im : a.in_im) + (size_t)fc * BINS;
 #pragma unroll
     for (int j = 0; j < JF; ++j) {
       const int p = t + j * T;
@@ -160,6 +175,16 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
     }
     mr = re[L / 2];
     mi = im[L / 2];
+    if constexpr (GLM) {
+      const float* __restrict__ mg = a.gl_mag + (size_t)fc * BINS;
+#pragma unroll
+      for (int j = 0; j < JF; ++j) {
+        const int p = t + j * T;
+        xm[2 * j] = mg[p];
+        xm[2 * j + 1] = mg[L - p];
+      }
+      mm = mg[L / 2];
+    }
   };
 
   const long long first = (OLA && f0 > 0) ? -(long long)a.warm : 0;
@@ -167,10 +192,10 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
   ZeroSpan zs;
   size_t zper = 0;
   if constexpr (OLA) {
-    zs = zero_span(a.out, a.ola_len, a.frames * (size_t)N, rank, gridDim.x);
+    zs = zero_span(a.out, a.ola_len, a.out_len, rank, gridDim.x);
     const size_t nsteps = (size_t)(a.steps - first);
     zper = (zs.e - zs.b + nsteps - 1) / nsteps;
-    zero_edges(a.out, a.ola_len, a.frames * (size_t)N, rank, gridDim.x);
+    zero_edges(a.out, a.ola_len, a.out_len, rank, gridDim.x);
   }
   const f32x4 zero4 = {0.0f, 0.0f, 0.0f, 0.0f};
 
@@ -215,6 +240,19 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
     // output keeps a clamped frame's values: the wave then rewrites the last frame's
     // samples with the same values, so its stores need no condition.
     const float keep = (xvalid || !OLA) ? 1.0f : 0.0f;
+    if constexpr (GLM) {
+      if (xupd) {  // X = |X| dir(previous forward spectrum)
+#pragma unroll
+        for (int j = 0; j < 2 * JF; ++j) {
+          const float2 z = gl_dir(xm[j], xr[j], xi[j]);
+          xr[j] = z.x;
+          xi[j] = z.y;
+        }
+        const float2 z = gl_dir(mm, mr, mi);
+        mr = z.x;
+        mi = z.y;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < JF; ++j) {
       const int p = t + j * T;
@@ -446,9 +484,63 @@ __global__ void __launch_bounds__(256) overlap_kernel(IstftArgs a, const float2*
   }
 }
 
-template <int L, int E, int MODE, int ACCM, int... Rs>
+// Griffin-Lim start (phase_infer.hpp:124-134): the float uniform on [-pi, pi) that
+// std::uniform_real_distribution<float> makes of one mt19937 word w: u = float(w) / 2^32
+// (generate_canonical<float, 24>, with u = 1 taken to the float below 1), angle =
+// u * (b - a) + a with one rounding (the reference is built with -ffast-math
+// -march=native, CMakeLists.txt:16, which fuses it), then mag * (cos, sin).
+__global__ void __launch_bounds__(256) gl_start_kernel(const float* __restrict__ mag,
+                                                       const uint32_t* __restrict__ words, size_t count,
+                                                       float* __restrict__ o_re, float* __restrict__ o_im) {
+  constexpr float kPi = 3.14159265358979323846f;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) {
+    float u = (float)words[i] * 2.3283064365386963e-10f;  // 2^-32
+    if (u >= 1.0f) u = 0.99999994f;                      // nextafter(1, 0)
+    const float ang = __builtin_fmaf(u, kPi - (-kPi), -kPi);
+    float sn, cs;
+    sincosf(ang, &sn, &cs);
+    const float m = mag[i];
+    o_re[i] = m * cs;
+    o_im[i] = m * sn;
+  }
+}
+
+// Griffin-Lim phase update (phase_infer.hpp:185-200).
+__global__ void __launch_bounds__(256) gl_update_kernel(const float* __restrict__ mag, const float* __restrict__ t_re,
+                                                        const float* __restrict__ t_im, size_t count, int bins,
+                                                        int half, int n_even, float* __restrict__ o_re,
+                                                        float* __restrict__ o_im) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) {
+    const float2 z = gl_dir(mag[i], t_re[i], t_im[i]);
+    const int k = (int)(i % (size_t)bins);
+    const bool real_bin = half && (k == 0 || (n_even && bins > 1 && k == bins - 1));
+    o_re[i] = z.x;
+    o_im[i] = real_bin ? 0.0f : z.y;
+  }
+}
+
+// Griffin-Lim overlap-add (phase_infer.hpp:153-173): each sample averages the frames that
+// cover it, summed in ascending frame order.
+__global__ void __launch_bounds__(256) gl_overlap_kernel(const float* __restrict__ fr, size_t frames, int n, int hop,
+                                                         float* __restrict__ out, size_t ola_len) {
+  const size_t N = (size_t)n, H = (size_t)hop;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < ola_len; t += (size_t)gridDim.x * 256) {
+    const size_t flo = t >= N ? (t - N) / H + 1 : 0;
+    size_t fhi = t / H;
+    if (fhi > frames - 1) fhi = frames - 1;
+    float sum = 0.0f, cnt = 0.0f;
+    for (size_t f = flo; f <= fhi; ++f) {
+      if (f * H + N > ola_len) break;
+      sum += fr[f * N + (t - f * H)];
+      cnt += 1.0f;
+    }
+    out[t] = cnt > 0.0f ? sum / cnt : sum;
+  }
+}
+
+template <int L, int E, int MODE, int ACCM, bool GLM, int... Rs>
 hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
-  const void* kern = reinterpret_cast<const void*>(&istft_kernel<L, E, MODE, ACCM, Rs...>);
+  const void* kern = reinterpret_cast<const void*>(&istft_kernel<L, E, MODE, ACCM, GLM, Rs...>);
   int dev = 0, cus = 0, occ = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -472,14 +564,19 @@ hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
 
 template <int L, int E, int... Rs>
 hipError_t istft_launch(const IstftArgs& a, hipStream_t s) {
-  if (!a.ola) return istft_launch_mode<L, E, kFrameMajor, 2, Rs...>(a, s);
+  if (!a.ola) return istft_launch_mode<L, E, kFrameMajor, 2, false, Rs...>(a, s);
   const int span = (kFps - 1) * a.hop + 2 * L;
-  if (a.cola) {
-    if (span <= 4 * L) return istft_launch_mode<L, E, kOverlapCola, 2, Rs...>(a, s);
-    return istft_launch_mode<L, E, kOverlapCola, 4, Rs...>(a, s);
+  if (a.gl_mag) {  // Griffin-Lim: overlap-add divided by the frame count
+    if (!a.cola || !a.unit_cola) return hipErrorInvalidValue;
+    if (span <= 4 * L) return istft_launch_mode<L, E, kOverlapCola, 2, true, Rs...>(a, s);
+    return istft_launch_mode<L, E, kOverlapCola, 4, true, Rs...>(a, s);
   }
-  if (span <= 4 * L) return istft_launch_mode<L, E, kOverlap, 2, Rs...>(a, s);
-  return istft_launch_mode<L, E, kOverlap, 4, Rs...>(a, s);
+  if (a.cola) {
+    if (span <= 4 * L) return istft_launch_mode<L, E, kOverlapCola, 2, false, Rs...>(a, s);
+    return istft_launch_mode<L, E, kOverlapCola, 4, false, Rs...>(a, s);
+  }
+  if (span <= 4 * L) return istft_launch_mode<L, E, kOverlap, 2, false, Rs...>(a, s);
+  return istft_launch_mode<L, E, kOverlap, 4, false, Rs...>(a, s);
 }
 
 }  // namespace
@@ -500,6 +597,8 @@ hipError_t launch_istft(IstftArgs a, hipStream_t s) {
   if (!istft_fused_supported(a.n, a.hop, a.ola) || a.frames == 0 || !a.out) return hipErrorInvalidValue;
   if (a.ola) {
     a.ola_len = istft_ola_len(a.frames, a.n, a.hop);
+    if (a.out_len == 0) a.out_len = a.frames * (size_t)a.n;
+    if (a.out_len < a.ola_len) return hipErrorInvalidValue;
     a.kd = (a.n + a.hop - 1) / a.hop;              // frames that touch one sample, at most
     a.warm = (a.kd - 1 + kFps - 1) / kFps;          // warm-up steps before a run
   }
@@ -526,6 +625,37 @@ hipError_t launch_overlap(IstftArgs a, const float2* cols, hipStream_t s) {
   size_t grid = (total + 255) / 256;
   if (grid > 65536) grid = 65536;
   hipLaunchKernelGGL(overlap_kernel, dim3((unsigned)grid), dim3(256), 0, s, a, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_gl_start(const float* mag, const uint32_t* words, size_t count, float* o_re, float* o_im,
+                           hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  size_t grid = (count + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(gl_start_kernel, dim3((unsigned)grid), dim3(256), 0, s, mag, words, count, o_re, o_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_gl_update(const float* mag, const float* t_re, const float* t_im, size_t frames, int bins,
+                            int half, int n_even, float* o_re, float* o_im, hipStream_t s) {
+  const size_t count = frames * (size_t)bins;
+  if (count == 0) return hipSuccess;
+  size_t grid = (count + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(gl_update_kernel, dim3((unsigned)grid), dim3(256), 0, s, mag, t_re, t_im, count, bins, half,
+                     n_even, o_re, o_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_gl_overlap(const float* frames_pcm, size_t frames, int n, int hop, float* out, size_t ola_len,
+                             hipStream_t s) {
+  if (ola_len == 0 || frames == 0) return hipSuccess;
+  if (hop <= 0) return hipErrorInvalidValue;
+  size_t grid = (ola_len + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(gl_overlap_kernel, dim3((unsigned)grid), dim3(256), 0, s, frames_pcm, frames, n, hop, out,
+                     ola_len);
   return hipGetLastError();
 }
 

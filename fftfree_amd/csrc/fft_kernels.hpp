@@ -89,7 +89,15 @@ struct IstftArgs {
   const float* awin = nullptr;   // N analysis taps, or nullptr; COLA weight = awin*swin
   int ola = 0;                   // 0: N samples per frame, frame-major; 1: overlap-add
   int cola = 0;                  // with ola: divide by the summed COLA weights
-  float* out = nullptr;          // frames*N floats
+  int unit_cola = 0;             // with cola: every weight is 1 (the sum counts frames)
+  float* out = nullptr;          // frames*N floats (out_len with overlap-add)
+  size_t out_len = 0;            // overlap-add: samples written (the rest up to it zero); 0 = frames*N
+  // Griffin-Lim: frames below gl_frames take X = gl_mag * dir(gl_re + i gl_im) (the
+  // phase update, dir(z) = (cos, sin)(atan2(Im z, Re z))) instead of in_re / in_im
+  const float* gl_mag = nullptr;
+  const float* gl_re = nullptr;
+  const float* gl_im = nullptr;
+  size_t gl_frames = 0;
   size_t ola_len = 0;            // set by the launcher: samples the overlap-add reaches
   int kd = 0;                    // set by the launcher: ceil(N / hop)
   int warm = 0;                  // set by the launcher: warm-up steps per run
@@ -105,6 +113,20 @@ hipError_t launch_istft(IstftArgs a, hipStream_t s);
 // then launch_overlap for the output.
 hipError_t launch_complex_columns(const IstftArgs& a, int hermitian, float2* cols, hipStream_t s);
 hipError_t launch_overlap(IstftArgs a, const float2* cols, hipStream_t s);
+
+// Griffin-Lim (phase_infer.hpp:69-204) pieces. Start: bins i < count get mag[i] *
+// (cos, sin)(a) with a the float uniform on [-pi, pi) that mt19937 word words[i] gives
+// (generate_canonical<float, 24>, then a + (b - a) u as a fused multiply-add).
+hipError_t launch_gl_start(const float* mag, const uint32_t* words, size_t count, float* o_re, float* o_im,
+                           hipStream_t s);
+// Phase update of frames [0, frames): o = mag * dir(t); with `half`, imag of bins 0 and
+// bins-1 set to 0 (when N is even).
+hipError_t launch_gl_update(const float* mag, const float* t_re, const float* t_im, size_t frames, int bins,
+                            int half, int n_even, float* o_re, float* o_im, hipStream_t s);
+// Overlap-add of frames*N frame-major samples with hop H into out[0, ola_len), divided by
+// the number of frames per sample.
+hipError_t launch_gl_overlap(const float* frames_pcm, size_t frames, int n, int hop, float* out, size_t ola_len,
+                             hipStream_t s);
 
 // Metadata kinds (eigfft::MetadataKind order, eigen_fft.hpp:1477-1491).
 enum MetaKind : int {

@@ -150,6 +150,26 @@ size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_
 size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
                                           size_t frames, float* d_out_pcm, size_t max_frames, void* stream);
 
+/* reference: fft_cffi.hpp:213-223 (fft_cffi.cpp:2176-2248, phase_infer.hpp:69-204).
+ * Griffin-Lim phase reconstruction of a frame-major magnitude spectrogram (frames x
+ * bins): a random start (mt19937 seeded with `seed`, 0x47d7bfad when 0), then
+ * `iterations` rounds (32 when <= 0) of inverse transform with ctx_inverse, overlap-add
+ * averaged over the frames per sample (hop: the argument, else ctx_inverse's, else
+ * ctx_forward's, else N), forward STFT with ctx_forward and the phase update
+ * X = mag * (cos, sin)(arg X). Writes the final spectrum to out_real / out_imag (frames x
+ * bins). Returns 1, or 0 on bad arguments (NULL or aliased buffers, half_spectrum
+ * differing from either context, contexts of different N). Host pointers. */
+int fft_griffin_lim(void* ctx_forward, void* ctx_inverse, const float* in_mag, size_t frames, int hop,
+                    int half_spectrum, int iterations, int pad_mode, unsigned int seed, float* out_real,
+                    float* out_imag);
+
+/* additive. fft_griffin_lim on device buffers, enqueued on `stream` (the mt19937 words
+ * of the start are drawn on the host and copied in before the call returns). Scratch
+ * lives in ctx_inverse. */
+int fft_griffin_lim_device(void* ctx_forward, void* ctx_inverse, const float* d_in_mag, size_t frames, int hop,
+                           int half_spectrum, int iterations, int pad_mode, unsigned int seed, float* d_out_real,
+                           float* d_out_imag, void* stream);
+
 /* additive. Batched STFT of `signals` signals of pcm_len samples each (signal s at
  * d_pcm + s*pcm_len), for a forward R2C half-spectrum context with N in {512, 1024,
  * 2048}: every signal is framed with the context's window/hop/pad rule (frames per

@@ -17,6 +17,7 @@
 //   window builder + normalisation    fft_cffi.cpp:163-208
 //   frame count + framing + output    fft_cffi.cpp:701-825
 //   complex-input batched inverse     fft_cffi.cpp:1042-1157 (output stage 1119-1157, 210-226)
+//   Griffin-Lim                       phase_infer.hpp:69-204, fft_cffi.cpp:2176-2248
 //   metadata writers (index maps)     eigen_fft.hpp:1529-1888 as called at 2672-2682,
 //                                     2981-3011, 3239-3244, 3478-3505
 // The reference's WorkerPool (eigen_fft.hpp:818-1419) is restated as a plain
@@ -33,6 +34,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -490,6 +492,74 @@ void ref_complex_output(int N, size_t frames, const float* cols_real, int window
   if (!norm.empty())
     for (size_t t = 0; t < total; ++t)
       if (norm[t] > 1e-12f) out_pcm[t] /= norm[t];
+}
+
+// The start angles of Griffin-Lim (phase_infer.hpp:124-129): std::mt19937 seeded with
+// `seed` (0x47d7bfad when 0) through std::uniform_real_distribution<float>(-pi, pi), in
+// the order the bins are visited. Built with the reference's flags.
+void ref_gl_angles(unsigned seed, size_t count, float* out) {
+  constexpr float kPi = 3.14159265358979323846f;
+  std::mt19937 rng(seed ? seed : 0x47d7bfadu);
+  std::uniform_real_distribution<float> dist(-kPi, kPi);
+  for (size_t i = 0; i < count; ++i) out[i] = dist(rng);
+}
+
+// Griffin-Lim (phase_infer.hpp:69-204) for a forward R2C context (kernel, window, hop,
+// context pad mode, analysis taps or NULL) and an inverse context (kernel, transform,
+// synthesis taps or NULL; no polar/magnitude hooks, no internal overlap-add), both of
+// size N with the same half_spectrum flag; hop is the overlap-add hop the C entry
+// resolves (fft_cffi.cpp:2214-2222). Returns 1, or 0 where the reference returns false.
+int ref_griffin_lim(int N, int half, const float* mag, size_t frames, int hop, int iterations, int pad_mode,
+                    unsigned seed, int inv_kernel, int inv_transform, const float* inv_synth, int fwd_kernel,
+                    int fwd_window, int fwd_hop, int fwd_ctx_pad, const float* fwd_taps, float* out_real,
+                    float* out_imag, int threads) {
+  if (!mag || !out_real || !out_imag || frames == 0 || N <= 0 || hop <= 0) return 0;
+  const size_t n = size_t(N);
+  const size_t bins = half ? n / 2 + 1 : n;
+  const size_t total = bins * frames;
+  const size_t ola_len = (frames - 1) * size_t(hop) + n;
+  const int iters = iterations > 0 ? iterations : 32;
+  std::vector<float> frame_pcm(frames * n), ola(ola_len), weight(ola_len), tr(total), ti(total), tm(total);
+  std::vector<float> ang(total);
+  ref_gl_angles(seed, total, ang.data());
+  for (size_t i = 0; i < total; ++i) {
+    out_real[i] = mag[i] * std::cos(ang[i]);
+    out_imag[i] = mag[i] * std::sin(ang[i]);
+  }
+  for (int it = 0; it < iters; ++it) {
+    const size_t produced = ref_execute_complex_batched(N, 1, inv_kernel, inv_transform, 0, 0, half, inv_synth,
+                                                        out_real, out_imag, frames, frame_pcm.data(), frames, threads);
+    if (produced == 0) return 0;
+    std::fill(ola.begin(), ola.end(), 0.0f);
+    std::fill(weight.begin(), weight.end(), 0.0f);
+    const size_t usable = std::min(produced, frames);
+    for (size_t f = 0; f < usable; ++f) {
+      const size_t start = f * size_t(hop);
+      if (start + n > ola_len) break;
+      for (size_t i = 0; i < n; ++i) {
+        ola[start + i] += frame_pcm[f * n + i];
+        weight[start + i] += 1.0f;
+      }
+    }
+    for (size_t t = 0; t < ola_len; ++t)
+      if (weight[t] > 0.0f) ola[t] /= weight[t];
+    const size_t fwd = ref_execute_batched(N, 0, fwd_kernel, fwd_window, fwd_hop, fwd_ctx_pad, 1, 0, 0, half,
+                                           fwd_taps, ola.data(), ola_len, tr.data(), ti.data(), tm.data(),
+                                           pad_mode, frames, threads);
+    if (fwd == 0) return 0;
+    const size_t uf = std::min(fwd, frames);
+    for (size_t i = 0; i < uf * bins; ++i) {
+      const float ph = std::atan2(ti[i], tr[i]);
+      out_real[i] = mag[i] * std::cos(ph);
+      out_imag[i] = mag[i] * std::sin(ph);
+    }
+    if (half)
+      for (size_t f = 0; f < uf; ++f) {
+        out_imag[f * bins] = 0.0f;
+        if ((n & 1) == 0 && bins > 1) out_imag[f * bins + bins - 1] = 0.0f;
+      }
+  }
+  return 1;
 }
 
 // Metadata of one column as the reference writers record it. kind follows

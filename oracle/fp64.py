@@ -371,3 +371,46 @@ def cola_norm64(frames, N, hop, weights):
         c = min(N, total - s)
         norm[s:s + c] += w[:c]
     return norm
+
+
+def griffin_lim64(mag, N, hop, angles, iterations, fwd_window=0, fwd_hop=0, pad=1, synth=None, analysis=None):
+    """FP64 Griffin-Lim (phase_infer.hpp:69-204) for an inverse C2R half-spectrum context
+    (synthesis taps or None) and a forward R2C half-spectrum context (window, hop, pad
+    rule, analysis taps or None), from given start angles (frames x bins). Per iteration:
+    x_f = IDFT_N of the Hermitian rebuild (inverse scale 1/N, imaginary parts of bins 0
+    and N/2 dropped) times the synthesis taps; overlap-add averaged over the frames per
+    sample; forward STFT; X = mag * (cos, sin)(arg X) for the frames the forward makes,
+    imaginary parts of bins 0 and N/2 set to 0. Returns the final spectrum (complex128)."""
+    mag = np.asarray(mag, dtype=np.float64)
+    frames, bins = mag.shape
+    W = fwd_window or N
+    Hf = fwd_hop or hop
+    X = mag * np.exp(1j * np.asarray(angles, dtype=np.float64).reshape(frames, bins))
+    L = (frames - 1) * hop + N
+    cnt = np.zeros(L)
+    for f in range(frames):
+        cnt[f * hop:f * hop + N] += 1
+    syn = None if synth is None else np.asarray(synth, dtype=np.float64)
+    ana = np.ones(N) if analysis is None else np.asarray(analysis, dtype=np.float64)
+    fwd_frames = min(frame_count(L, N, W, Hf, pad, pad, frames), frames)
+    for _ in range(iterations):
+        fr = np.fft.irfft(X, n=N, axis=-1)
+        if syn is not None:
+            fr = fr * syn
+        ola = np.zeros(L)
+        for f in range(frames):
+            ola[f * hop:f * hop + N] += fr[f]
+        ola = np.where(cnt > 0, ola / np.where(cnt > 0, cnt, 1), ola)
+        seg = np.zeros((fwd_frames, N))
+        for f in range(fwd_frames):
+            s = f * Hf
+            c = min(W, L - s)
+            seg[f, :c] = ola[s:s + c] * ana[:c]
+        Y = np.fft.rfft(seg, axis=-1)
+        Xn = mag[:fwd_frames] * np.exp(1j * np.angle(Y))
+        Xn[:, 0] = Xn[:, 0].real
+        if N % 2 == 0:
+            Xn[:, -1] = Xn[:, -1].real
+        X = X.copy()
+        X[:fwd_frames] = Xn
+    return X

@@ -48,6 +48,12 @@ def lib():
         L.ref_complex_output.argtypes = [ctypes.c_int, _sz, _f32p, ctypes.c_int, _f32p, _f32p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, _f32p]
         L.ref_complex_output.restype = None
+        L.ref_gl_angles.argtypes = [ctypes.c_uint, _sz, _f32p]
+        L.ref_gl_angles.restype = None
+        L.ref_griffin_lim.argtypes = ([ctypes.c_int, ctypes.c_int, _f32p, _sz, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int])
+        L.ref_griffin_lim.restype = ctypes.c_int
         L.ref_metadata.argtypes = [ctypes.c_int] * 6 + [_f32p, _sz]
         L.ref_metadata.restype = _sz
         _lib = L
@@ -136,6 +142,28 @@ def execute_complex_batched(in_real, in_imag, N, inverse=True, kernel=0, transfo
     lib().ref_complex_output(N, got, _p(cols[:got]), int(bool(windows_enabled)), _p(ana), _p(syn), int(bool(apply_ola)),
                              int(hop), int(cola_mode), _p(out))
     return int(got), (out.reshape(-1) if apply_ola else out)
+
+
+def gl_angles(seed: int, count: int) -> np.ndarray:
+    """The reference's Griffin-Lim start angles (std::mt19937 + uniform_real_distribution<float>)."""
+    out = np.zeros(count, np.float32)
+    lib().ref_gl_angles(seed, count, _p(out))
+    return out
+
+
+def griffin_lim(mag, N, hop, iterations=32, pad_mode=1, seed=0, half=True, inv_kernel=0, inv_transform=2,
+                inv_synth=None, fwd_kernel=0, fwd_window=0, fwd_hop=0, fwd_ctx_pad=1, fwd_taps=None, threads=None):
+    """Reference Griffin-Lim (phase_infer.hpp:69-204): returns (ok, out_real, out_imag), (frames, bins)."""
+    mag = np.ascontiguousarray(mag, dtype=np.float32)
+    frames = mag.shape[0]
+    re = np.zeros_like(mag)
+    im = np.zeros_like(mag)
+    syn = None if inv_synth is None else np.ascontiguousarray(inv_synth, dtype=np.float32)
+    taps = None if fwd_taps is None else np.ascontiguousarray(fwd_taps, dtype=np.float32)
+    ok = lib().ref_griffin_lim(N, int(half), _p(mag), frames, hop, iterations, pad_mode, seed, inv_kernel,
+                               inv_transform, _p(syn), fwd_kernel, fwd_window or N, fwd_hop or hop, fwd_ctx_pad,
+                               _p(taps), _p(re), _p(im), threads or default_threads())
+    return int(ok), re, im
 
 
 def metadata(N: int, kind: int, kernel: int = 0, inverse=False, transform=0, half_spectrum=False) -> np.ndarray:

@@ -414,3 +414,90 @@ def test_cpu_ref_complex_batched_vs_fp64(kernel, mode):
         norm = fp64.cola_norm64(frames, N, H, (w.astype(np.float64) ** 2) if windows else np.ones(N))
         m = norm > 1e-3 * norm.max()
     assert fp64.rel_l2(out[m], exp[m]) <= fp64.tolerance(N)
+
+
+# ---- Griffin-Lim ----------------------------------------------------------------------
+
+def _gl_problem(N=256, H=64, periods=40, seed=3):
+    from fftfree_amd.workloads import noise_plus_sines
+    x = noise_plus_sines(N * periods, seed=seed)
+    frames = fp64.frame_count(x.size, N, N, H, 1, 1)
+    _, _, mag = fp64.stft_planes(x, N, N, H, frames, None, False, fp64.R2C, False, False, True)
+    return mag.astype(np.float32), frames
+
+
+def _gl_inconsistency(X, mag, N, H):
+    """|| |STFT(averaged OLA of IDFT(X))| - mag || / ||mag|| (spectral convergence)."""
+    frames = mag.shape[0]
+    fr = np.fft.irfft(np.asarray(X, np.complex128), n=N, axis=-1)
+    L = (frames - 1) * H + N
+    ola, w = np.zeros(L), np.zeros(L)
+    for f in range(frames):
+        ola[f * H:f * H + N] += fr[f]
+        w[f * H:f * H + N] += 1
+    ola /= w
+    Y = np.abs(np.fft.rfft(np.stack([ola[f * H:f * H + N] for f in range(frames)]), axis=-1))
+    return np.linalg.norm(Y - mag) / np.linalg.norm(mag)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 12345])
+def test_gl_start_angles_are_the_fused_uniform(seed):
+    """The start angles of the reference (std::uniform_real_distribution<float> over
+    std::mt19937, built with the reference's -ffast-math -march flags) equal
+    fma(u, 2 pi, -pi) with u = float(word) / 2^32, on the raw mt19937 words (numpy's
+    MT19937 with the same init_genrand seeding). The GPU start kernel computes that
+    formula from the same words; an unfused multiply-add would differ in half the
+    draws."""
+    n = 50000
+    a = ref.gl_angles(seed, n)
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed if seed else 0x47d7bfad)
+    w = bg.random_raw(n).astype(np.uint64)
+    u = (w.astype(np.float32) * np.float32(2.0 ** -32)).astype(np.float32)
+    u = np.where(u >= 1, np.nextafter(np.float32(1), np.float32(0)), u)
+    kpi = np.float32(np.pi)
+    fused = (u.astype(np.float64) * np.float64(kpi - (-kpi)) + np.float64(-kpi)).astype(np.float32)
+    assert np.array_equal(fused, a)
+    unfused = (u * (kpi - (-kpi))).astype(np.float32) + (-kpi)
+    assert np.sum(unfused != a) > n // 10
+
+
+@pytest.mark.parametrize("iterations", [1, 8, 32])
+def test_gl_two_restatements_agree(iterations):
+    """The float restatement (cpu_ref, the reference loop over the restated inverse and
+    forward paths) and the FP64 one (numpy) agree within 2*tol(N)*sqrt(iterations) from
+    the same start angles."""
+    N, H = 256, 64
+    mag, frames = _gl_problem(N, H)
+    ok, r, i = ref.griffin_lim(mag, N, H, iterations=iterations, seed=7)
+    assert ok == 1
+    X64 = fp64.griffin_lim64(mag, N, H, ref.gl_angles(7, mag.size).reshape(mag.shape), iterations)
+    assert fp64.rel_l2(r + 1j * i, X64) <= 2 * fp64.tolerance(N) * max(1.0, np.sqrt(iterations))
+
+
+def test_gl_agreement_check_has_teeth():
+    """A planted mistake in the FP64 restatement (the overlap-add hop off by one) breaks
+    the agreement."""
+    N, H = 256, 64
+    mag, _ = _gl_problem(N, H)
+    ok, r, i = ref.griffin_lim(mag, N, H, iterations=4, seed=7)
+    ang = ref.gl_angles(7, mag.size).reshape(mag.shape)
+    bad = fp64.griffin_lim64(mag, N, H + 1, ang, 4, fwd_hop=H)
+    assert fp64.rel_l2(r + 1j * i, bad) > 1e-2
+
+
+def test_gl_oracle_properties():
+    """|X| stays the given magnitude, bins 0 and N/2 are real, and the spectral
+    convergence of the iterates does not increase (Griffin-Lim's monotonicity)."""
+    N, H = 256, 64
+    mag, _ = _gl_problem(N, H)
+    prev = np.inf
+    for it in [1, 4, 16, 64]:
+        ok, r, i = ref.griffin_lim(mag, N, H, iterations=it, seed=11)
+        assert ok == 1
+        assert np.max(np.abs(np.hypot(r, i) - mag)) <= 1e-6 * np.max(mag)
+        assert not i[:, 0].any() and not i[:, -1].any()
+        sc = _gl_inconsistency(r + 1j * i, mag, N, H)
+        assert sc < prev
+        prev = sc
+    assert prev < 0.2
