@@ -22,7 +22,11 @@ fft_execute_batched: PCIe transfers of PCM and planes included, wall clock); ist
 inverse of the default workload through fft_execute_complex_batched_device: 65533 frames x
 513 complex bins -> Hermitian rebuild, 1024-point inverse, Hann synthesis window,
 overlap-add with COLA normalisation into 65533*1024 output samples, zeros past the
-overlap-add included, as the reference writes them); copy
+overlap-add included, as the reference writes them); griffin_lim (fft_griffin_lim_device
+on the magnitude spectrogram of 2^20 samples, N = 1024, hop 256, 4093 frames, 32
+iterations; value = iterations/s from the 32-iteration call minus the 1-iteration call,
+each call synchronised);
+copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
 WRITE_SIZE counters).
 """
@@ -45,8 +49,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "c2c1024", "c2c_large", "stft_grad", "ragged",
-                                                     "copy"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "c2c1024", "c2c_large",
+                                                     "stft_grad", "ragged", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -199,6 +203,69 @@ def main():
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("istft_n1024"),
                              "algorithmic_bytes_per_launch": alg_bytes, "kernel": "istft_kernel"}}
+    elif args.workload == "griffin_lim":
+        n, H, L, iters = 1024, 256, 1 << 20, 32
+        fwd = ff.make_plan(n, window=n, hop=H, kernel=0, transform=1, half_spectrum=True, pad_mode=1)
+        inv = ff.make_plan(n, window=n, hop=H, inverse=1, kernel=0, transform=2, half_spectrum=True, pad_mode=1)
+        pcm = torch.from_numpy(noise_plus_sines(L, seed=2)).cuda()
+        frames = ff.frame_count(fwd, L, 1)
+        bins = n // 2 + 1
+        mag = torch.empty((frames, bins), device="cuda")
+        ff.stft_device(ff.make_plan(n, window=n, hop=H, transform=1, reduce_magnitude=True, half_spectrum=True,
+                                    pad_mode=1), pcm, mag, None, None, pad_mode=1)
+        out_re, out_im = torch.empty_like(mag), torch.empty_like(mag)
+
+        def call(k):
+            return lambda: ff.griffin_lim_device(fwd, inv, mag, out_re, out_im, hop=H, iterations=k, seed=1)
+
+        steps = max(1, args.steps // 20)
+
+        def synced(fn, k):
+            # each call synchronised: the host part of one call (the start's mt19937 words)
+            # must not overlap the previous call's kernels
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+                torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        t_full = synced(call(iters), steps)
+        t_one = synced(call(1), steps)
+        per_iter = (t_full - t_one) / steps / (iters - 1)
+        ola_len = (frames - 1) * H + n
+        # per iteration: inverse kernel reads |X| and the two forward planes and writes the
+        # signal; the forward STFT reads the signal and writes two planes
+        alg_bytes = frames * bins * 4 * 5 + ola_len * 4 * 2
+        achieved = alg_bytes / per_iter / 1e9
+        line = {"metric": "Griffin-Lim iterations/s (fft_griffin_lim_device), N=1024 hop 256, 4093 frames",
+                "value": 1.0 / per_iter, "unit": "iterations/s", "n_gpus": 1, "steps": steps,
+                "warmup": 1, "ms_per_step": t_full / steps * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic: |STFT| of 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
+                "config": {"workload": "Griffin-Lim, inverse C2R + forward R2C half spectrum, no windows, pad 1, "
+                                       f"{iters} iterations per call (ms_per_step), start included",
+                           "n": n, "hop": H, "frames": frames, "iterations": iters,
+                           "ms_per_iteration": per_iter * 1e3, "ms_one_iteration_call": t_one / steps * 1e3},
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                             "algorithmic_bytes_per_launch": alg_bytes,
+                             "kernel": "one iteration: istft_kernel (Griffin-Lim mode) + stft_r2c_mf_kernel"}}
+        if not args.no_cpu:
+            from oracle import ref
+            mag_h = mag.cpu().numpy()
+            t0 = time.perf_counter()
+            done = 0
+            while True:
+                ref.griffin_lim(mag_h, n, H, iterations=2, seed=1)
+                done += 2
+                el = time.perf_counter() - t0
+                if el >= args.cpu_seconds:
+                    break
+            line["cpu_baseline"] = {"value": done / el, "unit": "iterations/s", "cores": ref.default_threads(),
+                                    "kind": "port", "sample": f"{done} iterations of the restated reference loop "
+                                                              f"on the same spectrogram, {el:.1f} s"}
     elif args.workload == "c2c_large":
         n, B = 1 << args.lg, 16
         fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)

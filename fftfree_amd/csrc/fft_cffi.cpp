@@ -18,13 +18,13 @@
 #include <cstring>
 #include <map>
 #include <memory>
-#include <random>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/fft_cffi.h"
 #include "fft_kernels.hpp"
+#include "mt19937.hpp"
 #include "plan.hpp"
 
 namespace fftfree {
@@ -97,6 +97,9 @@ struct FftContext {
   DeviceBuffer d_sink;  // N floats that kernels write skipped output planes to
   DeviceBuffer d_halo;  // adjoint run-boundary partial sums
   DeviceBuffer d_gl;    // Griffin-Lim working set (on the inverse context)
+  uint32_t* h_words = nullptr;       // Griffin-Lim start words (pinned host memory)
+  size_t h_words_cap = 0;
+  hipEvent_t words_copied = nullptr;  // the last copy out of h_words
   bool windows_dirty = true;
 
   ~FftContext() {
@@ -108,6 +111,11 @@ struct FftContext {
     d_sink.release();
     d_halo.release();
     d_gl.release();
+    if (words_copied) {
+      (void)hipEventSynchronize(words_copied);
+      (void)hipEventDestroy(words_copied);
+    }
+    if (h_words) (void)hipHostFree(h_words);
     plan.reset();
     cache.clear();
     if (stream) {
@@ -1021,14 +1029,28 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
   float* d_tim = reinterpret_cast<float*>(base + o_tim);
   float* d_fr = reinterpret_cast<float*>(base + o_fr);
 
-  // the start: mt19937 is sequential, so its words come from the host (phase_infer.hpp:124-134)
+  // the start: mt19937 is sequential, so its words come from the host (phase_infer.hpp:124-134),
+  // into a pinned buffer of the context, copied in chunks while the next chunk is drawn
   {
-    std::mt19937 rng(seed ? seed : 0x47d7bfadu);
-    std::vector<uint32_t> w(total);
-    for (size_t i = 0; i < total; ++i) w[i] = (uint32_t)rng();
-    hip_check(hipMemcpyAsync(d_words, w.data(), total * 4, hipMemcpyHostToDevice, s), "H2D words");
+    if (iv->words_copied) hip_check(hipEventSynchronize(iv->words_copied), "wait for the last start copy");
+    else hip_check(hipEventCreateWithFlags(&iv->words_copied, hipEventDisableTiming), "hipEventCreate");
+    if (iv->h_words_cap < total) {
+      if (iv->h_words) hip_check(hipHostFree(iv->h_words), "hipHostFree");
+      iv->h_words = nullptr;
+      iv->h_words_cap = 0;
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&iv->h_words), total * 4, hipHostMallocDefault),
+                "hipHostMalloc");
+      iv->h_words_cap = total;
+    }
+    Mt19937Block rng(seed ? seed : 0x47d7bfadu);  // the values of std::mt19937
+    constexpr size_t kChunk = size_t(1) << 18;
+    for (size_t b = 0; b < total; b += kChunk) {
+      const size_t e = std::min(total, b + kChunk);
+      rng.fill(iv->h_words + b, e - b);
+      hip_check(hipMemcpyAsync(d_words + b, iv->h_words + b, (e - b) * 4, hipMemcpyHostToDevice, s), "H2D words");
+    }
+    hip_check(hipEventRecord(iv->words_copied, s), "hipEventRecord");
     hip_check(launch_gl_start(d_mag, d_words, total, d_re, d_im, s), "launch_gl_start");
-    hip_check(hipStreamSynchronize(s), "sync");  // w goes out of scope
   }
   const StftArgs fa = stft_args(fw, d_ola, ola_len);
   for (int it = 0; it < iters; ++it) {

@@ -551,7 +551,7 @@ hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
   const size_t positions = MODE == kFrameMajor ? a.frames : (a.ola_len + a.hop - 1) / a.hop;
   const size_t slots = (capacity * kIstftFill + 99) / 100;
   size_t steps = (positions + kFps * slots - 1) / (kFps * slots);
-  const size_t lo = MODE == kFrameMajor ? 1 : (size_t)(2 * a.warm > 4 ? 2 * a.warm : 4);
+  const size_t lo = MODE == kFrameMajor ? 1 : (size_t)(a.warm > 1 ? a.warm : 1);
   steps = steps < lo ? lo : (steps > 256 ? 256 : steps);
   a.steps = (int)steps;
   const size_t grid = (positions + kFps * steps - 1) / (kFps * steps);
