@@ -58,6 +58,12 @@ SIGNATURES = {
     "fft_execute_c2c_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fft_execute_complex_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
                                                       c_void_p]),
+    "fft_stream_reset": (None, [c_void_p]),
+    "fft_stream_push_pcm": (c_size_t, [c_void_p, f32p, c_size_t, f32p, f32p, f32p, c_size_t, c_int]),
+    "fft_stream_pending_frames": (c_size_t, [c_void_p]),
+    "fft_stream_backlog_samples": (c_size_t, [c_void_p]),
+    "fft_stream_push_spectrum": (c_size_t, [c_void_p, f32p, f32p, c_size_t, f32p, c_size_t, c_int]),
+    "fft_stream_pending_pcm": (c_size_t, [c_void_p]),
     "fft_griffin_lim": (c_int, [c_void_p, c_void_p, f32p, c_size_t, c_int, c_int, c_int, c_int, c_uint, f32p, f32p]),
     "fft_griffin_lim_device": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_int, c_int, c_int, c_int, c_uint,
                                        c_void_p, c_void_p, c_void_p]),

@@ -84,6 +84,13 @@ struct FftContext {
   std::vector<float> synthesis_win;
   bool windows_enabled = false;
   int apply_ola = 0;
+  // streaming state (fft_cffi.cpp:1342-1742): PCM not yet consumed, and the inverse
+  // direction's overlap-add accumulator and queue of finished samples
+  std::vector<float> stream_pcm;
+  size_t stream_offset = 0;
+  std::vector<float> stream_queue;
+  size_t stream_queue_offset = 0;
+  std::vector<float> stream_accum, stream_norm;
   PlanRuntimeConfig cfg;
   PlanCache cache;
   std::shared_ptr<Plan> plan;
@@ -1203,6 +1210,195 @@ size_t fft_execute_complex_batched(void* handle, const float* in_real, const flo
     return frames;
   } catch (const std::exception& e) {
     fftfree::report("fft_execute_complex_batched", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+// ---- streaming (fft_cffi.cpp:1342-1742) -----------------------------------------------
+// The sliding buffers live on the host, as the reference keeps them; every transform
+// runs through the batched entry points above (one launch per call).
+namespace fftfree {
+namespace {
+
+int stream_window(const FftContext* c) { return c->window > 0 ? c->window : c->N; }
+int stream_hop(const FftContext* c) { return c->hop > 0 ? c->hop : stream_window(c); }
+
+size_t stream_backlog(const FftContext* c) {
+  return c->stream_offset >= c->stream_pcm.size() ? 0 : c->stream_pcm.size() - c->stream_offset;
+}
+
+size_t stream_frames(const FftContext* c) {
+  const size_t backlog = stream_backlog(c);
+  const int W = stream_window(c), H = stream_hop(c);
+  if (W <= 0 || H <= 0 || backlog < (size_t)W) return 0;
+  return 1 + (backlog - (size_t)W) / (size_t)H;
+}
+
+void stream_compact(FftContext* c) {
+  if (c->stream_offset == 0) return;
+  if (c->stream_offset >= c->stream_pcm.size()) {
+    c->stream_pcm.clear();
+  } else {
+    c->stream_pcm.erase(c->stream_pcm.begin(), c->stream_pcm.begin() + (std::ptrdiff_t)c->stream_offset);
+  }
+  c->stream_offset = 0;
+}
+
+size_t stream_pending(const FftContext* c) {
+  return c->stream_queue_offset >= c->stream_queue.size() ? 0 : c->stream_queue.size() - c->stream_queue_offset;
+}
+
+// fft_cffi.cpp:1420-1442: up to max_samples (0 = all) finished samples to out_pcm.
+size_t stream_emit(FftContext* c, float* out_pcm, size_t max_samples) {
+  size_t take = stream_pending(c);
+  if (max_samples != 0 && take > max_samples) take = max_samples;
+  if (!out_pcm || take == 0) return 0;
+  std::copy_n(c->stream_queue.data() + c->stream_queue_offset, take, out_pcm);
+  c->stream_queue_offset += take;
+  if (c->stream_queue_offset >= c->stream_queue.size()) {
+    c->stream_queue.clear();
+  } else {
+    c->stream_queue.erase(c->stream_queue.begin(), c->stream_queue.begin() + (std::ptrdiff_t)c->stream_queue_offset);
+  }
+  c->stream_queue_offset = 0;
+  return take;
+}
+
+// cola_window_weight (fft_cffi.cpp:210-226).
+float stream_cola_weight(const FftContext* c, size_t i) {
+  if (!c->windows_enabled) return 1.0f;
+  const float a = i < c->analysis_win.size() ? c->analysis_win[i] : 1.0f;
+  const float b = i < c->synthesis_win.size() ? c->synthesis_win[i] : 1.0f;
+  return a * b;
+}
+
+}  // namespace
+}  // namespace fftfree
+
+void fft_stream_reset(void* handle) {
+  if (!handle) return;
+  auto* c = static_cast<FftContext*>(handle);
+  c->stream_pcm.clear();
+  c->stream_offset = 0;
+  c->stream_queue.clear();
+  c->stream_queue_offset = 0;
+  c->stream_accum.clear();
+  c->stream_norm.clear();
+}
+
+size_t fft_stream_pending_frames(void* handle) {
+  return handle ? fftfree::stream_frames(static_cast<FftContext*>(handle)) : 0;
+}
+
+size_t fft_stream_backlog_samples(void* handle) {
+  return handle ? fftfree::stream_backlog(static_cast<FftContext*>(handle)) : 0;
+}
+
+size_t fft_stream_pending_pcm(void* handle) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  return c->inverse ? fftfree::stream_pending(c) : 0;
+}
+
+size_t fft_stream_push_pcm(void* handle, const float* pcm, size_t samples, float* out_real, float* out_imag,
+                           float* out_mag, size_t max_frames, int flush_mode) {
+  (void)flush_mode;  // not read by the reference either
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  try {
+    if (samples > 0 && pcm) c->stream_pcm.insert(c->stream_pcm.end(), pcm, pcm + samples);
+    const int W = fftfree::stream_window(c), H = fftfree::stream_hop(c);
+    if (fftfree::verbose())
+      std::fprintf(stderr, "[diag] fft_stream_push_pcm samples=%zu W=%d H=%d offset=%zu buffered=%zu\n", samples, W,
+                   H, c->stream_offset, c->stream_pcm.size());
+    if (W <= 0 || H <= 0) return 0;
+    size_t frames = fftfree::stream_frames(c);
+    if (frames == 0) return 0;
+    if (max_frames != 0 && frames > max_frames) frames = max_frames;
+    const size_t backlog = fftfree::stream_backlog(c);
+    size_t len = (size_t)W + (frames - 1) * (size_t)H;
+    if (len > backlog) len = backlog;
+    // the frames of the next len samples (fft_cffi.cpp:1553-1590: the chunk the reference
+    // copies is exactly this stretch of the buffer), partial frames dropped (pad_mode 2)
+    const size_t bins = fftfree::bins_of(c);
+    std::vector<float> local;
+    if (!out_real || !out_imag || !out_mag) local.assign(3 * frames * bins, 0.0f);
+    float* re = out_real ? out_real : local.data();
+    float* im = out_imag ? out_imag : local.data() + frames * bins;
+    float* mg = out_mag ? out_mag : local.data() + 2 * frames * bins;
+    const size_t produced =
+        fft_execute_batched(handle, c->stream_pcm.data() + c->stream_offset, len, re, im, mg, 2, 0, frames);
+    if (produced == 0) return 0;
+    c->stream_offset += produced * (size_t)H;
+    fftfree::stream_compact(c);
+    return produced;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stream_push_pcm", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t fft_stream_push_spectrum(void* handle, const float* in_real, const float* in_imag, size_t frames,
+                                float* out_pcm, size_t max_samples, int flush_mode) {
+  if (!handle) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (!c->inverse) return 0;
+  try {
+    const size_t n = (size_t)std::max(c->N, 1);
+    if (c->stream_accum.size() != n) c->stream_accum.assign(n, 0.0f);
+    const bool use_norm = c->cola_mode == FFT_COLA_NORMALIZE;
+    if (use_norm) {
+      if (c->stream_norm.size() != n) c->stream_norm.assign(n, 0.0f);
+    } else {
+      c->stream_norm.clear();
+    }
+    if (frames > 0) {
+      if (!in_real) return fftfree::stream_emit(c, out_pcm, max_samples);
+      std::vector<float> fr(frames * n, 0.0f);
+      size_t produced = in_imag ? fft_execute_complex_batched(handle, in_real, in_imag, frames, fr.data(), 2, 0, frames)
+                                : 0;  // the reference's complex entry refuses a NULL imag plane
+      if (produced == 0) return fftfree::stream_emit(c, out_pcm, max_samples);
+      if (produced > frames) produced = frames;
+      const size_t hop = (size_t)std::max(fftfree::stream_hop(c), 1);
+      for (size_t f = 0; f < produced; ++f) {
+        const float* x = fr.data() + f * n;
+        for (size_t i = 0; i < n; ++i) {
+          c->stream_accum[i] += x[i];
+          if (use_norm) c->stream_norm[i] += fftfree::stream_cola_weight(c, i);
+        }
+        size_t emit = (flush_mode == FFT_STREAM_FLUSH_FINAL && f + 1 == produced) ? n : hop;
+        if (emit > n) emit = n;
+        for (size_t i = 0; i < emit; ++i) {
+          float v = c->stream_accum[i];
+          if (use_norm && std::fabs(c->stream_norm[i]) > 1e-12f) v /= c->stream_norm[i];
+          c->stream_queue.push_back(v);
+        }
+        // shift out the emitted samples (all of them when emit == n)
+        std::move(c->stream_accum.begin() + (std::ptrdiff_t)emit, c->stream_accum.end(), c->stream_accum.begin());
+        std::fill(c->stream_accum.end() - (std::ptrdiff_t)emit, c->stream_accum.end(), 0.0f);
+        if (use_norm) {
+          std::move(c->stream_norm.begin() + (std::ptrdiff_t)emit, c->stream_norm.end(), c->stream_norm.begin());
+          std::fill(c->stream_norm.end() - (std::ptrdiff_t)emit, c->stream_norm.end(), 0.0f);
+        }
+      }
+    }
+    if (flush_mode == FFT_STREAM_FLUSH_FINAL && frames == 0 && !c->stream_accum.empty()) {
+      const bool norm_local = !c->stream_norm.empty();
+      for (size_t i = 0; i < c->stream_accum.size(); ++i) {
+        float v = c->stream_accum[i];
+        if (norm_local && std::fabs(c->stream_norm[i]) > 1e-12f) v /= c->stream_norm[i];
+        c->stream_queue.push_back(v);
+      }
+      std::fill(c->stream_accum.begin(), c->stream_accum.end(), 0.0f);
+      if (norm_local) std::fill(c->stream_norm.begin(), c->stream_norm.end(), 0.0f);
+    }
+    return fftfree::stream_emit(c, out_pcm, max_samples);
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stream_push_spectrum", e.what());
     return 0;
   } catch (...) {
     return 0;

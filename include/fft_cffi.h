@@ -150,6 +150,27 @@ size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_
 size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
                                           size_t frames, float* d_out_pcm, size_t max_frames, void* stream);
 
+/* reference: fft_cffi.hpp:188-211 (fft_cffi.cpp:1342-1742). Streaming STFT on a context.
+ * fft_stream_push_pcm appends `samples` to the context's buffer and writes the frames that
+ * are now complete (at most max_frames, 0 = all; hop-aligned from the stream start,
+ * partial frames kept for later) to the planes (host pointers, frames*bins floats each;
+ * NULL planes are skipped); returns the frame count. flush_mode is accepted and, as in the
+ * reference, not used. fft_stream_push_spectrum (inverse contexts) transforms `frames`
+ * frames of bins, overlap-adds them into the context's accumulator (COLA-normalised when
+ * the context's cola_mode is FFT_COLA_NORMALIZE), queues `hop` finished samples per frame
+ * (all N of the last frame, or the whole accumulator when frames == 0, with
+ * FFT_STREAM_FLUSH_FINAL), and writes up to max_samples (0 = all) queued samples to
+ * out_pcm; returns the sample count written. The reference prints a line to stderr on
+ * every fft_stream_push_pcm call; here that line needs FFTFREE_VERBOSE. */
+void fft_stream_reset(void* handle);
+size_t fft_stream_push_pcm(void* handle, const float* pcm, size_t samples, float* out_real, float* out_imag,
+                           float* out_mag, size_t max_frames, int flush_mode);
+size_t fft_stream_pending_frames(void* handle);
+size_t fft_stream_backlog_samples(void* handle);
+size_t fft_stream_push_spectrum(void* handle, const float* in_real, const float* in_imag, size_t frames,
+                                float* out_pcm, size_t max_samples, int flush_mode);
+size_t fft_stream_pending_pcm(void* handle);
+
 /* reference: fft_cffi.hpp:213-223 (fft_cffi.cpp:2176-2248, phase_infer.hpp:69-204).
  * Griffin-Lim phase reconstruction of a frame-major magnitude spectrogram (frames x
  * bins): a random start (mt19937 seeded with `seed`, 0x47d7bfad when 0), then

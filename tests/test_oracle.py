@@ -501,3 +501,58 @@ def test_gl_oracle_properties():
         assert sc < prev
         prev = sc
     assert prev < 0.2
+
+
+# ---- streaming ------------------------------------------------------------------------
+
+def test_stream_oracle_forward_equals_batch():
+    """Pushing a signal in uneven chunks yields, in order, exactly the frames the batched
+    call makes of the whole signal (partial frames dropped): the buffering neither
+    skips nor repeats samples."""
+    from oracle.stream import Stream
+    N, H = 256, 64
+    x = np.random.default_rng(1).standard_normal(N * 30).astype(np.float32)
+    st = Stream(N, window=N, hop=H, transform=fp64.R2C)
+    got_re = []
+    pos, rng = 0, np.random.default_rng(2)
+    while pos < x.size:
+        k = int(rng.integers(1, 700))
+        n, planes = st.push_pcm(x[pos:pos + k])
+        pos += k
+        if n:
+            got_re.append(planes[0])
+    got = np.concatenate(got_re)
+    full, re, _, _ = ref.execute_batched(x, N, N, H, ctx_pad=0, pad_mode=2, transform=fp64.R2C, half_spectrum=True)
+    assert got.shape == re.shape and np.array_equal(got, re)
+    assert st.backlog() == x.size - full * H
+
+
+@pytest.mark.parametrize("cola", [0, 1])
+def test_stream_oracle_inverse_equals_batch_overlap_add(cola):
+    """Frames pushed in uneven groups, then a final flush: the samples that come out are
+    the batched overlap-add (with the same COLA division) over (F-1)*H + N samples,
+    followed by the H zeros the flush of the whole N-sample accumulator adds
+    (fft_cffi.cpp:1717-1736)."""
+    from oracle.stream import Stream, FLUSH_FINAL
+    N, H, F = 256, 64, 37
+    w = fp64.build_window(fp64.WINDOW_HANN, N)
+    rng = np.random.default_rng(3)
+    re = rng.standard_normal((F, N // 2 + 1)).astype(np.float32)
+    im = rng.standard_normal((F, N // 2 + 1)).astype(np.float32)
+    st = Stream(N, window=N, hop=H, inverse=True, transform=fp64.C2R, analysis=w, synthesis=w,
+                windows_enabled=True, cola_mode=cola)
+    outs, f = [], 0
+    while f < F:
+        k = int(rng.integers(1, 6))
+        outs.append(st.push_spectrum(re[f:f + k], im[f:f + k]))
+        f += k
+    outs.append(st.push_spectrum(None, None, flush=FLUSH_FINAL))
+    got = np.concatenate(outs)
+    L = (F - 1) * H + N
+    _, exp = ref.execute_complex_batched(re, im, N, synth_taps=w, analysis_taps=w, apply_ola=True, hop=H,
+                                         cola_mode=cola)
+    assert got.size == L + H
+    # the batched path fuses out += x*w into one rounding under the reference's flags, the
+    # streaming path rounds the windowed frame first: a few ulps apart
+    assert np.max(np.abs(got[:L] - exp[:L])) <= 4 * np.finfo(np.float32).eps * np.max(np.abs(exp))
+    assert not got[L:].any()
