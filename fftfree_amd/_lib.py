@@ -58,6 +58,10 @@ SIGNATURES = {
     "fft_execute_c2c_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "fft_execute_complex_batched_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t,
                                                       c_void_p]),
+    "phase_init": (c_void_p, [c_int, c_int, c_int, c_int, c_int]),
+    "phase_infer_execute": (c_size_t, [c_void_p, f32p, c_size_t, f32p, f32p]),
+    "phase_free": (None, [c_void_p]),
+    "phase_infer_execute_device": (c_size_t, [c_void_p, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
     "fft_stream_reset": (None, [c_void_p]),
     "fft_stream_push_pcm": (c_size_t, [c_void_p, f32p, c_size_t, f32p, f32p, f32p, c_size_t, c_int]),
     "fft_stream_pending_frames": (c_size_t, [c_void_p]),

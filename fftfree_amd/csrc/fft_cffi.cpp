@@ -1216,6 +1216,118 @@ size_t fft_execute_complex_batched(void* handle, const float* in_real, const flo
   }
 }
 
+// ---- phase inference (fft_cffi.cpp:2058-2174) -------------------------------------------
+namespace fftfree {
+namespace {
+
+// phase_init's context (fft_cffi.cpp:2060-2080): the configuration, and for mode 1 two
+// complex contexts of size N (inverse and forward) whose transforms the cepstrum runs on.
+struct PhaseCtx {
+  int n = 0, hop = 0, half = 1, mode = 0, iterations = 0;
+  FftContext* inv = nullptr;
+  FftContext* fwd = nullptr;
+  DeviceBuffer scratch;  // mode 0: phase checkpoints; mode 1: frames x N complex
+  DeviceBuffer staging;  // host entry: magnitude and output planes
+  ~PhaseCtx() {
+    scratch.release();
+    staging.release();
+    if (inv) fft_free(inv);
+    if (fwd) fft_free(fwd);
+  }
+};
+
+size_t phase_execute(PhaseCtx* c, const float* d_mag, size_t frames, float* d_re, float* d_im, hipStream_t s) {
+  const int N = c->n;
+  const int bins = c->half ? N / 2 + 1 : N;
+  if (c->mode == 0) {
+    const size_t words = phase_linear_scratch(frames, bins);
+    c->scratch.reserve(words * sizeof(double));
+    hip_check(launch_phase_linear(d_mag, frames, bins, N, c->hop, c->scratch.as<double>(), d_re, d_im, s),
+              "launch_phase_linear");
+    return frames;
+  }
+  if (c->mode != 1 || !c->inv || !c->fwd) return 0;  // unknown mode, or N with no plan
+  c->scratch.reserve(frames * (size_t)N * sizeof(float2));
+  float2* cols = c->scratch.as<float2>();
+  hip_check(launch_phase_log(d_mag, frames, bins, N, c->half, cols, s), "launch_phase_log");
+  if (fft_execute_c2c_batched_device(c->inv, reinterpret_cast<float*>(cols), reinterpret_cast<float*>(cols), frames,
+                                     s) != frames)
+    return 0;
+  hip_check(launch_phase_fold(frames, N, cols, s), "launch_phase_fold");
+  if (fft_execute_c2c_batched_device(c->fwd, reinterpret_cast<float*>(cols), reinterpret_cast<float*>(cols), frames,
+                                     s) != frames)
+    return 0;
+  hip_check(launch_phase_exp(cols, frames, bins, N, d_re, d_im, s), "launch_phase_exp");
+  return frames;
+}
+
+}  // namespace
+}  // namespace fftfree
+
+void* phase_init(int N, int hop, int half_spectrum, int mode, int iterations) {
+  if (N <= 0) return nullptr;
+  try {
+    auto* c = new fftfree::PhaseCtx();
+    c->n = N;
+    c->hop = hop > 0 ? hop : N;
+    c->half = half_spectrum != 0;
+    c->mode = mode;
+    c->iterations = iterations;
+    if (mode == 1 && fftfree::is_pow2((size_t)N)) {  // the reference's plans need a power of two
+      c->inv = static_cast<FftContext*>(fft_init_full_v2((size_t)N, 1, 1, 1, 0, 0, nullptr, 0, 0, 0, 0,
+                                                         FFT_TRANSFORM_C2C, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0.f,
+                                                         0.f, 0, 0.f, 0.f, 0, 0));
+      c->fwd = static_cast<FftContext*>(fft_init_full_v2((size_t)N, 1, 1, 0, 0, 0, nullptr, 0, 0, 0, 0,
+                                                         FFT_TRANSFORM_C2C, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0.f,
+                                                         0.f, 0, 0.f, 0.f, 0, 0));
+    }
+    return c;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+size_t phase_infer_execute_device(void* handle, const float* d_in_mag, size_t frames, float* d_out_real,
+                                  float* d_out_imag, void* stream) {
+  if (!handle || !d_in_mag || !d_out_real || !d_out_imag || frames == 0) return 0;
+  try {
+    return fftfree::phase_execute(static_cast<fftfree::PhaseCtx*>(handle), d_in_mag, frames, d_out_real, d_out_imag,
+                                  static_cast<hipStream_t>(stream));
+  } catch (const std::exception& e) {
+    fftfree::report("phase_infer_execute_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+size_t phase_infer_execute(void* handle, const float* in_mag, size_t frames, float* out_real, float* out_imag) {
+  if (!handle || !in_mag || !out_real || !out_imag || frames == 0) return 0;
+  auto* c = static_cast<fftfree::PhaseCtx*>(handle);
+  try {
+    const size_t bins = c->half ? (size_t)(c->n / 2 + 1) : (size_t)c->n;
+    const size_t bytes = frames * bins * sizeof(float);
+    const size_t off = (bytes + 255) & ~size_t(255);
+    c->staging.reserve(3 * off);
+    float* d_mag = c->staging.as<float>();
+    float* d_re = reinterpret_cast<float*>(c->staging.as<char>() + off);
+    float* d_im = reinterpret_cast<float*>(c->staging.as<char>() + 2 * off);
+    fftfree::hip_check(hipMemcpy(d_mag, in_mag, bytes, hipMemcpyHostToDevice), "H2D");
+    const size_t got = fftfree::phase_execute(c, d_mag, frames, d_re, d_im, nullptr);
+    if (got == 0) return 0;
+    fftfree::hip_check(hipMemcpy(out_real, d_re, bytes, hipMemcpyDeviceToHost), "D2H");
+    fftfree::hip_check(hipMemcpy(out_imag, d_im, bytes, hipMemcpyDeviceToHost), "D2H");
+    return got;
+  } catch (const std::exception& e) {
+    fftfree::report("phase_infer_execute", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
+void phase_free(void* handle) { delete static_cast<fftfree::PhaseCtx*>(handle); }
+
 // ---- streaming (fft_cffi.cpp:1342-1742) -----------------------------------------------
 // The sliding buffers live on the host, as the reference keeps them; every transform
 // runs through the batched entry points above (one launch per call).

@@ -1,0 +1,153 @@
+// fft_phase.hip -- phase inference from magnitude spectrograms (phase_infer_execute,
+// fft_cffi.cpp:2072-2162 and phase_infer.hpp:23-66), frame-major planes of `bins` floats.
+//
+// Mode 0 (infer_linear, phase_infer.hpp:36-66): phi[f][k] is the double-precision running
+// sum of dphi[k] = 2 pi kk hop / N over the frames before f (kk = k mod N), and
+// out = (float)(m cos phi), (float)(m sin phi) in double. The running sum is reproduced
+// exactly: a scan per bin writes phi at every 64th frame, then one thread per (64-frame
+// block, bin) continues the same additions.
+//
+// Mode 1 (minimum phase, fft_cffi.cpp:2095-2158): per frame the real-even log spectrum
+// log(max(1e-20, m)) (mirrored for a half spectrum), its inverse DFT (the real cepstrum),
+// the causal fold (samples 1 .. (N+1)/2-1 doubled, N/2+1 .. N-1 cleared), the forward DFT,
+// and out = exp(Re L) (cos Im L, sin Im L). The DFTs run on the batched complex kernels;
+// the three elementwise steps are here.
+#include "fft_device.hpp"
+#include "fft_kernels.hpp"
+
+namespace fftfree {
+
+namespace {
+
+constexpr int kPhaseBlock = 64;  // frames per checkpoint of the linear phase
+
+__device__ __forceinline__ double linear_dphi(int k, int n, int hop) {
+  const int kk = k < n ? k : k % n;
+  return 6.28318530717958647692 * (double)kk * (double)hop / (double)n;  // phase_infer.hpp:47-51
+}
+
+__global__ void __launch_bounds__(256) phase_linear_scan_kernel(int bins, size_t blocks, int n, int hop,
+                                                                double* __restrict__ ckpt) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= bins) return;
+  const double d = linear_dphi(k, n, hop);
+  double phi = 0.0;
+  for (size_t b = 0; b < blocks; ++b) {
+    ckpt[b * (size_t)bins + k] = phi;
+    for (int i = 0; i < kPhaseBlock; ++i) phi += d;
+  }
+}
+
+__global__ void __launch_bounds__(256) phase_linear_kernel(const float* __restrict__ mag, size_t frames, int bins,
+                                                           int n, int hop, const double* __restrict__ ckpt,
+                                                           float* __restrict__ o_re, float* __restrict__ o_im) {
+  const size_t blocks = (frames + kPhaseBlock - 1) / kPhaseBlock;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= blocks * (size_t)bins) return;
+  const size_t b = idx / (size_t)bins;
+  const int k = (int)(idx - b * (size_t)bins);
+  const double d = linear_dphi(k, n, hop);
+  double phi = ckpt[idx];
+  for (int i = 0; i < kPhaseBlock; ++i) {
+    const size_t f = b * kPhaseBlock + i;
+    if (f >= frames) break;
+    const size_t at = f * (size_t)bins + k;
+    const double m = (double)mag[at];
+    double s, c;
+    sincos(phi, &s, &c);
+    o_re[at] = (float)(m * c);
+    o_im[at] = (float)(m * s);
+    phi += d;
+  }
+}
+
+// Real-even log spectrum of every frame into N complex values per frame.
+__global__ void __launch_bounds__(256) phase_log_kernel(const float* __restrict__ mag, size_t frames, int bins, int n,
+                                                        int half, float2* __restrict__ cols) {
+  const size_t total = frames * (size_t)n;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const size_t f = idx / (size_t)n;
+    const int i = (int)(idx - f * (size_t)n);
+    int k = i;
+    if (half && i > n / 2) k = n - i;  // spec[N-k] = spec[k] (fft_cffi.cpp:2119-2126)
+    const bool set = !half || i != n / 2 || (n % 2) == 0;
+    const float m = mag[f * (size_t)bins + k];
+    cols[idx] = make_float2(set ? logf(fmaxf(1e-20f, m)) : 0.0f, 0.0f);
+  }
+}
+
+// Causal fold of the real cepstrum (fft_cffi.cpp:2138-2143).
+__global__ void __launch_bounds__(256) phase_fold_kernel(size_t frames, int n, float2* __restrict__ cols) {
+  const size_t total = frames * (size_t)n;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const int i = (int)(idx % (size_t)n);
+    float2 v = cols[idx];
+    if (i >= 1 && i < (n + 1) / 2) {
+      v.x *= 2.0f;
+      v.y *= 2.0f;
+    } else if (i > n / 2) {
+      v = make_float2(0.0f, 0.0f);
+    }
+    cols[idx] = v;
+  }
+}
+
+// exp of the complex log spectrum into the output planes (fft_cffi.cpp:2150-2156).
+__global__ void __launch_bounds__(256) phase_exp_kernel(const float2* __restrict__ cols, size_t frames, int bins,
+                                                        int n, float* __restrict__ o_re, float* __restrict__ o_im) {
+  const size_t total = frames * (size_t)bins;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const size_t f = idx / (size_t)bins;
+    const int k = (int)(idx - f * (size_t)bins);
+    const float2 L = cols[f * (size_t)n + k];
+    const float a = expf(L.x);
+    float s, c;
+    sincosf(L.y, &s, &c);
+    o_re[idx] = a * c;
+    o_im[idx] = a * s;
+  }
+}
+
+unsigned grid_for(size_t work) {
+  size_t g = (work + 255) / 256;
+  return (unsigned)(g > 65536 ? 65536 : (g == 0 ? 1 : g));
+}
+
+}  // namespace
+
+size_t phase_linear_scratch(size_t frames, int bins) {
+  return ((frames + kPhaseBlock - 1) / kPhaseBlock) * (size_t)bins;  // doubles
+}
+
+hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, double* ckpt, float* o_re,
+                               float* o_im, hipStream_t s) {
+  if (frames == 0 || bins <= 0) return hipSuccess;
+  const size_t blocks = (frames + kPhaseBlock - 1) / kPhaseBlock;
+  hipLaunchKernelGGL(phase_linear_scan_kernel, dim3((unsigned)((bins + 255) / 256)), dim3(256), 0, s, bins, blocks, n,
+                     hop, ckpt);
+  const size_t work = blocks * (size_t)bins;
+  if ((work + 255) / 256 > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(phase_linear_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, mag, frames, bins, n,
+                     hop, ckpt, o_re, o_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_log(const float* mag, size_t frames, int bins, int n, int half, float2* cols, hipStream_t s) {
+  hipLaunchKernelGGL(phase_log_kernel, dim3(grid_for(frames * (size_t)n)), dim3(256), 0, s, mag, frames, bins, n, half,
+                     cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_fold(size_t frames, int n, float2* cols, hipStream_t s) {
+  hipLaunchKernelGGL(phase_fold_kernel, dim3(grid_for(frames * (size_t)n)), dim3(256), 0, s, frames, n, cols);
+  return hipGetLastError();
+}
+
+hipError_t launch_phase_exp(const float2* cols, size_t frames, int bins, int n, float* o_re, float* o_im,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(phase_exp_kernel, dim3(grid_for(frames * (size_t)bins)), dim3(256), 0, s, cols, frames, bins, n,
+                     o_re, o_im);
+  return hipGetLastError();
+}
+
+}  // namespace fftfree

@@ -150,6 +150,20 @@ size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_
 size_t fft_execute_complex_batched_device(void* handle, const float* d_in_real, const float* d_in_imag,
                                           size_t frames, float* d_out_pcm, size_t max_frames, void* stream);
 
+/* reference: fft_cffi.hpp:269-277 (fft_cffi.cpp:2058-2174, phase_infer.hpp:23-66). Phase
+ * inference from a frame-major magnitude spectrogram (frames x bins, bins = N/2+1 with
+ * half_spectrum, else N). mode 0: the linear phase advance phi[f][k] = f * 2 pi k hop / N
+ * (accumulated in double, as the reference does); mode 1: the minimum-phase spectrum of
+ * each frame through the real cepstrum (N a power of two). phase_infer_execute returns
+ * frames, or 0 (unknown mode, bad arguments). Host pointers; phase_free(NULL) is a no-op. */
+void* phase_init(int N, int hop, int half_spectrum, int mode, int iterations);
+size_t phase_infer_execute(void* handle, const float* in_mag, size_t frames, float* out_real, float* out_imag);
+void phase_free(void* handle);
+
+/* additive. phase_infer_execute on device buffers, enqueued on `stream`. */
+size_t phase_infer_execute_device(void* handle, const float* d_in_mag, size_t frames, float* d_out_real,
+                                  float* d_out_imag, void* stream);
+
 /* reference: fft_cffi.hpp:188-211 (fft_cffi.cpp:1342-1742). Streaming STFT on a context.
  * fft_stream_push_pcm appends `samples` to the context's buffer and writes the frames that
  * are now complete (at most max_frames, 0 = all; hop-aligned from the stream start,

@@ -18,6 +18,7 @@
 //   frame count + framing + output    fft_cffi.cpp:701-825
 //   complex-input batched inverse     fft_cffi.cpp:1042-1157 (output stage 1119-1157, 210-226)
 //   Griffin-Lim                       phase_infer.hpp:69-204, fft_cffi.cpp:2176-2248
+//   phase inference (linear, cepstral) phase_infer.hpp:36-66, fft_cffi.cpp:2095-2158
 //   metadata writers (index maps)     eigen_fft.hpp:1529-1888 as called at 2672-2682,
 //                                     2981-3011, 3239-3244, 3478-3505
 // The reference's WorkerPool (eigen_fft.hpp:818-1419) is restated as a plain
@@ -558,6 +559,63 @@ int ref_griffin_lim(int N, int half, const float* mag, size_t frames, int hop, i
         out_imag[f * bins] = 0.0f;
         if ((n & 1) == 0 && bins > 1) out_imag[f * bins + bins - 1] = 0.0f;
       }
+  }
+  return 1;
+}
+
+// infer_linear (phase_infer.hpp:36-66): per-bin phase advance accumulated in double.
+void ref_phase_linear(int N, int hop, int half, const float* mag, size_t frames, float* out_real, float* out_imag) {
+  const int H = hop > 0 ? hop : N;
+  const int bins = half ? N / 2 + 1 : N;
+  const double two_pi = 6.28318530717958647692;
+  std::vector<double> dphi(static_cast<size_t>(bins), 0.0), phi(static_cast<size_t>(bins), 0.0);
+  for (int k = 0; k < bins; ++k) {
+    const int kk = k < N ? k : k % N;
+    dphi[size_t(k)] = two_pi * double(kk) * double(H) / double(N);
+  }
+  for (size_t f = 0; f < frames; ++f)
+    for (int k = 0; k < bins; ++k) {
+      const size_t i = f * size_t(bins) + size_t(k);
+      const double m = double(mag[i]);
+      out_real[i] = float(m * std::cos(phi[size_t(k)]));
+      out_imag[i] = float(m * std::sin(phi[size_t(k)]));
+      phi[size_t(k)] += dphi[size_t(k)];
+    }
+}
+
+// Minimum-phase spectra through the real cepstrum (fft_cffi.cpp:2095-2158), on the
+// restated float transforms (Cooley-Tukey, the plan cache's default kernel).
+int ref_phase_minimum(int N, int half, const float* mag, size_t frames, float* out_real, float* out_imag) {
+  RefPlan Pi, Pf;
+  if (!make_plan(N, true, Pi) || !make_plan(N, false, Pf)) return 0;
+  const int bins = half ? N / 2 + 1 : N;
+  const Flags c2c{0, false, false, false};
+  std::vector<cf> spec(size_t(N), cf(0, 0)), tmp;
+  const float eps = 1e-20f;
+  for (size_t f = 0; f < frames; ++f) {
+    const size_t base = f * size_t(bins);
+    auto set_log = [&](int k, float m) { spec[size_t(k)] = cf(std::log(std::max(eps, m)), 0.0f); };
+    set_log(0, mag[base]);
+    if (half) {
+      for (int k = 1; k < N / 2; ++k) {
+        set_log(k, mag[base + size_t(k)]);
+        spec[size_t(N - k)] = spec[size_t(k)];
+      }
+      if (N % 2 == 0) set_log(N / 2, mag[base + size_t(N / 2)]);
+    } else {
+      for (int k = 1; k < N; ++k) set_log(k, mag[base + size_t(k)]);
+    }
+    tmp = spec;
+    ct_column(Pi, c2c, tmp.data(), nullptr);
+    for (int n = 1; n < (N + 1) / 2; ++n) tmp[size_t(n)] *= 2.0f;
+    for (int n = N / 2 + 1; n < N; ++n) tmp[size_t(n)] = cf(0.0f, 0.0f);
+    ct_column(Pf, c2c, tmp.data(), nullptr);
+    for (int k = 0; k < bins; ++k) {
+      const cf L = tmp[size_t(k)];
+      const float a = std::exp(L.real());
+      out_real[base + size_t(k)] = a * std::cos(L.imag());
+      out_imag[base + size_t(k)] = a * std::sin(L.imag());
+    }
   }
   return 1;
 }

@@ -54,6 +54,10 @@ def lib():
                                        ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int])
         L.ref_griffin_lim.restype = ctypes.c_int
+        L.ref_phase_linear.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _sz, _f32p, _f32p]
+        L.ref_phase_linear.restype = None
+        L.ref_phase_minimum.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _sz, _f32p, _f32p]
+        L.ref_phase_minimum.restype = ctypes.c_int
         L.ref_metadata.argtypes = [ctypes.c_int] * 6 + [_f32p, _sz]
         L.ref_metadata.restype = _sz
         _lib = L
@@ -164,6 +168,17 @@ def griffin_lim(mag, N, hop, iterations=32, pad_mode=1, seed=0, half=True, inv_k
                                inv_transform, _p(syn), fwd_kernel, fwd_window or N, fwd_hop or hop, fwd_ctx_pad,
                                _p(taps), _p(re), _p(im), threads or default_threads())
     return int(ok), re, im
+
+
+def phase_infer(mag, N, hop=0, half=True, mode=0):
+    """Reference phase_infer_execute (modes 0 and 1): returns (real, imag) shaped like mag."""
+    mag = np.ascontiguousarray(mag, dtype=np.float32)
+    re, im = np.zeros_like(mag), np.zeros_like(mag)
+    if mode == 0:
+        lib().ref_phase_linear(N, hop, int(half), _p(mag), mag.shape[0], _p(re), _p(im))
+    elif not lib().ref_phase_minimum(N, int(half), _p(mag), mag.shape[0], _p(re), _p(im)):
+        raise ValueError("N must be a power of two")
+    return re, im
 
 
 def metadata(N: int, kind: int, kernel: int = 0, inverse=False, transform=0, half_spectrum=False) -> np.ndarray:

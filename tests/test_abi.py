@@ -17,7 +17,7 @@ HEADER = Path(__file__).resolve().parents[1] / "include" / "fft_cffi.h"
 
 def declared_functions():
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    names = re.findall(r"\b(?:void\s*\*|void|int|size_t)\s+(fft_[a-z0-9_]+)\s*\(", text)
+    names = re.findall(r"\b(?:void\s*\*|void|int|size_t)\s+((?:fft|phase)_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
 
 

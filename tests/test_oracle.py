@@ -556,3 +556,42 @@ def test_stream_oracle_inverse_equals_batch_overlap_add(cola):
     # streaming path rounds the windowed frame first: a few ulps apart
     assert np.max(np.abs(got[:L] - exp[:L])) <= 4 * np.finfo(np.float32).eps * np.max(np.abs(exp))
     assert not got[L:].any()
+
+
+# ---- phase inference ------------------------------------------------------------------
+
+def test_phase_linear_oracle_closed_form():
+    """Mode 0 equals m * e^{i f 2 pi k hop / N}, the accumulated phase agreeing with the
+    product to double rounding (checked in float64 on small frame counts); bin 0 is real."""
+    N, H, F = 64, 16, 50
+    mag = np.random.default_rng(4).random((F, N // 2 + 1)).astype(np.float32) + 0.1
+    re, im = ref.phase_infer(mag, N, H, half=True, mode=0)
+    f = np.arange(F)[:, None]
+    k = np.arange(N // 2 + 1)[None, :]
+    exp = mag.astype(np.float64) * np.exp(1j * f * 2 * np.pi * k * H / N)
+    assert np.max(np.abs((re + 1j * im) - exp)) <= 1e-6
+    assert not im[:, 0].any()
+
+
+@pytest.mark.parametrize("N", [64, 256])
+@pytest.mark.parametrize("a", [0.5, -0.7])
+def test_phase_minimum_oracle_one_pole(N, a):
+    """Mode 1 on |1 / (1 - a e^{-i w})| returns the minimum-phase filter itself,
+    1 / (1 - a e^{-i w}) (its cepstrum is causal: log H = sum a^n e^{-i n w} / n), up to
+    the cepstral aliasing a^N and float rounding; |out| equals the magnitude given."""
+    k = np.arange(N // 2 + 1)
+    h = 1.0 / (1.0 - a * np.exp(-2j * np.pi * k / N))
+    mag = np.abs(h)[None, :].astype(np.float32).repeat(3, axis=0)
+    re, im = ref.phase_infer(mag, N, 0, half=True, mode=1)
+    got = re + 1j * im
+    assert np.max(np.abs(got - h[None, :])) <= 1e-5 * np.max(np.abs(h))
+    assert np.max(np.abs(np.abs(got) - mag)) <= 1e-5 * np.max(mag)
+
+
+def test_phase_minimum_oracle_check_has_teeth():
+    """A planted mistake (the anti-causal half of the cepstrum kept, i.e. zero phase)
+    misses the one-pole filter by far."""
+    N, a = 256, 0.5
+    k = np.arange(N // 2 + 1)
+    h = 1.0 / (1.0 - a * np.exp(-2j * np.pi * k / N))
+    assert np.max(np.abs(np.abs(h) - h)) > 0.3 * np.max(np.abs(h))
