@@ -1,0 +1,257 @@
+// ft_tools.hip -- the time-frequency grid pipeline of ft_apply_complex_pipeline /
+// ft_apply_polar_pipeline (ft_tools_cffi.cpp:76-170 over ft_tools.hpp:144-477) on the
+// device: bilinear upsampling, the exponential harmonic magnitude quantizer, the local
+// contrast booster (a box stencil over magnitudes) and box downsampling. Grids are
+// time-major (index t * freq_bins + f). Each step is one elementwise or stencil launch;
+// every output cell is computed by one thread, in the reference's order of float
+// operations within that cell.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ft_kernels.hpp"
+
+namespace fftfree {
+
+namespace {
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+
+// ft_tools.hpp:144-183: bilinear interpolation at (t / tf, f / ff).
+__global__ void __launch_bounds__(256) ft_upsample_kernel(const float2* __restrict__ src, int ts, int fs, int tf,
+                                                          int ff, float2* __restrict__ dst) {
+  const int td = ts * tf, fd = fs * ff;
+  const size_t total = (size_t)td * fd;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const int t = (int)(idx / (size_t)fd);
+    const int f = (int)(idx - (size_t)t * fd);
+    const double st = (double)t / (double)tf;
+    const int t0 = (int)floor(st);
+    const int t1 = min(t0 + 1, ts - 1);
+    const float tw = (float)(st - (double)t0);
+    const double sf = (double)f / (double)ff;
+    const int f0 = (int)floor(sf);
+    const int f1 = min(f0 + 1, fs - 1);
+    const float fw = (float)(sf - (double)f0);
+    const float2 c00 = src[(size_t)t0 * fs + f0], c10 = src[(size_t)t1 * fs + f0];
+    const float2 c01 = src[(size_t)t0 * fs + f1], c11 = src[(size_t)t1 * fs + f1];
+    const float2 a = make_float2(lerpf(c00.x, c10.x, tw), lerpf(c00.y, c10.y, tw));
+    const float2 b = make_float2(lerpf(c01.x, c11.x, tw), lerpf(c01.y, c11.y, tw));
+    dst[idx] = make_float2(lerpf(a.x, b.x, fw), lerpf(a.y, b.y, fw));
+  }
+}
+
+// ft_tools.hpp:185-227: box average over factor x factor source cells (sum when
+// preserve_energy is off), source rows outer, columns inner.
+__global__ void __launch_bounds__(256) ft_downsample_kernel(const float2* __restrict__ src, int ts, int fs, int tf,
+                                                            int ff, int preserve, float2* __restrict__ dst) {
+  const int td = (ts + tf - 1) / tf, fd = (fs + ff - 1) / ff;
+  const size_t total = (size_t)td * fd;
+  for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+    const int t = (int)(idx / (size_t)fd);
+    const int f = (int)(idx - (size_t)t * fd);
+    const int tb = t * tf, te = min(tb + tf, ts);
+    const int fb = f * ff, fe = min(fb + ff, fs);
+    float2 acc = make_float2(0.0f, 0.0f);
+    int count = 0;
+    for (int a = tb; a < te; ++a)
+      for (int b = fb; b < fe; ++b) {
+        const float2 v = src[(size_t)a * fs + b];
+        acc.x += v.x;
+        acc.y += v.y;
+        ++count;
+      }
+    if (preserve && count > 0) {
+      const float inv = 1.0f / (float)count;
+      acc.x *= inv;
+      acc.y *= inv;
+    }
+    dst[idx] = acc;
+  }
+}
+
+// std::polar / std::abs / std::arg between the two grid forms (ft_tools.hpp:123-142).
+__global__ void __launch_bounds__(256) ft_polar_to_complex_kernel(const float* __restrict__ mag,
+                                                                  const float* __restrict__ ph, size_t total,
+                                                                  float2* __restrict__ z) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    float s, c;
+    sincosf(ph[i], &s, &c);
+    z[i] = make_float2(mag[i] * c, mag[i] * s);
+  }
+}
+
+__global__ void __launch_bounds__(256) ft_complex_to_polar_kernel(const float2* __restrict__ z, size_t total,
+                                                                  float* __restrict__ mag, float* __restrict__ ph) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const float2 v = z[i];
+    mag[i] = hypotf(v.x, v.y);
+    ph[i] = atan2f(v.y, v.x);
+  }
+}
+
+// ExponentialHarmonicQuantizer (ft_tools.hpp:262-356).
+__device__ float quant_weight(const FtQuantizer& q, float freq) {
+  if (q.mix <= 0.0f) return 0.0f;
+  if (q.base_frequency <= 0.0f) return fminf(fmaxf(q.mix, 0.0f), 1.0f);
+  const float ratio = freq / q.base_frequency;
+  if (!isfinite((double)ratio)) return 0.0f;
+  const float nearest = (float)round((double)ratio);  // halves away from zero, as std::round
+  const float distance = fabsf(ratio - nearest);
+  const float attraction = expf(-q.distance_decay * distance);
+  return fminf(fmaxf(q.mix * attraction, 0.0f), 1.0f);
+}
+
+__device__ float quant_magnitude(const FtQuantizer& q, float m, float freq) {
+  const float w = quant_weight(q, freq);
+  if (w <= 0.0f) return m;
+  float target = m;
+  if (q.magnitude_step > 0.0f) target = (float)round((double)(m / q.magnitude_step)) * q.magnitude_step;
+  return lerpf(m, target, w);
+}
+
+__global__ void __launch_bounds__(256) ft_quantize_kernel(float2* __restrict__ z, float* __restrict__ mag, int tb,
+                                                          int fb, FtQuantizer q, double freq_origin,
+                                                          double freq_step) {
+  const size_t total = (size_t)tb * fb;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const int f = (int)(i % (size_t)fb);
+    const float freq = (float)(freq_origin + freq_step * (double)f);
+    if (mag) {  // polar grid: the magnitude alone
+      mag[i] = quant_magnitude(q, mag[i], freq);
+    } else {    // complex grid: |z| moved, the phase kept; tiny values untouched
+      const float2 v = z[i];
+      const float m = hypotf(v.x, v.y);
+      if (m <= 1.1920928955078125e-07f) continue;  // FLT_EPSILON
+      const float adj = fmaxf(quant_magnitude(q, m, freq), 0.0f);
+      float s, c;
+      sincosf(atan2f(v.y, v.x), &s, &c);
+      z[i] = make_float2(adj * c, adj * s);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) ft_abs_kernel(const float2* __restrict__ z, size_t total,
+                                                     float* __restrict__ mag) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
+    mag[i] = hypotf(z[i].x, z[i].y);
+}
+
+// Mean of all magnitudes (the global contrast mode): one workgroup.
+__global__ void __launch_bounds__(256) ft_mean_kernel(const float* __restrict__ mag, size_t total,
+                                                      float* __restrict__ out) {
+  __shared__ float part[256];
+  float s = 0.0f;
+  for (size_t i = threadIdx.x; i < total; i += 256) s += mag[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0] / (float)total;
+}
+
+// SpectralLocalContrastBooster (ft_tools.hpp:358-461): local box mean, contrast about it,
+// brightness, optional clamp at 0 and gamma.
+__global__ void __launch_bounds__(256) ft_contrast_kernel(const float* __restrict__ mag, int tb, int fb,
+                                                          FtContrast c, const float* __restrict__ global_mean,
+                                                          float* __restrict__ out) {
+  const bool gt = c.kernel_time <= 0, gf = c.kernel_freq <= 0;
+  const int rt = gt ? tb : max(c.kernel_time / 2, 0);
+  const int rf = gf ? fb : max(c.kernel_freq / 2, 0);
+  const float gamma = c.gamma > 0.0f ? c.gamma : 1.0f;
+  const bool use_gamma = fabsf(gamma - 1.0f) > 1e-5f;
+  const float inv_gamma = use_gamma ? 1.0f / gamma : 1.0f;
+  const size_t total = (size_t)tb * fb;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const int t = (int)(i / (size_t)fb);
+    const int f = (int)(i - (size_t)t * fb);
+    float mean;
+    if (gt && gf) {
+      mean = global_mean[0];
+    } else {
+      const int t0 = gt ? 0 : max(0, t - rt), t1 = gt ? tb - 1 : min(tb - 1, t + rt);
+      const int f0 = gf ? 0 : max(0, f - rf), f1 = gf ? fb - 1 : min(fb - 1, f + rf);
+      float sum = 0.0f;
+      int count = 0;
+      for (int a = t0; a <= t1; ++a)
+        for (int b = f0; b <= f1; ++b) {
+          sum += mag[(size_t)a * fb + b];
+          ++count;
+        }
+      mean = count > 0 ? sum / (float)count : 0.0f;
+    }
+    float adj = mean + (mag[i] - mean) * (1.0f + c.contrast) + c.brightness;
+    if (c.clamp_zero) adj = fmaxf(adj, 0.0f);
+    if (use_gamma) adj = (float)pow((double)fmaxf(adj, 1e-12f), (double)inv_gamma);
+    out[i] = adj;
+  }
+}
+
+// z = std::polar(mag, arg z) (ft_tools.hpp:381-385).
+__global__ void __launch_bounds__(256) ft_set_magnitude_kernel(float2* __restrict__ z, const float* __restrict__ mag,
+                                                               size_t total) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    float s, c;
+    sincosf(atan2f(z[i].y, z[i].x), &s, &c);
+    z[i] = make_float2(mag[i] * c, mag[i] * s);
+  }
+}
+
+unsigned grid_of(size_t work) {
+  const size_t g = (work + 255) / 256;
+  return (unsigned)(g == 0 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+}  // namespace
+
+hipError_t ft_upsample(const float2* src, int ts, int fs, int tf, int ff, float2* dst, hipStream_t s) {
+  hipLaunchKernelGGL(ft_upsample_kernel, dim3(grid_of((size_t)ts * tf * fs * ff)), dim3(256), 0, s, src, ts, fs, tf,
+                     ff, dst);
+  return hipGetLastError();
+}
+
+hipError_t ft_downsample(const float2* src, int ts, int fs, int tf, int ff, int preserve, float2* dst,
+                         hipStream_t s) {
+  const size_t out = (size_t)((ts + tf - 1) / tf) * ((fs + ff - 1) / ff);
+  hipLaunchKernelGGL(ft_downsample_kernel, dim3(grid_of(out)), dim3(256), 0, s, src, ts, fs, tf, ff, preserve, dst);
+  return hipGetLastError();
+}
+
+hipError_t ft_polar_to_complex(const float* mag, const float* ph, size_t total, float2* z, hipStream_t s) {
+  hipLaunchKernelGGL(ft_polar_to_complex_kernel, dim3(grid_of(total)), dim3(256), 0, s, mag, ph, total, z);
+  return hipGetLastError();
+}
+
+hipError_t ft_complex_to_polar(const float2* z, size_t total, float* mag, float* ph, hipStream_t s) {
+  hipLaunchKernelGGL(ft_complex_to_polar_kernel, dim3(grid_of(total)), dim3(256), 0, s, z, total, mag, ph);
+  return hipGetLastError();
+}
+
+hipError_t ft_quantize(float2* z, float* mag, int tb, int fb, const FtQuantizer& q, double freq_origin,
+                       double freq_step, hipStream_t s) {
+  hipLaunchKernelGGL(ft_quantize_kernel, dim3(grid_of((size_t)tb * fb)), dim3(256), 0, s, z, mag, tb, fb, q,
+                     freq_origin, freq_step);
+  return hipGetLastError();
+}
+
+hipError_t ft_contrast(float2* z, float* mag, int tb, int fb, const FtContrast& c, float* scratch, hipStream_t s) {
+  const size_t total = (size_t)tb * fb;
+  // scratch: total floats of magnitudes (complex grids), total floats of output, 1 mean
+  float* m = z ? scratch : mag;
+  float* out = scratch + total;
+  float* mean = scratch + 2 * total;
+  if (z) hipLaunchKernelGGL(ft_abs_kernel, dim3(grid_of(total)), dim3(256), 0, s, z, total, m);
+  if (c.kernel_time <= 0 && c.kernel_freq <= 0)
+    hipLaunchKernelGGL(ft_mean_kernel, dim3(1), dim3(256), 0, s, m, total, mean);
+  hipLaunchKernelGGL(ft_contrast_kernel, dim3(grid_of(total)), dim3(256), 0, s, m, tb, fb, c, mean, out);
+  if (z) {
+    hipLaunchKernelGGL(ft_set_magnitude_kernel, dim3(grid_of(total)), dim3(256), 0, s, z, out, total);
+  } else {
+    const hipError_t e = hipMemcpyAsync(mag, out, total * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fftfree

@@ -19,6 +19,7 @@
 //   complex-input batched inverse     fft_cffi.cpp:1042-1157 (output stage 1119-1157, 210-226)
 //   Griffin-Lim                       phase_infer.hpp:69-204, fft_cffi.cpp:2176-2248
 //   phase inference (linear, cepstral) phase_infer.hpp:36-66, fft_cffi.cpp:2095-2158
+//   time-frequency grid pipeline      ft_tools.hpp:144-477, ft_tools_cffi.cpp:18-170
 //   metadata writers (index maps)     eigen_fft.hpp:1529-1888 as called at 2672-2682,
 //                                     2981-3011, 3239-3244, 3478-3505
 // The reference's WorkerPool (eigen_fft.hpp:818-1419) is restated as a plain
@@ -35,6 +36,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <random>
 #include <thread>
 #include <vector>
@@ -616,6 +618,188 @@ int ref_phase_minimum(int N, int half, const float* mag, size_t frames, float* o
       out_real[base + size_t(k)] = a * std::cos(L.imag());
       out_imag[base + size_t(k)] = a * std::sin(L.imag());
     }
+  }
+  return 1;
+}
+
+// The grid pipeline of ft_apply_complex_pipeline / ft_apply_polar_pipeline: bilinear
+// upsample (ft_tools.hpp:144-183), harmonic quantizer (262-356), local contrast booster
+// (358-461), box downsample (185-227), polar grids resampled through complex values
+// (229-245). Settings arrive unpacked; returns 1, or 0 when the result's shape differs.
+namespace {
+struct FtField {
+  int tb = 0, fb = 0;
+  std::vector<cf> z;  // complex grid
+  std::vector<float> mag, ph;  // polar grid
+};
+
+std::vector<cf> ft_up(const std::vector<cf>& src, int ts, int fs, int tf, int ff) {
+  const int td = ts * tf, fd = fs * ff;
+  std::vector<cf> dst(size_t(td) * size_t(fd));
+  auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+  for (int t = 0; t < td; ++t) {
+    const double st = double(t) / double(tf);
+    const int t0 = int(std::floor(st)), t1 = std::min(t0 + 1, ts - 1);
+    const float tw = float(st - double(t0));
+    for (int f = 0; f < fd; ++f) {
+      const double sf = double(f) / double(ff);
+      const int f0 = int(std::floor(sf)), f1 = std::min(f0 + 1, fs - 1);
+      const float fw = float(sf - double(f0));
+      const cf c00 = src[size_t(t0) * fs + f0], c10 = src[size_t(t1) * fs + f0];
+      const cf c01 = src[size_t(t0) * fs + f1], c11 = src[size_t(t1) * fs + f1];
+      const cf a(lerp(c00.real(), c10.real(), tw), lerp(c00.imag(), c10.imag(), tw));
+      const cf b(lerp(c01.real(), c11.real(), tw), lerp(c01.imag(), c11.imag(), tw));
+      dst[size_t(t) * fd + f] = cf(lerp(a.real(), b.real(), fw), lerp(a.imag(), b.imag(), fw));
+    }
+  }
+  return dst;
+}
+
+std::vector<cf> ft_down(const std::vector<cf>& src, int ts, int fs, int tf, int ff, bool preserve) {
+  const int td = (ts + tf - 1) / tf, fd = (fs + ff - 1) / ff;
+  std::vector<cf> dst(size_t(td) * size_t(fd));
+  for (int t = 0; t < td; ++t)
+    for (int f = 0; f < fd; ++f) {
+      cf acc(0.0f, 0.0f);
+      int count = 0;
+      for (int a = t * tf; a < std::min(t * tf + tf, ts); ++a)
+        for (int b = f * ff; b < std::min(f * ff + ff, fs); ++b) {
+          acc += src[size_t(a) * fs + b];
+          ++count;
+        }
+      if (preserve && count > 0) acc *= 1.0f / float(count);
+      dst[size_t(t) * fd + f] = acc;
+    }
+  return dst;
+}
+
+float ft_quant_mag(float m, float freq, float base, float step, float mix, float decay) {
+  float w;
+  if (mix <= 0.0f) {
+    w = 0.0f;
+  } else if (base <= 0.0f) {
+    w = std::min(std::max(mix, 0.0f), 1.0f);
+  } else {
+    const float ratio = freq / base;
+    if (!std::isfinite(double(ratio))) {
+      w = 0.0f;
+    } else {
+      const float nearest = float(std::round(double(ratio)));
+      const float attraction = std::exp(-decay * std::fabs(ratio - nearest));
+      w = std::min(std::max(mix * attraction, 0.0f), 1.0f);
+    }
+  }
+  if (w <= 0.0f) return m;
+  float target = m;
+  if (step > 0.0f) target = float(std::round(double(m / step))) * step;
+  return m + (target - m) * w;
+}
+
+void ft_contrast_mags(std::vector<float>& mags, int tb, int fb, int kt, int kf, float bright, float contrast,
+                      float gamma, bool clamp_zero) {
+  std::vector<float> out(mags.size());
+  const bool gt = kt <= 0, gf = kf <= 0;
+  const int rt = gt ? tb : std::max(kt / 2, 0), rf = gf ? fb : std::max(kf / 2, 0);
+  float gmean = 0.0f;
+  if (gt && gf) {
+    for (float v : mags) gmean += v;
+    gmean /= float(mags.size());
+  }
+  const float g = gamma > 0.0f ? gamma : 1.0f;
+  const bool use_gamma = std::fabs(g - 1.0f) > 1e-5f;
+  for (int t = 0; t < tb; ++t)
+    for (int f = 0; f < fb; ++f) {
+      float mean = 0.0f;
+      if (gt && gf) {
+        mean = gmean;
+      } else {
+        int count = 0;
+        for (int a = gt ? 0 : std::max(0, t - rt); a <= (gt ? tb - 1 : std::min(tb - 1, t + rt)); ++a)
+          for (int b = gf ? 0 : std::max(0, f - rf); b <= (gf ? fb - 1 : std::min(fb - 1, f + rf)); ++b) {
+            mean += mags[size_t(a) * fb + b];
+            ++count;
+          }
+        if (count > 0) mean /= float(count);
+      }
+      const size_t i = size_t(t) * fb + f;
+      float adj = mean + (mags[i] - mean) * (1.0f + contrast) + bright;
+      if (clamp_zero) adj = std::max(adj, 0.0f);
+      if (use_gamma) adj = float(std::pow(double(std::max(adj, 1e-12f)), double(1.0f / g)));
+      out[i] = adj;
+    }
+  mags.swap(out);
+}
+}  // namespace
+
+int ref_ft_pipeline(int polar, double f_origin, double f_step, int ut, int uf, int dt, int df, int preserve,
+                    int q_on, float q_base, float q_step, float q_mix, float q_decay, int c_on, int c_kt, int c_kf,
+                    float c_bright, float c_contrast, float c_gamma, int c_clamp, float* a, float* b, int tb,
+                    int fb) {
+  if (!a || !b || tb <= 0 || fb <= 0) return 0;
+  ut = ut > 0 ? ut : 1; uf = uf > 0 ? uf : 1; dt = dt > 0 ? dt : 1; df = df > 0 ? df : 1;
+  if (f_step == 0.0) f_step = 1.0;
+  int t = tb, f = fb;
+  const size_t n0 = size_t(tb) * fb;
+  std::vector<cf> z(n0);
+  std::vector<float> mag(a, a + n0), ph(b, b + n0);
+  auto to_z = [&]() {
+    z.resize(mag.size());
+    for (size_t i = 0; i < mag.size(); ++i) z[i] = std::polar(mag[i], ph[i]);
+  };
+  auto from_z = [&]() {
+    mag.resize(z.size());
+    ph.resize(z.size());
+    for (size_t i = 0; i < z.size(); ++i) {
+      mag[i] = std::abs(z[i]);
+      ph[i] = std::arg(z[i]);
+    }
+  };
+  if (!polar)
+    for (size_t i = 0; i < n0; ++i) z[i] = cf(a[i], b[i]);
+  if (ut > 1 || uf > 1) {
+    if (polar) to_z();
+    z = ft_up(z, t, f, ut, uf);
+    t *= ut;
+    f *= uf;
+    f_step /= double(uf);
+    if (polar) from_z();
+  }
+  if (q_on && q_mix > 0.0f) {
+    for (int i = 0; i < t; ++i)
+      for (int k = 0; k < f; ++k) {
+        const float freq = float(f_origin + f_step * double(k));
+        const size_t at = size_t(i) * f + k;
+        if (polar) {
+          mag[at] = ft_quant_mag(mag[at], freq, q_base, q_step, q_mix, q_decay);
+        } else {
+          const float m = std::abs(z[at]);
+          if (m <= std::numeric_limits<float>::epsilon()) continue;
+          const float adj = std::max(ft_quant_mag(m, freq, q_base, q_step, q_mix, q_decay), 0.0f);
+          z[at] = std::polar(adj, std::arg(z[at]));
+        }
+      }
+  }
+  if (c_on) {
+    if (polar) {
+      ft_contrast_mags(mag, t, f, c_kt, c_kf, c_bright, c_contrast, c_gamma, c_clamp != 0);
+    } else {
+      std::vector<float> m(z.size());
+      for (size_t i = 0; i < z.size(); ++i) m[i] = std::abs(z[i]);
+      ft_contrast_mags(m, t, f, c_kt, c_kf, c_bright, c_contrast, c_gamma, c_clamp != 0);
+      for (size_t i = 0; i < z.size(); ++i) z[i] = std::polar(m[i], std::arg(z[i]));
+    }
+  }
+  if (dt > 1 || df > 1) {
+    if (polar) to_z();
+    z = ft_down(z, t, f, dt, df, preserve != 0);
+    t = (t + dt - 1) / dt;
+    f = (f + df - 1) / df;
+    if (polar) from_z();
+  }
+  if (t != tb || f != fb) return 0;
+  for (size_t i = 0; i < n0; ++i) {
+    a[i] = polar ? mag[i] : z[i].real();
+    b[i] = polar ? ph[i] : z[i].imag();
   }
   return 1;
 }

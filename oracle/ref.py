@@ -58,6 +58,10 @@ def lib():
         L.ref_phase_linear.restype = None
         L.ref_phase_minimum.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _sz, _f32p, _f32p]
         L.ref_phase_minimum.restype = ctypes.c_int
+        L.ref_ft_pipeline.argtypes = ([ctypes.c_int, ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 6 +
+                                      [ctypes.c_float] * 4 + [ctypes.c_int] * 3 + [ctypes.c_float] * 3 +
+                                      [ctypes.c_int, _f32p, _f32p, ctypes.c_int, ctypes.c_int])
+        L.ref_ft_pipeline.restype = ctypes.c_int
         L.ref_metadata.argtypes = [ctypes.c_int] * 6 + [_f32p, _sz]
         L.ref_metadata.restype = _sz
         _lib = L
@@ -179,6 +183,21 @@ def phase_infer(mag, N, hop=0, half=True, mode=0):
     elif not lib().ref_phase_minimum(N, int(half), _p(mag), mag.shape[0], _p(re), _p(im)):
         raise ValueError("N must be a power of two")
     return re, im
+
+
+def ft_pipeline(a, b, polar=False, freq_origin=0.0, freq_step=1.0, up=(1, 1), down=(1, 1), preserve=True,
+                quant=None, contrast=None):
+    """Reference ft_apply_*_pipeline on copies of the two (time, freq) planes; quant =
+    (base, step, mix, decay), contrast = (kernel_time, kernel_freq, brightness, contrast,
+    gamma, clamp_zero). Returns (ok, a, b)."""
+    a = np.ascontiguousarray(a, dtype=np.float32).copy()
+    b = np.ascontiguousarray(b, dtype=np.float32).copy()
+    q = quant or (0.0, 0.0, 0.0, 0.0)
+    c = contrast or (0, 0, 0.0, 0.0, 1.0, 0)
+    ok = lib().ref_ft_pipeline(int(polar), freq_origin, freq_step, up[0], up[1], down[0], down[1], int(preserve),
+                               int(quant is not None), *q, int(contrast is not None), c[0], c[1], c[2], c[3], c[4],
+                               int(c[5]), _p(a), _p(b), a.shape[0], a.shape[1])
+    return int(ok), a, b
 
 
 def metadata(N: int, kind: int, kernel: int = 0, inverse=False, transform=0, half_spectrum=False) -> np.ndarray:

@@ -12,12 +12,14 @@ import pytest
 from fftfree_amd import _lib
 from oracle import fp64
 
-HEADER = Path(__file__).resolve().parents[1] / "include" / "fft_cffi.h"
+HEADERS = [Path(__file__).resolve().parents[1] / "include" / name for name in ("fft_cffi.h", "ft_tools_cffi.h")]
 
 
 def declared_functions():
-    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    names = re.findall(r"\b(?:void\s*\*|void|int|size_t)\s+((?:fft|phase)_[a-z0-9_]+)\s*\(", text)
+    names = []
+    for header in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names += re.findall(r"\b(?:void\s*\*|void|int|size_t)\s+((?:fft|phase|ft)_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
 
 
