@@ -60,6 +60,8 @@ SIGNATURES = {
                                                       c_void_p]),
     "ft_apply_complex_pipeline": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, f32p, f32p, c_int, c_int]),
     "ft_apply_polar_pipeline": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, f32p, f32p, c_int, c_int]),
+    "ft_apply_complex_pipeline_device": (c_int, [c_void_p] * 8 + [c_int, c_int, c_void_p]),
+    "ft_apply_polar_pipeline_device": (c_int, [c_void_p] * 8 + [c_int, c_int, c_void_p]),
     "phase_init": (c_void_p, [c_int, c_int, c_int, c_int, c_int]),
     "phase_infer_execute": (c_size_t, [c_void_p, f32p, c_size_t, f32p, f32p]),
     "phase_free": (None, [c_void_p]),

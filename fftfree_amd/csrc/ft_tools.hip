@@ -101,12 +101,15 @@ __device__ float quant_weight(const FtQuantizer& q, float freq) {
   return fminf(fmaxf(q.mix * attraction, 0.0f), 1.0f);
 }
 
-__device__ float quant_magnitude(const FtQuantizer& q, float m, float freq) {
-  const float w = quant_weight(q, freq);
+__device__ __forceinline__ float quant_apply(const FtQuantizer& q, float m, float w) {
   if (w <= 0.0f) return m;
   float target = m;
   if (q.magnitude_step > 0.0f) target = (float)round((double)(m / q.magnitude_step)) * q.magnitude_step;
   return lerpf(m, target, w);
+}
+
+__device__ float quant_magnitude(const FtQuantizer& q, float m, float freq) {
+  return quant_apply(q, m, quant_weight(q, freq));
 }
 
 __global__ void __launch_bounds__(256) ft_quantize_kernel(float2* __restrict__ z, float* __restrict__ mag, int tb,
@@ -121,7 +124,7 @@ __global__ void __launch_bounds__(256) ft_quantize_kernel(float2* __restrict__ z
     } else {    // complex grid: |z| moved, the phase kept; tiny values untouched
       const float2 v = z[i];
       const float m = hypotf(v.x, v.y);
-      if (m <= 1.1920928955078125e-07f) continue;  // FLT_EPSILON
+      if (m <= kFtEpsilon) continue;
       const float adj = fmaxf(quant_magnitude(q, m, freq), 0.0f);
       float s, c;
       sincosf(atan2f(v.y, v.x), &s, &c);
@@ -198,6 +201,123 @@ __global__ void __launch_bounds__(256) ft_set_magnitude_kernel(float2* __restric
   }
 }
 
+// Contrast adjustment of one magnitude about its box mean (ft_tools.hpp:436-452).
+__device__ __forceinline__ float contrast_adjust(const FtContrast& c, float m, float mean, bool use_gamma,
+                                                 float inv_gamma) {
+  float adj = mean + (m - mean) * (1.0f + c.contrast) + c.brightness;
+  if (c.clamp_zero) adj = fmaxf(adj, 0.0f);
+  if (use_gamma) adj = (float)pow((double)fmaxf(adj, 1e-12f), (double)inv_gamma);
+  return adj;
+}
+
+__device__ __forceinline__ int xcd_order(int i, int G) {  // contiguous tile runs per XCD
+  const int q = G >> 3, rem = G & 7;
+  const int x = i & 7, s = i >> 3;
+  return x * q + (x < rem ? x : rem) + s;
+}
+
+// The pipeline without resampling in one pass: quantizer, then the local contrast
+// booster, read from and written to planes. A workgroup owns a tile of kFtTileT x
+// kFtTileF cells; the quantized magnitudes of the tile and its box-stencil halo go to LDS
+// once, and each output cell takes its box mean there. Complex grids: the result is
+// polar(m', arg z_q) with z_q = polar(adj, arg z), as the two units compute it one after
+// the other. Tiles are dealt to XCDs in runs along time, so vertical halos hit L2.
+template <bool POLAR>
+__global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const float* in_b, float* out_a, float* out_b,
+                                                       int tb, int fb, FtQuantizer q, int use_q, double freq_origin,
+                                                       double freq_step, FtContrast c, int use_c, int tiles_t,
+                                                       int tiles_f) {
+  __shared__ float mq[(kFtTileT + 2 * kFtMaxRadiusT) * (kFtTileF + 2 * kFtMaxRadiusF)];
+  __shared__ float wcol[kFtTileF + 2 * kFtMaxRadiusF];
+  const int tid = threadIdx.x;
+  const int tile = xcd_order(blockIdx.x, tiles_t * tiles_f);
+  const int tfi = tile / tiles_t, tti = tile - tfi * tiles_t;
+  const int t0 = tti * kFtTileT, f0 = tfi * kFtTileF;
+  const int rt = use_c ? max(c.kernel_time / 2, 0) : 0;
+  const int rf = use_c ? max(c.kernel_freq / 2, 0) : 0;
+  const int er = kFtTileT + 2 * rt, ec = kFtTileF + 2 * rf;
+  for (int j = tid; j < ec; j += 256) {
+    const int f = f0 - rf + j;
+    wcol[j] = (use_q && f >= 0 && f < fb) ? quant_weight(q, (float)(freq_origin + freq_step * (double)f)) : 0.0f;
+  }
+  __syncthreads();
+  for (int i = tid; i < er * ec; i += 256) {
+    const int r = i / ec, j = i - r * ec;
+    const int t = t0 - rt + r, f = f0 - rf + j;
+    float v = 0.0f;
+    if (t >= 0 && t < tb && f >= 0 && f < fb) {
+      const size_t at = (size_t)t * fb + f;
+      const float w = wcol[j];
+      if (POLAR) {
+        v = quant_apply(q, in_a[at], w);
+      } else {
+        const float m = hypotf(in_a[at], in_b[at]);
+        v = (w > 0.0f && m > kFtEpsilon) ? fmaxf(quant_apply(q, m, w), 0.0f) : m;
+      }
+    }
+    mq[i] = v;
+  }
+  __syncthreads();
+  const float gamma = c.gamma > 0.0f ? c.gamma : 1.0f;
+  const bool use_gamma = fabsf(gamma - 1.0f) > 1e-5f;
+  const float inv_gamma = use_gamma ? 1.0f / gamma : 1.0f;
+  const int j = tid & (kFtTileF - 1);
+  const int f = f0 + j;
+  if (f >= fb) return;
+  const int f_lo = max(0, f - rf) - f0 + rf, f_hi = min(fb - 1, f + rf) - f0 + rf;
+  for (int r = tid / kFtTileF; r < kFtTileT; r += 256 / kFtTileF) {
+    const int t = t0 + r;
+    if (t >= tb) break;
+    const float mq0 = mq[(r + rt) * ec + j + rf];
+    float mc = mq0;
+    if (use_c) {
+      const int t_lo = max(0, t - rt) - t0 + rt, t_hi = min(tb - 1, t + rt) - t0 + rt;
+      float sum = 0.0f;
+      for (int a = t_lo; a <= t_hi; ++a)
+        for (int b = f_lo; b <= f_hi; ++b) sum += mq[a * ec + b];
+      const float mean = sum / (float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1));
+      mc = contrast_adjust(c, mq0, mean, use_gamma, inv_gamma);
+    }
+    const size_t at = (size_t)t * fb + f;
+    if (POLAR) {
+      out_a[at] = mc;
+      if (out_b != in_b) out_b[at] = in_b[at];
+    } else {
+      const float x = in_a[at], y = in_b[at];
+      const bool quantized = wcol[j + rf] > 0.0f && hypotf(x, y) > kFtEpsilon;
+      float sn, cs;
+      if (!quantized && !use_c) {
+        out_a[at] = x;
+        out_b[at] = y;
+        continue;
+      }
+      sincosf(atan2f(y, x), &sn, &cs);
+      if (quantized && use_c && mq0 == 0.0f) {
+        // z_q is a signed zero: its argument is 0, +-0 or +-pi
+        sincosf(atan2f(mq0 * sn, mq0 * cs), &sn, &cs);
+      }
+      out_a[at] = mc * cs;
+      out_b[at] = mc * sn;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) ft_planes_to_complex_kernel(const float* __restrict__ a,
+                                                                   const float* __restrict__ b, size_t total,
+                                                                   float2* __restrict__ z) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
+    z[i] = make_float2(a[i], b[i]);
+}
+
+__global__ void __launch_bounds__(256) ft_complex_to_planes_kernel(const float2* __restrict__ z, size_t total,
+                                                                   float* __restrict__ a, float* __restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    const float2 v = z[i];
+    a[i] = v.x;
+    b[i] = v.y;
+  }
+}
+
 unsigned grid_of(size_t work) {
   const size_t g = (work + 255) / 256;
   return (unsigned)(g == 0 ? 1 : (g > 65536 ? 65536 : g));
@@ -251,6 +371,37 @@ hipError_t ft_contrast(float2* z, float* mag, int tb, int fb, const FtContrast& 
     const hipError_t e = hipMemcpyAsync(mag, out, total * sizeof(float), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
   }
+  return hipGetLastError();
+}
+
+bool ft_fused_supported(int use_c, const FtContrast& c) {
+  if (!use_c) return true;
+  return c.kernel_time > 0 && c.kernel_freq > 0 && c.kernel_time / 2 <= kFtMaxRadiusT &&
+         c.kernel_freq / 2 <= kFtMaxRadiusF;
+}
+
+hipError_t ft_fused(bool polar, const float* in_a, const float* in_b, float* out_a, float* out_b, int tb, int fb,
+                    const FtQuantizer& q, int use_q, double freq_origin, double freq_step, const FtContrast& c,
+                    int use_c, hipStream_t s) {
+  const int tiles_t = (tb + kFtTileT - 1) / kFtTileT, tiles_f = (fb + kFtTileF - 1) / kFtTileF;
+  if ((long long)tiles_t * tiles_f > 0x7fffffffll) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)(tiles_t * tiles_f));
+  if (polar)
+    hipLaunchKernelGGL(ft_fused_kernel<true>, grid, dim3(256), 0, s, in_a, in_b, out_a, out_b, tb, fb, q, use_q,
+                       freq_origin, freq_step, c, use_c, tiles_t, tiles_f);
+  else
+    hipLaunchKernelGGL(ft_fused_kernel<false>, grid, dim3(256), 0, s, in_a, in_b, out_a, out_b, tb, fb, q, use_q,
+                       freq_origin, freq_step, c, use_c, tiles_t, tiles_f);
+  return hipGetLastError();
+}
+
+hipError_t ft_planes_to_complex(const float* a, const float* b, size_t total, float2* z, hipStream_t s) {
+  hipLaunchKernelGGL(ft_planes_to_complex_kernel, dim3(grid_of(total)), dim3(256), 0, s, a, b, total, z);
+  return hipGetLastError();
+}
+
+hipError_t ft_complex_to_planes(const float2* z, size_t total, float* a, float* b, hipStream_t s) {
+  hipLaunchKernelGGL(ft_complex_to_planes_kernel, dim3(grid_of(total)), dim3(256), 0, s, z, total, a, b);
   return hipGetLastError();
 }
 

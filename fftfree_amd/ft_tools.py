@@ -4,7 +4,8 @@ The structs and the call follow the reference's pipeline tool, which fills the s
 structs and calls ft_apply_complex_pipeline on the STFT's real/imag planes
 (tools/ft_tools_pipeline.py:359-401). Grids are (time_bins, freq_bins) float32 arrays,
 changed in place on success; a call that fails (a result of another shape, NULL planes,
-an empty grid) returns False and leaves them as they were.
+an empty grid) returns False and leaves them as they were. The ``*_device`` forms take
+torch tensors on the GPU and enqueue on torch's current stream.
 """
 from __future__ import annotations
 
@@ -94,3 +95,37 @@ def apply_polar_pipeline(magnitude: np.ndarray, phase: np.ndarray, spec: Optiona
                          contrast_cfg: Optional[ContrastConfig] = None) -> bool:
     """ft_apply_polar_pipeline on (time, freq) magnitude/phase planes, in place."""
     return _apply(_lib.load().ft_apply_polar_pipeline, magnitude, phase, spec, plan, quant, contrast_cfg)
+
+
+def _device_args(a, b, out_a, out_b):
+    for t in (a, b, out_a, out_b):
+        if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 4 and t.is_contiguous() and t.dim() == 2):
+            raise TypeError("device planes must be contiguous 2-D float32 CUDA tensors (time_bins, freq_bins)")
+        if t.shape != a.shape:
+            raise ValueError(f"plane shapes differ: {tuple(t.shape)} vs {tuple(a.shape)}")
+    return a.data_ptr(), b.data_ptr(), out_a.data_ptr(), out_b.data_ptr(), int(a.shape[0]), int(a.shape[1])
+
+
+def _stream(stream):
+    import torch
+    return (stream or torch.cuda.current_stream()).cuda_stream
+
+
+def apply_complex_pipeline_device(real, imag, out_real=None, out_imag=None, spec=None, plan=None, quant=None,
+                                  contrast_cfg=None, stream=None) -> bool:
+    """ft_apply_complex_pipeline_device: out planes default to the input planes (in place)."""
+    out_real = real if out_real is None else out_real
+    out_imag = imag if out_imag is None else out_imag
+    args = _device_args(real, imag, out_real, out_imag)
+    return bool(_lib.load().ft_apply_complex_pipeline_device(_ref(spec), _ref(plan), _ref(quant), _ref(contrast_cfg),
+                                                             *args, _stream(stream)))
+
+
+def apply_polar_pipeline_device(magnitude, phase, out_magnitude=None, out_phase=None, spec=None, plan=None,
+                                quant=None, contrast_cfg=None, stream=None) -> bool:
+    """ft_apply_polar_pipeline_device: out planes default to the input planes (in place)."""
+    out_magnitude = magnitude if out_magnitude is None else out_magnitude
+    out_phase = phase if out_phase is None else out_phase
+    args = _device_args(magnitude, phase, out_magnitude, out_phase)
+    return bool(_lib.load().ft_apply_polar_pipeline_device(_ref(spec), _ref(plan), _ref(quant), _ref(contrast_cfg),
+                                                           *args, _stream(stream)))

@@ -57,6 +57,23 @@ int ft_apply_polar_pipeline(const struct ft_grid_spec* spec, const struct ft_res
                             const struct ft_quantizer_config* quant, const struct ft_contrast_config* contrast,
                             float* magnitude, float* phase, int time_bins, int freq_bins);
 
+/* Additive (no counterpart in the reference): the same pipelines on device planes,
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream). The output planes may
+ * be the input planes (in place) or separate buffers; other overlaps are not allowed.
+ * Without resampling and with a local contrast box of at most 17 x 33 cells (or none),
+ * the pipeline is one launch; otherwise it runs step by step on a workspace taken from
+ * the stream-ordered allocator. Returns 1 once the work is enqueued, 0 on the argument
+ * and shape errors of the host forms. */
+int ft_apply_complex_pipeline_device(const struct ft_grid_spec* spec, const struct ft_resample_plan_c* plan,
+                                     const struct ft_quantizer_config* quant,
+                                     const struct ft_contrast_config* contrast, const float* d_real,
+                                     const float* d_imag, float* d_out_real, float* d_out_imag, int time_bins,
+                                     int freq_bins, void* stream);
+int ft_apply_polar_pipeline_device(const struct ft_grid_spec* spec, const struct ft_resample_plan_c* plan,
+                                   const struct ft_quantizer_config* quant, const struct ft_contrast_config* contrast,
+                                   const float* d_magnitude, const float* d_phase, float* d_out_magnitude,
+                                   float* d_out_phase, int time_bins, int freq_bins, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,7 +62,9 @@ def _run(polar, a, b, freq_step=1.0, up=(1, 1), down=(1, 1), preserve=True, quan
 
 # (time bins, freq bins, up, down, preserve, quant, contrast): the reference tool's defaults
 # (quantizer 55 Hz / 0.05 / 0.5 / 8, 3x3 contrast 0.5), STFT-shaped grids of N = 1024 and
-# 2048, ragged downsampling blocks (a grid of at most u rows upsampled by u and
+# 2048, the largest contrast box of the one-pass kernel (17 x 33) and one past it, a
+# quantizer that sets many cells to zero ahead of an unclamped contrast, ragged
+# downsampling blocks (a grid of at most u rows upsampled by u and
 # downsampled by u + 1 keeps its shape with a short last block), the global and semi-global contrast modes, gamma.
 CASES = [
     (256, 513, (1, 1), (1, 1), True, (55.0, 0.05, 0.5, 8.0), (3, 3, 0.0, 0.5, 1.0, 1)),
@@ -74,6 +76,9 @@ CASES = [
     (33, 47, (3, 1), (3, 1), True, (0.0, 0.02, 1.0, 8.0), (0, 0, -0.1, 0.8, 0.5, 0)),
     (50, 70, (2, 2), (2, 2), True, (55.0, 0.05, 0.5, 8.0), (0, 3, 0.0, 0.5, 1.0, 1)),
     (17, 9, (1, 1), (1, 1), True, None, (4, 0, 0.2, -0.5, 1.0, 1)),
+    (70, 300, (1, 1), (1, 1), True, (55.0, 0.05, 0.5, 8.0), (17, 33, 0.0, 0.5, 1.0, 1)),
+    (70, 300, (1, 1), (1, 1), True, (55.0, 0.05, 0.5, 8.0), (19, 35, 0.05, 0.5, 1.5, 1)),
+    (37, 260, (1, 1), (1, 1), True, (0.0, 0.5, 1.0, 8.0), (3, 3, 0.0, 2.0, 1.0, 0)),
     (1, 1, (3, 3), (3, 3), True, (55.0, 0.05, 0.5, 8.0), (3, 3, 0.0, 0.5, 1.0, 1)),
 ]
 
@@ -128,3 +133,59 @@ def test_disabled_units_are_skipped():
     ga, gb = a.copy(), b.copy()
     assert ft.apply_complex_pipeline(ga, gb, None, None, ft.quantizer(enabled=False), ft.contrast(enabled=False))
     assert np.array_equal(ga, a) and np.array_equal(gb, b)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+DEVICE_CASES = [
+    ((1, 1), (1, 1), (55.0, 0.05, 0.5, 8.0), (3, 3, 0.0, 0.5, 1.0, 1)),  # one pass
+    ((1, 1), (1, 1), (55.0, 0.05, 0.5, 8.0), None),  # one pass, elementwise
+    ((1, 1), (1, 1), None, (0, 0, 0.0, 0.5, 1.0, 1)),  # global mean: step by step
+    ((2, 2), (2, 2), (55.0, 0.05, 0.5, 8.0), (3, 3, 0.0, 0.5, 1.0, 1)),  # resampling: step by step
+]
+
+
+@pytest.mark.parametrize("polar", [False, True], ids=["complex", "polar"])
+@pytest.mark.parametrize("case", DEVICE_CASES, ids=["fused", "fused-elementwise", "global", "resampled"])
+def test_device_entry_matches_host_entry(polar, case):
+    """The device form gives the host form's planes bit for bit: out of place, in place,
+    and (polar) with the phase plane alone in place."""
+    torch = _torch()
+    up, down, quant, contrast = case
+    print(f"ft device entry polar={polar} up={up} down={down} quant={quant} contrast={contrast}", flush=True)
+    a, b = _grid(96, 513, 77, "polar" if polar else "complex")
+    spec = ft.grid_spec(freq_step=44100.0 / 1024.0)
+    plan = ft.resample_plan(up, down)
+    q = ft.quantizer(*quant) if quant is not None else None
+    c = ft.contrast(*contrast) if contrast is not None else None
+    ha, hb = a.copy(), b.copy()
+    host = ft.apply_polar_pipeline if polar else ft.apply_complex_pipeline
+    dev = ft.apply_polar_pipeline_device if polar else ft.apply_complex_pipeline_device
+    assert host(ha, hb, spec, plan, q, c)
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    oa, ob = torch.full_like(da, float("nan")), torch.full_like(db, float("nan"))
+    assert dev(da, db, oa, ob, spec, plan, q, c)
+    torch.cuda.synchronize()
+    assert np.array_equal(oa.cpu().numpy(), ha) and np.array_equal(ob.cpu().numpy(), hb)
+    assert np.array_equal(da.cpu().numpy(), a) and np.array_equal(db.cpu().numpy(), b)  # inputs untouched
+    ia, ib = da.clone(), db.clone()
+    assert dev(ia, ib, None, None, spec, plan, q, c)
+    torch.cuda.synchronize()
+    assert np.array_equal(ia.cpu().numpy(), ha) and np.array_equal(ib.cpu().numpy(), hb)
+    if polar:
+        pa, pb = torch.full_like(da, float("nan")), db.clone()
+        assert dev(da, pb, pa, pb, spec, plan, q, c)
+        torch.cuda.synchronize()
+        assert np.array_equal(pa.cpu().numpy(), ha) and np.array_equal(pb.cpu().numpy(), hb)
+
+
+def test_device_entry_rejects_shape_change():
+    torch = _torch()
+    a = torch.zeros((8, 8), device="cuda")
+    b = torch.zeros((8, 8), device="cuda")
+    assert not ft.apply_complex_pipeline_device(a, b, None, None, None, ft.resample_plan((2, 2), (1, 1)))
+    lib = ft._lib.load()
+    assert lib.ft_apply_polar_pipeline_device(None, None, None, None, None, None, None, None, 8, 8, None) == 0
