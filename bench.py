@@ -25,8 +25,11 @@ overlap-add with COLA normalisation into 65533*1024 output samples, zeros past t
 overlap-add included, as the reference writes them); griffin_lim (fft_griffin_lim_device
 on the magnitude spectrogram of 2^20 samples, N = 1024, hop 256, 4093 frames, 32
 iterations; value = iterations/s from the 32-iteration call minus the 1-iteration call,
-each call synchronised);
-copy
+each call synchronised); phase (phase_infer_execute_device on the default workload's
+|X| planes: mode 0, the linear running phase, timed with its roofline, and mode 1,
+minimum phase, timed alongside); ft_pipeline (ft_apply_complex_pipeline_device on the
+default workload's complex planes with the reference tool's default quantizer and 3x3
+contrast, no resampling: the one-pass kernel); copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
 WRITE_SIZE counters).
 """
@@ -49,8 +52,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "c2c1024", "c2c_large",
-                                                     "stft_grad", "ragged", "copy"])
+    ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "phase", "ft_pipeline",
+                                                     "c2c1024", "c2c_large", "stft_grad", "ragged", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -105,6 +108,25 @@ def cpu_baseline_stft(pcm, n, W, H, seconds):
     return {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"{runs} full runs of the 2^24-sample workload (65533 frames each), Stockham code 2, "
                       f"restated reference CPU path, {threads} threads, {el:.1f} s"}
+
+
+def stft_grid(ff, torch):
+    """The headline STFT's planes (Hann 1024, hop 256, 2^24 samples) on the GPU, the input
+    of the phase-inference and grid-pipeline workloads."""
+    from fftfree_amd.workloads import noise_plus_sines
+    n, H, L = 1024, 256, 1 << 24
+    p = ff.make_plan(n, window=n, hop=H, kernel=2, transform=1, half_spectrum=True, apply_windows=True,
+                     analysis_window=1, pad_mode=0)
+    frames = ff.frame_count(p, L, 0)
+    bins = n // 2 + 1
+    re = torch.empty((frames, bins), device="cuda")
+    im = torch.empty((frames, bins), device="cuda")
+    mag = torch.empty((frames, bins), device="cuda")
+    ff.stft_device(p, torch.from_numpy(noise_plus_sines(L, seed=2)).cuda(), re, im, mag, pad_mode=0)
+    out_a, out_b = torch.empty_like(re), torch.empty_like(re)
+    common = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+              "data": "synthetic: STFT planes (Hann 1024, hop 256) of 0.1*U[-1,1] noise + 8 sinusoids, seed 2"}
+    return n, H, frames, bins, frames * bins, re, im, mag, out_a, out_b, common
 
 
 def main():
@@ -266,6 +288,90 @@ def main():
             line["cpu_baseline"] = {"value": done / el, "unit": "iterations/s", "cores": ref.default_threads(),
                                     "kind": "port", "sample": f"{done} iterations of the restated reference loop "
                                                               f"on the same spectrogram, {el:.1f} s"}
+    elif args.workload == "phase":
+        n, H, frames, bins, cells, re, im, mag, out_a, out_b, common = stft_grid(ff, torch)
+        lib = ff.lib
+        handles = {m: lib.phase_init(n, H, 1, m, 0) for m in (0, 1)}
+        assert all(handles.values())
+
+        def run(m):
+            return lambda: lib.phase_infer_execute_device(handles[m], mag.data_ptr(), frames, out_a.data_ptr(),
+                                                          out_b.data_ptr(), torch.cuda.current_stream().cuda_stream)
+
+        gpu_s, wall = timed(run(0), args.steps, args.warmup, torch)
+        per_launch = gpu_s / args.steps
+        gpu1_s, _ = timed(run(1), max(1, args.steps // 4), max(1, args.warmup // 4), torch)
+        per1 = gpu1_s / max(1, args.steps // 4)
+        alg_bytes = cells * (4 + 8)
+        achieved = alg_bytes / per_launch / 1e9
+        common.update(steps=args.steps, warmup=args.warmup)
+        line = dict(common, **{
+            "metric": "phase inference frames/s (phase_infer_execute_device, linear phase), N=1024 hop 256",
+            "value": frames * args.steps / gpu_s, "unit": "frames/s", "ms_per_step": per_launch * 1e3,
+            "config": {"workload": "phase_infer_execute mode 0 (linear running phase, double), 65533 frames x "
+                                   "513 bins |X| -> (re, im); mode 1 (minimum phase: log, inverse C2C 1024, fold, "
+                                   "forward C2C, exp) timed alongside",
+                       "n": n, "hop": H, "frames": frames, "mode1_ms_per_call": per1 * 1e3,
+                       "mode1_frames_per_s": frames / per1},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg_bytes,
+                         "kernel": "phase_linear_kernel"}})
+        if not args.no_cpu:
+            from oracle import ref
+            sample = mag[:4096].cpu().numpy()
+            t0 = time.perf_counter()
+            done = 0
+            while True:
+                ref.phase_infer(sample, n, H, half=True, mode=0)
+                done += sample.shape[0]
+                el = time.perf_counter() - t0
+                if el >= args.cpu_seconds:
+                    break
+            line["cpu_baseline"] = {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
+                                    "sample": f"{done} frames (4096-frame slices of the same |X|) through the "
+                                              f"restated mode-0 loop, 1 thread, {el:.1f} s"}
+        for h in handles.values():
+            lib.phase_free(h)
+    elif args.workload == "ft_pipeline":
+        n, H, frames, bins, cells, re, im, mag, out_a, out_b, common = stft_grid(ff, torch)
+        from fftfree_amd import ft_tools as ft
+        spec = ft.grid_spec(freq_step=44100.0 / n)
+        quant, contrast = ft.quantizer(), ft.contrast()
+
+        def step():
+            assert ft.apply_complex_pipeline_device(re, im, out_a, out_b, spec, None, quant, contrast)
+
+        gpu_s, wall = timed(step, args.steps, args.warmup, torch)
+        per_launch = gpu_s / args.steps
+        alg_bytes = cells * 16
+        achieved = alg_bytes / per_launch / 1e9
+        common.update(steps=args.steps, warmup=args.warmup)
+        line = dict(common, **{
+            "metric": "grid pipeline frames/s (ft_apply_complex_pipeline_device: harmonic quantizer + 3x3 local "
+                      "contrast), 513 bins",
+            "value": frames * args.steps / gpu_s, "unit": "frames/s", "ms_per_step": per_launch * 1e3,
+            "config": {"workload": "ft pipeline on the complex STFT grid, 65533 time bins x 513 freq bins, "
+                                   "quantizer 55 Hz / step 0.05 / mix 0.5 / decay 8, contrast 3x3 gain 0.5, "
+                                   "freq step 44100/1024, no resampling (one pass)",
+                       "frames": frames, "bins": bins, "cells": cells},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("ft_fused_65533x513"),
+                         "algorithmic_bytes_per_launch": alg_bytes, "kernel": "ft_fused_kernel<false>"}})
+        if not args.no_cpu:
+            from oracle import ref
+            ra, rb = re[:2048].cpu().numpy(), im[:2048].cpu().numpy()
+            t0 = time.perf_counter()
+            done = 0
+            while True:
+                ref.ft_pipeline(ra, rb, freq_step=44100.0 / n, quant=(55.0, 0.05, 0.5, 8.0),
+                                contrast=(3, 3, 0.0, 0.5, 1.0, 1))
+                done += ra.shape[0]
+                el = time.perf_counter() - t0
+                if el >= args.cpu_seconds:
+                    break
+            line["cpu_baseline"] = {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
+                                    "sample": f"{done} frames (2048 x 513 slices of the same grid) through the "
+                                              f"restated pipeline, 1 thread, {el:.1f} s"}
     elif args.workload == "c2c_large":
         n, B = 1 << args.lg, 16
         fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)

@@ -1240,10 +1240,7 @@ size_t phase_execute(PhaseCtx* c, const float* d_mag, size_t frames, float* d_re
   const int N = c->n;
   const int bins = c->half ? N / 2 + 1 : N;
   if (c->mode == 0) {
-    const size_t words = phase_linear_scratch(frames, bins);
-    c->scratch.reserve(words * sizeof(double));
-    hip_check(launch_phase_linear(d_mag, frames, bins, N, c->hop, c->scratch.as<double>(), d_re, d_im, s),
-              "launch_phase_linear");
+    hip_check(launch_phase_linear(d_mag, frames, bins, N, c->hop, d_re, d_im, s), "launch_phase_linear");
     return frames;
   }
   if (c->mode != 1 || !c->inv || !c->fwd) return 0;  // unknown mode, or N with no plan

@@ -129,12 +129,11 @@ hipError_t launch_gl_overlap(const float* frames_pcm, size_t frames, int n, int 
                              hipStream_t s);
 
 // Phase inference (phase_infer_execute, fft_cffi.cpp:2072-2162). Mode 0: the linear phase
-// of phase_infer.hpp:36-66, through a scratch of phase_linear_scratch() doubles. Mode 1:
+// of phase_infer.hpp:36-66, one launch, no scratch. Mode 1:
 // log spectrum into frames x N complex columns, then (after the inverse DFT) the causal
 // cepstrum fold, then (after the forward DFT) exp into the planes.
-size_t phase_linear_scratch(size_t frames, int bins);
-hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, double* ckpt, float* o_re,
-                               float* o_im, hipStream_t s);
+hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, float* o_re, float* o_im,
+                               hipStream_t s);
 hipError_t launch_phase_log(const float* mag, size_t frames, int bins, int n, int half, float2* cols, hipStream_t s);
 hipError_t launch_phase_fold(size_t frames, int n, float2* cols, hipStream_t s);
 hipError_t launch_phase_exp(const float2* cols, size_t frames, int bins, int n, float* o_re, float* o_im,

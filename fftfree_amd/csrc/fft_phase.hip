@@ -4,8 +4,9 @@
 // Mode 0 (infer_linear, phase_infer.hpp:36-66): phi[f][k] is the double-precision running
 // sum of dphi[k] = 2 pi kk hop / N over the frames before f (kk = k mod N), and
 // out = (float)(m cos phi), (float)(m sin phi) in double. The running sum is reproduced
-// exactly: a scan per bin writes phi at every 64th frame, then one thread per (64-frame
-// block, bin) continues the same additions.
+// exactly: one thread per (64-frame block, bin) finds phi at its first frame in closed
+// form (running_sum, phase_sum.hpp: a constant step per binade) and continues the same
+// additions.
 //
 // Mode 1 (minimum phase, fft_cffi.cpp:2095-2158): per frame the real-even log spectrum
 // log(max(1e-20, m)) (mirrored for a half spectrum), its inverse DFT (the real cepstrum),
@@ -14,6 +15,7 @@
 // the three elementwise steps are here.
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
+#include "phase_sum.hpp"
 
 namespace fftfree {
 
@@ -26,28 +28,16 @@ __device__ __forceinline__ double linear_dphi(int k, int n, int hop) {
   return 6.28318530717958647692 * (double)kk * (double)hop / (double)n;  // phase_infer.hpp:47-51
 }
 
-__global__ void __launch_bounds__(256) phase_linear_scan_kernel(int bins, size_t blocks, int n, int hop,
-                                                                double* __restrict__ ckpt) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= bins) return;
-  const double d = linear_dphi(k, n, hop);
-  double phi = 0.0;
-  for (size_t b = 0; b < blocks; ++b) {
-    ckpt[b * (size_t)bins + k] = phi;
-    for (int i = 0; i < kPhaseBlock; ++i) phi += d;
-  }
-}
-
 __global__ void __launch_bounds__(256) phase_linear_kernel(const float* __restrict__ mag, size_t frames, int bins,
-                                                           int n, int hop, const double* __restrict__ ckpt,
-                                                           float* __restrict__ o_re, float* __restrict__ o_im) {
+                                                           int n, int hop, float* __restrict__ o_re,
+                                                           float* __restrict__ o_im) {
   const size_t blocks = (frames + kPhaseBlock - 1) / kPhaseBlock;
   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (idx >= blocks * (size_t)bins) return;
   const size_t b = idx / (size_t)bins;
   const int k = (int)(idx - b * (size_t)bins);
   const double d = linear_dphi(k, n, hop);
-  double phi = ckpt[idx];
+  double phi = running_sum(d, (uint64_t)(b * kPhaseBlock));
   for (int i = 0; i < kPhaseBlock; ++i) {
     const size_t f = b * kPhaseBlock + i;
     if (f >= frames) break;
@@ -115,20 +105,14 @@ unsigned grid_for(size_t work) {
 
 }  // namespace
 
-size_t phase_linear_scratch(size_t frames, int bins) {
-  return ((frames + kPhaseBlock - 1) / kPhaseBlock) * (size_t)bins;  // doubles
-}
-
-hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, double* ckpt, float* o_re,
-                               float* o_im, hipStream_t s) {
+hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, float* o_re, float* o_im,
+                               hipStream_t s) {
   if (frames == 0 || bins <= 0) return hipSuccess;
   const size_t blocks = (frames + kPhaseBlock - 1) / kPhaseBlock;
-  hipLaunchKernelGGL(phase_linear_scan_kernel, dim3((unsigned)((bins + 255) / 256)), dim3(256), 0, s, bins, blocks, n,
-                     hop, ckpt);
   const size_t work = blocks * (size_t)bins;
   if ((work + 255) / 256 > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(phase_linear_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, mag, frames, bins, n,
-                     hop, ckpt, o_re, o_im);
+                     hop, o_re, o_im);
   return hipGetLastError();
 }
 

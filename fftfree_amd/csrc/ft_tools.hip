@@ -95,7 +95,7 @@ __device__ float quant_weight(const FtQuantizer& q, float freq) {
   if (q.base_frequency <= 0.0f) return fminf(fmaxf(q.mix, 0.0f), 1.0f);
   const float ratio = freq / q.base_frequency;
   if (!isfinite((double)ratio)) return 0.0f;
-  const float nearest = (float)round((double)ratio);  // halves away from zero, as std::round
+  const float nearest = roundf(ratio);  // halves away from zero: std::round on the double, exactly
   const float distance = fabsf(ratio - nearest);
   const float attraction = expf(-q.distance_decay * distance);
   return fminf(fmaxf(q.mix * attraction, 0.0f), 1.0f);
@@ -104,7 +104,7 @@ __device__ float quant_weight(const FtQuantizer& q, float freq) {
 __device__ __forceinline__ float quant_apply(const FtQuantizer& q, float m, float w) {
   if (w <= 0.0f) return m;
   float target = m;
-  if (q.magnitude_step > 0.0f) target = (float)round((double)(m / q.magnitude_step)) * q.magnitude_step;
+  if (q.magnitude_step > 0.0f) target = roundf(m / q.magnitude_step) * q.magnitude_step;
   return lerpf(m, target, w);
 }
 
@@ -218,16 +218,24 @@ __device__ __forceinline__ int xcd_order(int i, int G) {  // contiguous tile run
 
 // The pipeline without resampling in one pass: quantizer, then the local contrast
 // booster, read from and written to planes. A workgroup owns a tile of kFtTileT x
-// kFtTileF cells; the quantized magnitudes of the tile and its box-stencil halo go to LDS
-// once, and each output cell takes its box mean there. Complex grids: the result is
-// polar(m', arg z_q) with z_q = polar(adj, arg z), as the two units compute it one after
-// the other. Tiles are dealt to XCDs in runs along time, so vertical halos hit L2.
-template <bool POLAR>
+// kFtTileF cells. Each thread first issues every load it needs (its share of the tile
+// plus the box-stencil halo, and the raw values of its own output cells), then the
+// quantized magnitudes go to LDS once, and each output cell takes its box mean there.
+// Complex grids: the result is polar(m', arg z_q) with z_q = polar(adj, arg z), as the
+// two units compute it one after the other; arg z_q = arg z except where adj is 0, when
+// z_q is a signed zero. Tiles are dealt to XCDs in runs along time, so vertical halos
+// hit L2.
+template <bool POLAR, int KPER>
 __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const float* in_b, float* out_a, float* out_b,
                                                        int tb, int fb, FtQuantizer q, int use_q, double freq_origin,
                                                        double freq_step, FtContrast c, int use_c, int tiles_t,
                                                        int tiles_f) {
-  __shared__ float mq[(kFtTileT + 2 * kFtMaxRadiusT) * (kFtTileF + 2 * kFtMaxRadiusF)];
+  constexpr int kExt = (kFtTileT + 2 * kFtMaxRadiusT) * (kFtTileF + 2 * kFtMaxRadiusF);
+  constexpr int kPer = KPER;                         // halo-tile cells per thread, at most
+  constexpr int kOut = kFtTileT * kFtTileF / 256;    // output cells per thread
+  constexpr int kRowStep = 256 / kFtTileF;
+  static_assert(kExt % 256 == 0 && (kFtTileT * kFtTileF) % 256 == 0, "tile shape");
+  __shared__ float mq[kExt];
   __shared__ float wcol[kFtTileF + 2 * kFtMaxRadiusF];
   const int tid = threadIdx.x;
   const int tile = xcd_order(blockIdx.x, tiles_t * tiles_f);
@@ -236,39 +244,70 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
   const int rt = use_c ? max(c.kernel_time / 2, 0) : 0;
   const int rf = use_c ? max(c.kernel_freq / 2, 0) : 0;
   const int er = kFtTileT + 2 * rt, ec = kFtTileF + 2 * rf;
+  const int n_ext = er * ec;
+  const float inv_ec = 1.0f / (float)ec;  // (i + 0.5) * inv_ec truncates to i / ec for i < kExt
+  // 1. loads, all issued before any is used: the halo tile and this thread's output
+  //    cells. Halo cells outside the grid read element 0; their LDS values are never
+  //    read, since the box is clipped to the grid.
+  float xa[kPer], xb[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + 256 * k;
+    const int r = (int)(((float)i + 0.5f) * inv_ec), j = i - r * ec;
+    const int t = t0 - rt + r, f = f0 - rf + j;
+    const bool valid = i < n_ext && t >= 0 && t < tb && f >= 0 && f < fb;
+    const size_t at = valid ? (size_t)t * fb + f : 0;
+    xa[k] = in_a[at];
+    xb[k] = POLAR ? 0.0f : in_b[at];
+  }
+  const int jo = tid & (kFtTileF - 1), ro = tid / kFtTileF;
+  const int fo = f0 + jo;
+  const bool copy_phase = POLAR && out_b != in_b;
+  float oa[kOut], ob[kOut];
+#pragma unroll
+  for (int m = 0; m < kOut; ++m) {
+    const int t = t0 + ro + kRowStep * m;
+    const bool valid = t < tb && fo < fb;
+    const size_t at = valid ? (size_t)t * fb + fo : 0;
+    oa[m] = POLAR ? 0.0f : in_a[at];
+    ob[m] = (!POLAR || copy_phase) ? in_b[at] : 0.0f;
+  }
+  // 2. quantizer weights per column, then quantized magnitudes into LDS
   for (int j = tid; j < ec; j += 256) {
     const int f = f0 - rf + j;
     wcol[j] = (use_q && f >= 0 && f < fb) ? quant_weight(q, (float)(freq_origin + freq_step * (double)f)) : 0.0f;
   }
   __syncthreads();
-  for (int i = tid; i < er * ec; i += 256) {
-    const int r = i / ec, j = i - r * ec;
-    const int t = t0 - rt + r, f = f0 - rf + j;
-    float v = 0.0f;
-    if (t >= 0 && t < tb && f >= 0 && f < fb) {
-      const size_t at = (size_t)t * fb + f;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + 256 * k;
+    if (i < n_ext) {
+      const int r = (int)(((float)i + 0.5f) * inv_ec), j = i - r * ec;
       const float w = wcol[j];
+      float v;
       if (POLAR) {
-        v = quant_apply(q, in_a[at], w);
+        v = quant_apply(q, xa[k], w);
       } else {
-        const float m = hypotf(in_a[at], in_b[at]);
+        const float m = hypotf(xa[k], xb[k]);
         v = (w > 0.0f && m > kFtEpsilon) ? fmaxf(quant_apply(q, m, w), 0.0f) : m;
       }
+      mq[i] = v;
     }
-    mq[i] = v;
   }
   __syncthreads();
+  // 3. box mean, contrast, direction, stores
+  if (fo >= fb) return;
   const float gamma = c.gamma > 0.0f ? c.gamma : 1.0f;
   const bool use_gamma = fabsf(gamma - 1.0f) > 1e-5f;
   const float inv_gamma = use_gamma ? 1.0f / gamma : 1.0f;
-  const int j = tid & (kFtTileF - 1);
-  const int f = f0 + j;
-  if (f >= fb) return;
-  const int f_lo = max(0, f - rf) - f0 + rf, f_hi = min(fb - 1, f + rf) - f0 + rf;
-  for (int r = tid / kFtTileF; r < kFtTileT; r += 256 / kFtTileF) {
+  const int f_lo = max(0, fo - rf) - f0 + rf, f_hi = min(fb - 1, fo + rf) - f0 + rf;
+  const float wq = wcol[jo + rf];
+#pragma unroll
+  for (int m = 0; m < kOut; ++m) {
+    const int r = ro + kRowStep * m;
     const int t = t0 + r;
     if (t >= tb) break;
-    const float mq0 = mq[(r + rt) * ec + j + rf];
+    const float mq0 = mq[(r + rt) * ec + jo + rf];
     float mc = mq0;
     if (use_c) {
       const int t_lo = max(0, t - rt) - t0 + rt, t_hi = min(tb - 1, t + rt) - t0 + rt;
@@ -278,27 +317,40 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       const float mean = sum / (float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1));
       mc = contrast_adjust(c, mq0, mean, use_gamma, inv_gamma);
     }
-    const size_t at = (size_t)t * fb + f;
+    const size_t at = (size_t)t * fb + fo;
     if (POLAR) {
       out_a[at] = mc;
-      if (out_b != in_b) out_b[at] = in_b[at];
-    } else {
-      const float x = in_a[at], y = in_b[at];
-      const bool quantized = wcol[j + rf] > 0.0f && hypotf(x, y) > kFtEpsilon;
-      float sn, cs;
-      if (!quantized && !use_c) {
-        out_a[at] = x;
-        out_b[at] = y;
-        continue;
-      }
-      sincosf(atan2f(y, x), &sn, &cs);
-      if (quantized && use_c && mq0 == 0.0f) {
-        // z_q is a signed zero: its argument is 0, +-0 or +-pi
-        sincosf(atan2f(mq0 * sn, mq0 * cs), &sn, &cs);
-      }
-      out_a[at] = mc * cs;
-      out_b[at] = mc * sn;
+      if (copy_phase) out_b[at] = ob[m];
+      continue;
     }
+    const float x = oa[m], y = ob[m];
+    const float m0 = hypotf(x, y);
+    const bool quantized = wq > 0.0f && m0 > kFtEpsilon;
+    if (!quantized && !use_c) {
+      out_a[at] = x;
+      out_b[at] = y;
+      continue;
+    }
+    float dx, dy;
+    const float inv = 1.0f / m0;
+    if (m0 > 0.0f && !(quantized && use_c && mq0 == 0.0f)) {
+      if (inv <= 3.4028234663852886e38f) {  // cos(atan2(y, x)), sin(atan2(y, x))
+        dx = x * inv;
+        dy = y * inv;
+      } else {                               // |z| below 1 / FLT_MAX
+        dx = x / m0;
+        dy = y / m0;
+      }
+    } else {
+      // a zero: the argument follows atan2's signed-zero cases
+      float sn, cs;
+      sincosf(atan2f(y, x), &sn, &cs);
+      if (m0 > 0.0f) sincosf(atan2f(mq0 * sn, mq0 * cs), &sn, &cs);
+      dx = cs;
+      dy = sn;
+    }
+    out_a[at] = mc * dx;
+    out_b[at] = mc * dy;
   }
 }
 
@@ -386,12 +438,18 @@ hipError_t ft_fused(bool polar, const float* in_a, const float* in_b, float* out
   const int tiles_t = (tb + kFtTileT - 1) / kFtTileT, tiles_f = (fb + kFtTileF - 1) / kFtTileF;
   if ((long long)tiles_t * tiles_f > 0x7fffffffll) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(tiles_t * tiles_f));
+  const int rt = use_c ? max(c.kernel_time / 2, 0) : 0, rf = use_c ? max(c.kernel_freq / 2, 0) : 0;
+  const int n_ext = (kFtTileT + 2 * rt) * (kFtTileF + 2 * rf);
+  constexpr int kSmall = 10, kLarge = (kFtTileT + 2 * kFtMaxRadiusT) * (kFtTileF + 2 * kFtMaxRadiusF) / 256;
+  const bool small = n_ext <= 256 * kSmall;  // boxes up to 5 x 5
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, in_a, in_b, out_a, out_b, tb, fb, q, use_q, freq_origin,
+                       freq_step, c, use_c, tiles_t, tiles_f);
+  };
   if (polar)
-    hipLaunchKernelGGL(ft_fused_kernel<true>, grid, dim3(256), 0, s, in_a, in_b, out_a, out_b, tb, fb, q, use_q,
-                       freq_origin, freq_step, c, use_c, tiles_t, tiles_f);
+    small ? go(ft_fused_kernel<true, kSmall>) : go(ft_fused_kernel<true, kLarge>);
   else
-    hipLaunchKernelGGL(ft_fused_kernel<false>, grid, dim3(256), 0, s, in_a, in_b, out_a, out_b, tb, fb, q, use_q,
-                       freq_origin, freq_step, c, use_c, tiles_t, tiles_f);
+    small ? go(ft_fused_kernel<false, kSmall>) : go(ft_fused_kernel<false, kLarge>);
   return hipGetLastError();
 }
 
