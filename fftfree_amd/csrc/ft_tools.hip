@@ -16,6 +16,11 @@ namespace {
 
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
 
+// |z| as the reference's build computes std::abs of a complex float: its -ffast-math
+// flags turn it into sqrt(re * re + im * im) with the sum fused (mul, fma, sqrt), and a
+// correctly rounded sqrtf of the same fma gives the same bits.
+__device__ __forceinline__ float ft_abs(float x, float y) { return sqrtf(__builtin_fmaf(x, x, y * y)); }
+
 // ft_tools.hpp:144-183: bilinear interpolation at (t / tf, f / ff).
 __global__ void __launch_bounds__(256) ft_upsample_kernel(const float2* __restrict__ src, int ts, int fs, int tf,
                                                           int ff, float2* __restrict__ dst) {
@@ -84,7 +89,7 @@ __global__ void __launch_bounds__(256) ft_complex_to_polar_kernel(const float2* 
                                                                   float* __restrict__ mag, float* __restrict__ ph) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
     const float2 v = z[i];
-    mag[i] = hypotf(v.x, v.y);
+    mag[i] = ft_abs(v.x, v.y);
     ph[i] = atan2f(v.y, v.x);
   }
 }
@@ -123,7 +128,7 @@ __global__ void __launch_bounds__(256) ft_quantize_kernel(float2* __restrict__ z
       mag[i] = quant_magnitude(q, mag[i], freq);
     } else {    // complex grid: |z| moved, the phase kept; tiny values untouched
       const float2 v = z[i];
-      const float m = hypotf(v.x, v.y);
+      const float m = ft_abs(v.x, v.y);
       if (m <= kFtEpsilon) continue;
       const float adj = fmaxf(quant_magnitude(q, m, freq), 0.0f);
       float s, c;
@@ -136,7 +141,7 @@ __global__ void __launch_bounds__(256) ft_quantize_kernel(float2* __restrict__ z
 __global__ void __launch_bounds__(256) ft_abs_kernel(const float2* __restrict__ z, size_t total,
                                                      float* __restrict__ mag) {
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
-    mag[i] = hypotf(z[i].x, z[i].y);
+    mag[i] = ft_abs(z[i].x, z[i].y);
 }
 
 // Mean of all magnitudes (the global contrast mode): one workgroup.
@@ -288,7 +293,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       if (POLAR) {
         v = quant_apply(q, xa[k], w);
       } else {
-        const float m = hypotf(xa[k], xb[k]);
+        const float m = ft_abs(xa[k], xb[k]);
         v = (w > 0.0f && m > kFtEpsilon) ? fmaxf(quant_apply(q, m, w), 0.0f) : m;
       }
       mq[i] = v;
@@ -324,7 +329,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       continue;
     }
     const float x = oa[m], y = ob[m];
-    const float m0 = hypotf(x, y);
+    const float m0 = ft_abs(x, y);
     const bool quantized = wq > 0.0f && m0 > kFtEpsilon;
     if (!quantized && !use_c) {
       out_a[at] = x;

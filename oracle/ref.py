@@ -185,6 +185,44 @@ def phase_infer(mag, N, hop=0, half=True, mode=0):
     return re, im
 
 
+REF_FT_PATH = Path(__file__).resolve().parent / "_ref" / "libft_ref.so"
+_ref_ft = None
+
+
+def ref_ft_available() -> bool:
+    """True when oracle/_ref/libft_ref.so (the reference's own ft_tools.hpp) is built."""
+    return REF_FT_PATH.exists()
+
+
+def _ft_args(a, b, polar, freq_origin, freq_step, up, down, preserve, quant, contrast):
+    q = quant or (0.0, 0.0, 0.0, 0.0)
+    c = contrast or (0, 0, 0.0, 0.0, 1.0, 0)
+    return (int(polar), freq_origin, freq_step, up[0], up[1], down[0], down[1], int(preserve), int(quant is not None),
+            *q, int(contrast is not None), c[0], c[1], c[2], c[3], c[4], int(c[5]), _p(a), _p(b), a.shape[0],
+            a.shape[1])
+
+
+_FT_ARGTYPES = ([ctypes.c_int, ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 6 + [ctypes.c_float] * 4 +
+                [ctypes.c_int] * 3 + [ctypes.c_float] * 3 + [ctypes.c_int, _f32p, _f32p, ctypes.c_int, ctypes.c_int])
+
+
+def refsrc_ft_pipeline(a, b, polar=False, freq_origin=0.0, freq_step=1.0, up=(1, 1), down=(1, 1), preserve=True,
+                       quant=None, contrast=None):
+    """The reference's own ft_tools.hpp templates (oracle/_ref/libft_ref.so), same
+    arguments and result as ft_pipeline."""
+    global _ref_ft
+    if _ref_ft is None:
+        L = ctypes.CDLL(str(REF_FT_PATH))
+        L.refsrc_ft_pipeline.argtypes = _FT_ARGTYPES
+        L.refsrc_ft_pipeline.restype = ctypes.c_int
+        _ref_ft = L
+    a = np.ascontiguousarray(a, dtype=np.float32).copy()
+    b = np.ascontiguousarray(b, dtype=np.float32).copy()
+    ok = _ref_ft.refsrc_ft_pipeline(*_ft_args(a, b, polar, freq_origin, freq_step, up, down, preserve, quant,
+                                              contrast))
+    return int(ok), a, b
+
+
 def ft_pipeline(a, b, polar=False, freq_origin=0.0, freq_step=1.0, up=(1, 1), down=(1, 1), preserve=True,
                 quant=None, contrast=None):
     """Reference ft_apply_*_pipeline on copies of the two (time, freq) planes; quant =
@@ -192,11 +230,7 @@ def ft_pipeline(a, b, polar=False, freq_origin=0.0, freq_step=1.0, up=(1, 1), do
     gamma, clamp_zero). Returns (ok, a, b)."""
     a = np.ascontiguousarray(a, dtype=np.float32).copy()
     b = np.ascontiguousarray(b, dtype=np.float32).copy()
-    q = quant or (0.0, 0.0, 0.0, 0.0)
-    c = contrast or (0, 0, 0.0, 0.0, 1.0, 0)
-    ok = lib().ref_ft_pipeline(int(polar), freq_origin, freq_step, up[0], up[1], down[0], down[1], int(preserve),
-                               int(quant is not None), *q, int(contrast is not None), c[0], c[1], c[2], c[3], c[4],
-                               int(c[5]), _p(a), _p(b), a.shape[0], a.shape[1])
+    ok = lib().ref_ft_pipeline(*_ft_args(a, b, polar, freq_origin, freq_step, up, down, preserve, quant, contrast))
     return int(ok), a, b
 
 

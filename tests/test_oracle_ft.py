@@ -209,3 +209,46 @@ def test_shape_mismatch_leaves_planes():
     # factors <= 0 read as 1: the identity
     ok, ra, rb = ref.ft_pipeline(a, b, up=(0, -2), down=(0, 0))
     assert ok == 1 and np.array_equal(ra, a) and np.array_equal(rb, b)
+
+
+# The oracle against the reference's own ft_tools.hpp, compiled in place with the
+# reference's flags (oracle/_ref/libft_ref.so; skipped where that is not built).
+# Without the harmonic weight (base frequency <= 0 or no quantizer) the two agree bit for
+# bit. The weight mix * exp(-decay * distance) goes through expf, which -ffast-math lets
+# the compiler vectorise (glibc's vector expf) in one build and not in the other; there
+# they agree to 1e-5 of the grid's largest value.
+REF_CASES = [
+    ({}, True),
+    ({"up": (2, 2), "down": (2, 2)}, True),
+    ({"up": (4, 3), "down": (5, 4)}, True),
+    ({"up": (3, 1), "down": (3, 1), "preserve": False}, True),
+    ({"quant": (0.0, 0.5, 1.0, 8.0), "contrast": (3, 3, 0.0, 2.0, 1.0, 0)}, True),
+    ({"contrast": (0, 0, -0.1, 0.8, 0.5, 0)}, True),
+    ({"contrast": (5, 2, 0.1, 1.0, 2.0, 1)}, True),
+    ({"quant": (55.0, 0.05, 0.5, 8.0), "contrast": (3, 3, 0.0, 0.5, 1.0, 1), "freq_step": 43.06640625}, False),
+    ({"up": (2, 2), "down": (2, 2), "quant": (55.0, 0.05, 0.5, 8.0), "contrast": (5, 2, 0.1, 1.0, 2.0, 1),
+      "freq_step": 43.06640625}, False),
+]
+
+
+@pytest.mark.skipif(not ref.ref_ft_available(), reason="oracle/_ref/libft_ref.so not built (needs the reference tree)")
+@pytest.mark.parametrize("polar", [False, True], ids=["complex", "polar"])
+@pytest.mark.parametrize("case", range(len(REF_CASES)))
+def test_oracle_matches_reference_header(polar, case):
+    cfg, exact = REF_CASES[case]
+    T, F = (4, 3) if cfg.get("up") == (4, 3) else (37, 65)
+    rng = np.random.default_rng(100 + case)
+    a = rng.standard_normal((T, F)).astype(np.float32)
+    b = rng.standard_normal((T, F)).astype(np.float32)
+    if polar:
+        a = np.abs(a)
+    ok, oa, ob = ref.ft_pipeline(a, b, polar=polar, **cfg)
+    rok, ra, rb = ref.refsrc_ft_pipeline(a, b, polar=polar, **cfg)
+    assert ok == rok == 1
+    if exact:
+        assert np.array_equal(oa, ra) and np.array_equal(ob, rb)
+    else:
+        scale = max(1.0, float(np.max(np.abs(ra))))
+        assert np.max(np.abs(oa - ra)) <= 1e-5 * scale and np.max(np.abs(ob - rb)) <= 1e-5 * scale
+    # a result of another shape: both refuse
+    assert ref.refsrc_ft_pipeline(a, b, polar=polar, up=(2, 1))[0] == ref.ft_pipeline(a, b, polar=polar, up=(2, 1))[0] == 0
