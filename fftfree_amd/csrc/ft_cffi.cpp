@@ -56,6 +56,8 @@ bool make_settings(const ft_grid_spec* spec, const ft_resample_plan_c* plan, con
     st.q.mix = quant->mix;
     st.q.distance_decay = quant->distance_decay;
   }
+  st.q.inv_base = 1.0f / st.q.base_frequency;  // used only where base > 0 / step > 0
+  st.q.inv_step = 1.0f / st.q.magnitude_step;
   if (st.use_c) {
     st.c.kernel_time = contrast->kernel_time;
     st.c.kernel_freq = contrast->kernel_freq;

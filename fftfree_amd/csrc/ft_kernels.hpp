@@ -12,6 +12,11 @@ struct FtQuantizer {
   float magnitude_step = 0.05f;
   float mix = 0.5f;
   float distance_decay = 8.0f;
+  // 1 / base_frequency and 1 / magnitude_step, rounded once on the host: the reference's
+  // -ffast-math build hoists these reciprocals out of its loops and multiplies by them in
+  // place of freq / base and magnitude / step.
+  float inv_base = 1.0f / 55.0f;
+  float inv_step = 1.0f / 0.05f;
 };
 
 // SpectralLocalContrastBooster settings (ft_tools.hpp:358-366).

@@ -94,11 +94,11 @@ __global__ void __launch_bounds__(256) ft_complex_to_polar_kernel(const float2* 
   }
 }
 
-// ExponentialHarmonicQuantizer (ft_tools.hpp:262-356).
+// ExponentialHarmonicQuantizer (ft_tools.hpp:267-356).
 __device__ float quant_weight(const FtQuantizer& q, float freq) {
   if (q.mix <= 0.0f) return 0.0f;
   if (q.base_frequency <= 0.0f) return fminf(fmaxf(q.mix, 0.0f), 1.0f);
-  const float ratio = freq / q.base_frequency;
+  const float ratio = freq * q.inv_base;
   if (!isfinite((double)ratio)) return 0.0f;
   const float nearest = roundf(ratio);  // halves away from zero: std::round on the double, exactly
   const float distance = fabsf(ratio - nearest);
@@ -109,7 +109,7 @@ __device__ float quant_weight(const FtQuantizer& q, float freq) {
 __device__ __forceinline__ float quant_apply(const FtQuantizer& q, float m, float w) {
   if (w <= 0.0f) return m;
   float target = m;
-  if (q.magnitude_step > 0.0f) target = roundf(m / q.magnitude_step) * q.magnitude_step;
+  if (q.magnitude_step > 0.0f) target = roundf(m * q.inv_step) * q.magnitude_step;
   return lerpf(m, target, w);
 }
 
@@ -240,7 +240,9 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
   constexpr int kOut = kFtTileT * kFtTileF / 256;    // output cells per thread
   constexpr int kRowStep = 256 / kFtTileF;
   static_assert(kExt % 256 == 0 && (kFtTileT * kFtTileF) % 256 == 0, "tile shape");
-  __shared__ float mq[kExt];
+  static_assert(KPER * 256 <= kExt, "halo tile");
+  __shared__ float mq[KPER * 256];                  // quantized magnitudes of the halo tile
+  __shared__ float mabs[POLAR ? 1 : KPER * 256];    // |z| of the halo tile (complex grids)
   __shared__ float wcol[kFtTileF + 2 * kFtMaxRadiusF];
   const int tid = threadIdx.x;
   const int tile = xcd_order(blockIdx.x, tiles_t * tiles_f);
@@ -295,6 +297,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       } else {
         const float m = ft_abs(xa[k], xb[k]);
         v = (w > 0.0f && m > kFtEpsilon) ? fmaxf(quant_apply(q, m, w), 0.0f) : m;
+        mabs[i] = m;
       }
       mq[i] = v;
     }
@@ -319,7 +322,8 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       float sum = 0.0f;
       for (int a = t_lo; a <= t_hi; ++a)
         for (int b = f_lo; b <= f_hi; ++b) sum += mq[a * ec + b];
-      const float mean = sum / (float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1));
+      // the hardware reciprocal (1 ulp) in place of an IEEE divide
+      const float mean = sum * __builtin_amdgcn_rcpf((float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1)));
       mc = contrast_adjust(c, mq0, mean, use_gamma, inv_gamma);
     }
     const size_t at = (size_t)t * fb + fo;
@@ -329,7 +333,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       continue;
     }
     const float x = oa[m], y = ob[m];
-    const float m0 = ft_abs(x, y);
+    const float m0 = mabs[(r + rt) * ec + jo + rf];
     const bool quantized = wq > 0.0f && m0 > kFtEpsilon;
     if (!quantized && !use_c) {
       out_a[at] = x;
@@ -337,7 +341,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
       continue;
     }
     float dx, dy;
-    const float inv = 1.0f / m0;
+    const float inv = __builtin_amdgcn_rcpf(m0);
     if (m0 > 0.0f && !(quantized && use_c && mq0 == 0.0f)) {
       if (inv <= 3.4028234663852886e38f) {  // cos(atan2(y, x)), sin(atan2(y, x))
         dx = x * inv;
