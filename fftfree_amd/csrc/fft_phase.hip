@@ -5,8 +5,9 @@
 // sum of dphi[k] = 2 pi kk hop / N over the frames before f (kk = k mod N), and
 // out = (float)(m cos phi), (float)(m sin phi) in double. The running sum is reproduced
 // exactly: one thread per (64-frame block, bin) finds phi at its first frame in closed
-// form (running_sum, phase_sum.hpp: a constant step per binade) and continues the same
-// additions.
+// form (running_sum, phase_sum.hpp: a constant step per binade) and walks on through
+// segments of constant exact steps, rotating (cos phi, sin phi) by each step; the
+// rotation's rounding (~64 * 2^-53) stays far below the float outputs' ulp.
 //
 // Mode 1 (minimum phase, fft_cffi.cpp:2095-2158): per frame the real-even log spectrum
 // log(max(1e-20, m)) (mirrored for a half spectrum), its inverse DFT (the real cepstrum),
@@ -21,7 +22,8 @@ namespace fftfree {
 
 namespace {
 
-constexpr int kPhaseBlock = 64;  // frames per checkpoint of the linear phase
+constexpr int kPhaseBlock = 64;  // frames per thread of the linear phase
+constexpr int kPhaseChunk = 32;  // magnitudes loaded together
 
 __device__ __forceinline__ double linear_dphi(int k, int n, int hop) {
   const int kk = k < n ? k : k % n;
@@ -37,17 +39,47 @@ __global__ void __launch_bounds__(256) phase_linear_kernel(const float* __restri
   const size_t b = idx / (size_t)bins;
   const int k = (int)(idx - b * (size_t)bins);
   const double d = linear_dphi(k, n, hop);
-  double phi = running_sum(d, (uint64_t)(b * kPhaseBlock));
-  for (int i = 0; i < kPhaseBlock; ++i) {
-    const size_t f = b * kPhaseBlock + i;
-    if (f >= frames) break;
-    const size_t at = f * (size_t)bins + k;
-    const double m = (double)mag[at];
-    double s, c;
-    sincos(phi, &s, &c);
-    o_re[at] = (float)(m * c);
-    o_im[at] = (float)(m * s);
-    phi += d;
+  const size_t f0 = b * kPhaseBlock;
+  const int count = frames - f0 < (size_t)kPhaseBlock ? (int)(frames - f0) : kPhaseBlock;
+  double phi = running_sum(d, (uint64_t)f0);
+  // Within a segment of constant exact steps (running_segment) cos and sin of the next
+  // phase come from rotating by the step; at a segment's end the literal addition is
+  // made and cos, sin are evaluated afresh. All lanes advance frame by frame together.
+  double c = 1.0, s = 0.0, cd = 1.0, sd = 0.0, step = 0.0;
+  uint64_t run = 0;
+  bool seed = true;
+  for (int i0 = 0; i0 < count; i0 += kPhaseChunk) {
+    float mv[kPhaseChunk];  // this chunk's magnitudes, all loads issued before any is used
+#pragma unroll
+    for (int j = 0; j < kPhaseChunk; ++j) {
+      const int i = i0 + j < count ? i0 + j : count - 1;
+      mv[j] = mag[(f0 + i) * (size_t)bins + k];
+    }
+#pragma unroll
+    for (int j = 0; j < kPhaseChunk; ++j) {
+      const int i = i0 + j;
+      if (i >= count) break;
+      if (seed) {
+        running_segment(phi, d, &step, &run);
+        sincos(phi, &s, &c);
+        sincos(step, &sd, &cd);
+        seed = false;
+      }
+      const size_t at = (f0 + i) * (size_t)bins + k;
+      const double m = (double)mv[j];
+      o_re[at] = (float)(m * c);
+      o_im[at] = (float)(m * s);
+      if (run == 0) {
+        phi = phi + d;
+        seed = true;
+      } else {
+        const double c2 = c * cd - s * sd;
+        s = s * cd + c * sd;
+        c = c2;
+        phi += step;
+        --run;
+      }
+    }
   }
 }
 

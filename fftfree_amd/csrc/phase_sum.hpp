@@ -64,4 +64,39 @@ FFTFREE_HD inline double running_sum(double d, uint64_t f) {
   return negative ? -phi : phi;
 }
 
+// The additions from phi on that all add the same exact amount: after this call, the next
+// `run` additions of d each add `step` (a multiple of the binade's spacing), and the one
+// after them must be made literally as fl(phi + d). For d >= 0 and finite phi >= 0 (the
+// linear phase); run == 0 means the next addition is literal. With S == 0, phi never moves
+// again and run is UINT64_MAX.
+FFTFREE_HD inline void running_segment(double phi, double d, double* step, uint64_t* run) {
+  *step = 0.0;
+  *run = 0;
+  if (d == 0.0) {
+    *run = UINT64_MAX;
+    return;
+  }
+  if (phi == 0.0) return;  // fl(0 + d) = d: literal
+  int e = 0;
+  (void)frexp(phi, &e);
+  const int se = 53 - e < 1074 ? 53 - e : 1074;
+  const double q = ldexp(d, se);
+  const double n = floor(q);
+  double S = q - n < 0.5 ? n : n + 1.0;
+  if (q - n == 0.5) {
+    // half-way: round-half-even adds the even one of n, n + 1 once P = phi / u is even
+    if (fmod(ldexp(phi, se), 2.0) != 0.0) return;
+    S = fmod(n, 2.0) == 0.0 ? n : n + 1.0;
+  }
+  if (S == 0.0) {
+    *run = UINT64_MAX;
+    return;
+  }
+  const double P = e < 1024 ? ldexp(ldexp(1.0, e) - phi, se) : ldexp(ldexp(1.0, e - 1) - 0.5 * phi, se + 1);
+  uint64_t m = (uint64_t)floor((P - 1.0) / S);
+  if (m * (uint64_t)S > (uint64_t)P - 1) --m;
+  *step = ldexp(S, -se);
+  *run = m;
+}
+
 }  // namespace fftfree

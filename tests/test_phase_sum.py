@@ -7,7 +7,9 @@ Cases: the linear-phase increments 2 pi kk hop / N of the reference's phase infe
 counts up to 70000; increments with few significant bits, where d / u falls half-way
 between integers and round-half-even depends on phi (d = k * 2^-j, 0.5, 3, 0.1, ...);
 increments below half a unit of the sum (phi stops moving); negative increments; long
-runs to 2^21 steps."""
+runs to 2^21 steps. The segment walk (running_segment) that the phase kernel uses to
+rotate (cos, sin) instead of evaluating them per frame visits exactly the same sums,
+and its rotated values stay within 1e-12 of cos/sin of them."""
 import os
 import subprocess
 import tempfile
@@ -81,7 +83,54 @@ int main() {
   lng.push_back(1u << 21);
   const double long_d[] = {0.1, 2.4543692606170259, 804.24771931898704, std::ldexp(3.0, -7)};
   for (double d : long_d) bad |= check(d, lng);
-  std::printf("checked %ld\n", checked);
+  // segments: the kernel's walk over a block of frames (segment, rotate, literal step)
+  // must visit exactly the running sums, and the rotated (cos, sin) stay within 1e-12
+  long seg_checked = 0;
+  auto walk = [&](double d, uint64_t f0, int count) {
+    double phi = fftfree::running_sum(d, f0);
+    double want = phi;
+    int i = 0;
+    double worst = 0.0;
+    while (i < count) {
+      double step; uint64_t run;
+      fftfree::running_segment(phi, d, &step, &run);
+      double c = std::cos(phi), s = std::sin(phi);
+      const double cd = std::cos(step), sd = std::sin(step);
+      for (;;) {
+        if (std::memcmp(&phi, &want, sizeof(double)) != 0) {
+          std::printf("SEGMENT d=%.17g f=%llu got=%.17g want=%.17g\n", d, (unsigned long long)(f0 + i), phi, want);
+          return 1;
+        }
+        worst = std::fmax(worst, std::fmax(std::fabs(c - std::cos(phi)), std::fabs(s - std::sin(phi))));
+        ++seg_checked;
+        ++i;
+        want += d;  // the literal next running sum
+        if (i >= count) break;
+        if (run == 0) { phi = phi + d; break; }
+        const double c2 = c * cd - s * sd;
+        s = s * cd + c * sd;
+        c = c2;
+        phi += step;
+        --run;
+      }
+    }
+    if (worst > 1e-12) { std::printf("ROTATION d=%.17g f0=%llu err=%.3g\n", d, (unsigned long long)f0, worst); return 1; }
+    return 0;
+  };
+  for (int N : Ns) {
+    const int hops[] = {1, 100, N / 4, N};
+    for (int hop : hops)
+      for (int kk = 0; kk <= N / 2; kk += 1 + (int)(rng() % 61)) {
+        const double d = 6.28318530717958647692 * (double)kk * (double)hop / (double)N;
+        for (uint64_t f0 = 0; f0 < 70000; f0 += 64 * (1 + rng() % 40)) bad |= walk(d, f0, 64);
+      }
+  }
+  for (int j = 0; j < 30; ++j)
+    for (int k = 1; k < 20; k += 2) {
+      const double d = std::ldexp((double)k, -j);
+      for (uint64_t f0 = 0; f0 < 5000; f0 += 64) bad |= walk(d, f0, 64);
+    }
+  std::printf("checked %ld segments-checked %ld\n", checked, seg_checked);
   return bad;
 }
 """
