@@ -1,9 +1,11 @@
 """ft_apply_complex_pipeline / ft_apply_polar_pipeline (ft_tools_cffi.cpp:76-170) on the
 GPU against the float restatement (oracle/cpu_ref.cpp ref_ft_pipeline).
 
-Tolerance: every step is a float computation per cell in the reference's order, but the
-device's hypotf/atan2f/sincosf/expf and its fused multiply-adds differ from the host's
-by an ulp or two, so cells agree to 2e-5 of the grid's largest value. The quantizer
+Tolerance: every step is a float computation per cell in the reference's order, and |z|
+is the reference build's sqrt(fma(re, re, im * im)) bit for bit, but the device's
+atan2f/sincosf/expf, its reciprocals in the box mean and the direction, and the
+placement of fused multiply-adds differ from the host's by an ulp or two, so cells agree
+to 2e-5 of the grid's largest value. The quantizer
 rounds |z| / step to the nearest integer; where |z| lies within an ulp of a half step the
 two sides may round to neighbouring steps. Such a cell differs by one step times the
 quantizer weight, scaled by at most 1 + |contrast gain| when the contrast stage follows,
