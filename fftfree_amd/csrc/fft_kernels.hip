@@ -755,6 +755,8 @@ const StftEntry kStft[] = {
     FFTFREE_STFT(512, 8, 4, 0, "8x8x8 pingpong", 8, 8, 8),
     FFTFREE_STFT(512, 8, 8, 1, "8x8x8 inplace", 8, 8, 8),
     FFTFREE_STFT(512, 16, 8, 1, "16x16x2 inplace", 16, 16, 2),
+    FFTFREE_STFT_PL(512, 8, 2, "8x8x8 small-batch tile-preload k=2", 8, 8, 8),
+    FFTFREE_STFT_PL(512, 8, 1, "8x8x8 small-batch tile-preload k=1", 8, 8, 8),
     FFTFREE_STFT_MF(1024, 16, 8, "16x16x4 multiframe k=8", 16, 16, 4),
     FFTFREE_STFT(1024, 16, 4, 1, "16x16x4 inplace", 16, 16, 4),
     FFTFREE_STFT(1024, 8, 2, 1, "8x8x8x2 inplace", 8, 8, 8, 2),
@@ -822,6 +824,8 @@ const VariantInfo* c2c_variant(int n, int idx) {
 // the first entry's kind (same description up to "k="), the tile size K whose whole
 // tiles fill the resident workgroup slots best: fewest rounds x K, ties to the larger K.
 // Tiles are P = 256/(L/E) frames by K; the multi-frame kernels hold 3 workgroups per CU.
+// When that choice leaves more than half of the slots empty (a small batch), the
+// "small-batch" entries of N (the same kernel with K = 2, 1) compete on the same cost.
 int default_stft_variant(const StftArgs& a) {
   const int count = count_variants(kStft, a.n);
   const StftEntry* first = find_variant(kStft, a.n, 0);
@@ -842,6 +846,26 @@ int default_stft_variant(const StftArgs& a) {
     const double tiles = std::floor((double)a.frames / (double)(P * e->info.fpw));
     const double cost = std::ceil(tiles / slots) * e->info.fpw;
     if (cost < best_cost || (cost == best_cost && e->info.fpw > find_variant(kStft, a.n, best)->info.fpw)) {
+      best_cost = cost;
+      best = i;
+    }
+  }
+  // small batches: partial tiles count too (the launcher runs a remainder in workgroup 0)
+  auto small_cost = [&](const StftEntry* e) {
+    const int P = 256 / (e->info.l / e->info.e);
+    const double tiles = std::ceil((double)a.frames / (double)(P * e->info.fpw));
+    return std::ceil(tiles / slots) * e->info.fpw;
+  };
+  const StftEntry* b = find_variant(kStft, a.n, best);
+  const double best_tiles = std::floor((double)a.frames / (double)(256 / (b->info.l / b->info.e) * b->info.fpw));
+  if (best_tiles * 2.0 >= slots) return best;
+  const int chosen = best;
+  best_cost = small_cost(b);
+  for (int i = 0; i < count; ++i) {
+    const StftEntry* e = find_variant(kStft, a.n, i);
+    if (!std::strstr(e->info.radices, "small-batch")) continue;
+    const double cost = small_cost(e);
+    if (cost < best_cost || (cost == best_cost && best != chosen && e->info.fpw > find_variant(kStft, a.n, best)->info.fpw)) {
       best_cost = cost;
       best = i;
     }
