@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "phase", "ft_pipeline",
                                                      "c2c1024", "c2c_large", "stft_grad", "ragged", "copy"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
+    ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--variant", type=int, default=-1)
@@ -520,8 +521,10 @@ def main():
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "ones",
                 "config": {"workload": "elementwise copy, 512 MiB read + 512 MiB written per launch"}}
     else:
-        n, B = 1024, 64
+        n, B = 1024, args.batch
         p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
+        if args.variant >= 0:
+            assert ff.set_variant(p, args.variant)
         x = torch.from_numpy(complex_normal(B, n, seed=1)).cuda()
         y = torch.empty_like(x)
 
@@ -532,11 +535,12 @@ def main():
         per_launch = gpu_s / args.steps
         alg_bytes = 2 * 8 * n * B
         achieved = alg_bytes / per_launch / 1e9
-        line = {"metric": "C2C transforms/s, N=1024 batch 64", "value": B * args.steps / gpu_s,
+        line = {"metric": f"C2C transforms/s, N=1024 batch {B}", "value": B * args.steps / gpu_s,
                 "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": per_launch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
-                "config": {"workload": "C2C forward N=1024 x 64, Stockham"},
+                "config": {"workload": f"C2C forward N=1024 x {B}, Stockham", "variant": ff.variants(p)[
+                    max(args.variant, 0)]},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
     print(json.dumps(line), flush=True)

@@ -374,15 +374,20 @@ FftContext* make_context(size_t n, int threads, int lanes, int inverse, int kern
 
 // Which kernel family fft_set_variant tunes: the fused STFT kernel when the context's
 // STFT runs fused, else the complex transform (single-launch LDS or two-pass large).
+// Which kernel table fft_set_variant / fft_variant_* address: the fused STFT tiles for
+// real-input and real-output contexts, the complex-transform kernels for C2C contexts
+// (whose batched complex entries run those) and for contexts the fused STFT cannot take.
+bool stft_variants(const FftContext* c) { return fused_ok(c) && c->transform != FFT_TRANSFORM_C2C; }
+
 int variant_count(const FftContext* c) {
-  if (fused_ok(c)) return stft_variant_count(c->N);
+  if (stft_variants(c)) return stft_variant_count(c->N);
   if (c->N > c2c_max_lds_n()) return large_variant_count(c->N);
   return c2c_variant_count(c->N);
 }
 
 const char* variant_desc(const FftContext* c, int idx, char* buf, size_t len) {
-  if (!fused_ok(c) && c->N > c2c_max_lds_n()) return large_variant_desc(c->N, idx, buf, len) ? buf : nullptr;
-  const VariantInfo* v = fused_ok(c) ? stft_variant(c->N, idx) : c2c_variant(c->N, idx);
+  if (!stft_variants(c) && c->N > c2c_max_lds_n()) return large_variant_desc(c->N, idx, buf, len) ? buf : nullptr;
+  const VariantInfo* v = stft_variants(c) ? stft_variant(c->N, idx) : c2c_variant(c->N, idx);
   if (!v) return nullptr;
   std::snprintf(buf, len, "%s fpw=%d", v->radices, v->fpw);
   return buf;
@@ -1634,7 +1639,7 @@ int fft_set_variant(void* handle, int variant) {
     return 1;
   }
   if (variant >= fftfree::variant_count(c)) return 0;
-  if (fftfree::fused_ok(c))
+  if (fftfree::stft_variants(c))
     c->plan->stft_variant = variant;
   else
     c->plan->c2c_variant = variant;

@@ -773,8 +773,8 @@ const C2CEntry kC2C[] = {
     FFTFREE_C2C(128, 16, 32, "16x8", 16, 8),
     FFTFREE_C2C(256, 16, 16, "16x16", 16, 16),
     FFTFREE_C2C(512, 8, 4, "8x8x8", 8, 8, 8),
+    FFTFREE_C2C(1024, 8, 2, "8x8x8x2", 8, 8, 8, 2),  // 13% faster at 65533 transforms, equal at 64
     FFTFREE_C2C(1024, 16, 4, "16x16x4", 16, 16, 4),
-    FFTFREE_C2C(1024, 8, 2, "8x8x8x2", 8, 8, 8, 2),
     FFTFREE_C2C(2048, 16, 2, "16x16x8", 16, 16, 8),
     FFTFREE_C2C(4096, 16, 1, "16x16x16", 16, 16, 16),
 };

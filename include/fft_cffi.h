@@ -266,7 +266,9 @@ size_t fft_plan_metadata(void* handle, int kind, float* out, size_t capacity);
 
 /* additive. Selects the kernel variant (radix pattern / frames per workgroup) for the
  * context's size; -1 restores the default. Returns 1 if the variant exists. The count
- * and a description ("8x8x8 fpw=4") are queryable for autotuners. */
+ * and a description ("8x8x8 fpw=4") are queryable for autotuners. R2C and C2R contexts
+ * select among the fused STFT kernels; C2C contexts among the complex-transform kernels
+ * that their batched complex entries run. */
 int fft_set_variant(void* handle, int variant);
 int fft_variant_count(void* handle);
 int fft_variant_name(void* handle, int variant, char* buf, size_t buflen);
