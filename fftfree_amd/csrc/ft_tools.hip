@@ -290,6 +290,8 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
     const int i = tid + 256 * k;
     if (i < n_ext) {
       const int r = (int)(((float)i + 0.5f) * inv_ec), j = i - r * ec;
+      const int t = t0 - rt + r, f = f0 - rf + j;
+      const bool in_grid = t >= 0 && t < tb && f >= 0 && f < fb;
       const float w = wcol[j];
       float v;
       if (POLAR) {
@@ -299,7 +301,7 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
         v = (w > 0.0f && m > kFtEpsilon) ? fmaxf(quant_apply(q, m, w), 0.0f) : m;
         mabs[i] = m;
       }
-      mq[i] = v;
+      mq[i] = in_grid ? v : 0.0f;  // zeros outside the grid: the 3 x 3 sum below adds them
     }
   }
   __syncthreads();
@@ -320,8 +322,17 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
     if (use_c) {
       const int t_lo = max(0, t - rt) - t0 + rt, t_hi = min(tb - 1, t + rt) - t0 + rt;
       float sum = 0.0f;
-      for (int a = t_lo; a <= t_hi; ++a)
-        for (int b = f_lo; b <= f_hi; ++b) sum += mq[a * ec + b];
+      if (rt == 1 && rf == 1) {
+        // the 3 x 3 box unrolled, row-major as the reference sums it; cells outside the
+        // grid hold 0 and adding 0 leaves every partial sum as the clipped loop has it
+        const float* q0 = mq + r * ec + jo;
+        sum = q0[0] + q0[1] + q0[2];
+        sum = sum + q0[ec] + q0[ec + 1] + q0[ec + 2];
+        sum = sum + q0[2 * ec] + q0[2 * ec + 1] + q0[2 * ec + 2];
+      } else {
+        for (int a = t_lo; a <= t_hi; ++a)
+          for (int b = f_lo; b <= f_hi; ++b) sum += mq[a * ec + b];
+      }
       // the hardware reciprocal (1 ulp) in place of an IEEE divide
       const float mean = sum * __builtin_amdgcn_rcpf((float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1)));
       mc = contrast_adjust(c, mq0, mean, use_gamma, inv_gamma);
