@@ -132,3 +132,20 @@ def test_variant_queries(lib):
     assert lib.fft_set_variant(h, 0) == 1
     assert lib.fft_set_variant(h, -1) == 1
     lib.fft_free(h)
+
+
+def test_cpp_caller_compiles_against_the_header(tmp_path):
+    """tests/abi/caller.cpp, a C++ caller in the reference's own call sequence
+    (tests/restore_test_cffi.cpp:120-210), compiles against include/fft_cffi.h with every
+    warning an error, and links to the built library (no symbol left unresolved)."""
+    import importlib.util
+    import subprocess
+    spec = importlib.util.spec_from_file_location("abi_build", Path(__file__).resolve().parent / "abi" / "build.py")
+    abi_build = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(abi_build)
+    cmd = abi_build.command(tmp_path / "caller")
+    if not _lib.LIB_PATH.exists():
+        cmd = [c for c in cmd if not c.startswith(("-L", "-l", "-Wl")) and c != str(_lib.LIB_PATH.parent)]
+        cmd = [c for c in cmd if c != "-o" and c != str(tmp_path / "caller")] + ["-fsyntax-only"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
