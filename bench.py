@@ -73,6 +73,47 @@ def read_traffic(name: str):
     return None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (the CPU baseline's hardware)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def host_info() -> dict:
+    return {"cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count()}
+
+
+def stft_rel_l2_fp64(pcm, planes, n, W, H, frames, win, reduce_magnitude, chunk=4096):
+    """Relative L2 of the device planes against the FP64 oracle over EVERY frame: framed
+    float32 input (pcm*win rounded to float32, as the reference multiplies), float64 DFT,
+    chunked so host memory stays small. planes = (re, im, mag) torch tensors, frames x
+    bins. Returns (complex rel-L2 of re + i im, rel-L2 of mag)."""
+    import numpy as np
+    from numpy.lib.stride_tricks import sliding_window_view
+    bins = n // 2 + 1
+    acc = np.zeros(4)
+    for f0 in range(0, frames, chunk):
+        f1 = min(frames, f0 + chunk)
+        x = sliding_window_view(pcm[f0 * H:(f1 - 1) * H + W], W)[::H][:f1 - f0]
+        x = (x * win[None, :W]).astype(np.float32) if win is not None else x.astype(np.float32)
+        X = np.fft.rfft(x.astype(np.float64), n=n, axis=-1)[:, :bins]
+        X[:, 0] = X[:, 0].real
+        X[:, -1] = X[:, -1].real
+        m = np.abs(X)
+        re, im, mg = (p[f0:f1].cpu().numpy().astype(np.float64) for p in planes)
+        want = m + 0j if reduce_magnitude else X
+        acc += [np.sum(np.abs(re + 1j * im - want) ** 2), np.sum(np.abs(want) ** 2),
+                np.sum((mg - m) ** 2), np.sum(m ** 2)]
+    return float((acc[0] / acc[1]) ** 0.5), float((acc[2] / acc[3]) ** 0.5)
+
+
 def timed(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -106,9 +147,9 @@ def cpu_baseline_stft(pcm, n, W, H, seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{runs} full runs of the 2^24-sample workload (65533 frames each), Stockham code 2, "
-                      f"restated reference CPU path, {threads} threads, {el:.1f} s"}
+    return dict({"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
+                 "sample": f"{runs} full runs of the 2^24-sample workload (65533 frames each), Stockham code 2, "
+                           f"restated reference CPU path, {threads} threads, {el:.1f} s"}, **host_info())
 
 
 def stft_grid(ff, torch):
@@ -165,6 +206,9 @@ def main():
         alg_bytes = frames * (4 * H + 12 * bins)
         achieved = alg_bytes / per_launch / 1e9
         traffic = read_traffic("stft_r2c_n1024")
+        from oracle import fp64
+        err_c, err_m = stft_rel_l2_fp64(pcm, out, n, W, H, frames, fp64.build_window(1, W), True)
+        chosen = ff.default_variant(p, frames)
         line = {
             "metric": "STFT frames/s & %HBM peak at N=1024; ragged-batch and fwd+bwd vs plain, 1 GPU",
             "value": frames * args.steps / gpu_s,
@@ -180,8 +224,10 @@ def main():
             "data": "synthetic: 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
             "config": {"workload": "batched R2C STFT, Hann 1024, hop 256, half spectrum + magnitude, 2^24 samples "
                                    "-> 65533 frames x 513 bins, three planes",
-                       "n": n, "window": W, "hop": H, "samples": L, "frames": frames, "variant": ff.variants(p)[
-                           max(args.variant, 0)]},
+                       "n": n, "window": W, "hop": H, "samples": L, "frames": frames, "variant_index": chosen,
+                       "variant": ff.variants(p)[chosen],
+                       "rel_l2_fp64": {"planes_re_im": err_c, "plane_mag": err_m, "frames_checked": frames,
+                                       "tolerance": fp64.tolerance(n)}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                          "algorithmic_bytes_per_launch": alg_bytes},
@@ -213,6 +259,17 @@ def main():
         # interior samples (every frame present): the round trip returns the signal
         inner = slice(n, span - n)
         rt_err = float(torch.linalg.vector_norm(out[inner] - d_pcm[inner]) / torch.linalg.vector_norm(d_pcm[inner]))
+        # FP64 oracle of the same inverse on frames [f0, f1): the samples whose covering
+        # frames all lie in the slice, [(f0 + N/H - 1) H, f1 H), are exact there
+        from oracle import fp64
+        f0, f1 = frames // 2, frames // 2 + 4096
+        hann = fp64.build_window(1, n)
+        ref_out = fp64.complex_batched64(re[f0:f1].cpu().numpy(), im[f0:f1].cpu().numpy(), n, inverse=True,
+                                         kernel=2, transform=2, half_spectrum=True, windows_enabled=True,
+                                         analysis=hann, synthesis=hann, apply_ola=True, hop=H, cola_mode=1)
+        lo, hi = (n // H - 1) * H, (f1 - f0) * H
+        got = out[f0 * H + lo:f0 * H + hi].cpu().numpy()
+        inv_err = fp64.rel_l2(got, ref_out[lo:hi])
         alg_bytes = frames * (8 * bins + 4 * n)
         achieved = alg_bytes / per_launch / 1e9
         line = {"metric": "inverse STFT frames/s (fft_execute_complex_batched, overlap-add + COLA), N=1024",
@@ -222,7 +279,9 @@ def main():
                 "data": "synthetic: planes of the forward STFT of 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
                 "config": {"workload": "inverse C2R half spectrum, Hann synthesis 1024, hop 256, overlap-add with "
                                        "COLA normalisation, 65533 frames x 513 bins -> 65533*1024 samples",
-                           "n": n, "hop": H, "frames": frames, "interior_roundtrip_rel_l2": rt_err},
+                           "n": n, "hop": H, "frames": frames, "interior_roundtrip_rel_l2": rt_err,
+                           "rel_l2_fp64": {"output": inv_err, "frames_checked": f1 - f0,
+                                           "tolerance": 2 * fp64.tolerance(n)}},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("istft_n1024"),
                              "algorithmic_bytes_per_launch": alg_bytes, "kernel": "istft_kernel"}}
@@ -270,7 +329,9 @@ def main():
                 "config": {"workload": "Griffin-Lim, inverse C2R + forward R2C half spectrum, no windows, pad 1, "
                                        f"{iters} iterations per call (ms_per_step), start included",
                            "n": n, "hop": H, "frames": frames, "iterations": iters,
-                           "ms_per_iteration": per_iter * 1e3, "ms_one_iteration_call": t_one / steps * 1e3},
+                           "ms_per_iteration": per_iter * 1e3, "ms_one_iteration_call": t_one / steps * 1e3,
+                           "rel_l2_fp64": None,
+                           "rel_l2_note": "iterative; tests/test_gpu_gl.py checks it against the oracle's loop"},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                              "algorithmic_bytes_per_launch": alg_bytes,
@@ -286,9 +347,10 @@ def main():
                 el = time.perf_counter() - t0
                 if el >= args.cpu_seconds:
                     break
-            line["cpu_baseline"] = {"value": done / el, "unit": "iterations/s", "cores": ref.default_threads(),
-                                    "kind": "port", "sample": f"{done} iterations of the restated reference loop "
-                                                              f"on the same spectrogram, {el:.1f} s"}
+            line["cpu_baseline"] = dict({"value": done / el, "unit": "iterations/s", "cores": ref.default_threads(),
+                                         "kind": "port", "sample": f"{done} iterations of the restated reference "
+                                                                   f"loop on the same spectrogram, {el:.1f} s"},
+                                        **host_info())
     elif args.workload == "phase":
         n, H, frames, bins, cells, re, im, mag, out_a, out_b, common = stft_grid(ff, torch)
         lib = ff.lib
@@ -303,6 +365,13 @@ def main():
         per_launch = gpu_s / args.steps
         gpu1_s, _ = timed(run(1), max(1, args.steps // 4), max(1, args.warmup // 4), torch)
         per1 = gpu1_s / max(1, args.steps // 4)
+        # against the oracle's restatement of the reference loop (pinned to the literal
+        # double running sum), frames [0, 4096) of the mode-0 output
+        from oracle import fp64, ref
+        run(0)()
+        torch.cuda.synchronize()
+        wre, wim = ref.phase_infer(mag[:4096].cpu().numpy(), n, H, half=True, mode=0)
+        phase_err = fp64.rel_l2(out_a[:4096].cpu().numpy() + 1j * out_b[:4096].cpu().numpy(), wre + 1j * wim)
         alg_bytes = cells * (4 + 8)
         achieved = alg_bytes / per_launch / 1e9
         common.update(steps=args.steps, warmup=args.warmup)
@@ -313,7 +382,9 @@ def main():
                                    "513 bins |X| -> (re, im); mode 1 (minimum phase: log, inverse C2C 1024, fold, "
                                    "forward C2C, exp) timed alongside",
                        "n": n, "hop": H, "frames": frames, "mode1_ms_per_call": per1 * 1e3,
-                       "mode1_frames_per_s": frames / per1},
+                       "mode1_frames_per_s": frames / per1,
+                       "rel_l2_oracle": {"mode0": phase_err, "frames_checked": 4096,
+                                         "oracle": "oracle/cpu_ref.cpp ref_phase_linear"}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg_bytes,
                          "kernel": "phase_linear_kernel"}})
@@ -328,9 +399,9 @@ def main():
                 el = time.perf_counter() - t0
                 if el >= args.cpu_seconds:
                     break
-            line["cpu_baseline"] = {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
-                                    "sample": f"{done} frames (4096-frame slices of the same |X|) through the "
-                                              f"restated mode-0 loop, 1 thread, {el:.1f} s"}
+            line["cpu_baseline"] = dict({"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
+                                         "sample": f"{done} frames (4096-frame slices of the same |X|) through the "
+                                                   f"restated mode-0 loop, 1 thread, {el:.1f} s"}, **host_info())
         for h in handles.values():
             lib.phase_free(h)
     elif args.workload == "ft_pipeline":
@@ -346,6 +417,14 @@ def main():
         per_launch = gpu_s / args.steps
         alg_bytes = cells * 16
         achieved = alg_bytes / per_launch / 1e9
+        # against the oracle on time rows [0, 2048): rows below 2047 see no slice edge
+        # through the 3x3 contrast box
+        from oracle import fp64, ref
+        step()
+        torch.cuda.synchronize()
+        ok, wa, wb = ref.ft_pipeline(re[:2048].cpu().numpy(), im[:2048].cpu().numpy(), freq_step=44100.0 / n,
+                                     quant=(55.0, 0.05, 0.5, 8.0), contrast=(3, 3, 0.0, 0.5, 1.0, 1))
+        ft_err = fp64.rel_l2(out_a[:2047].cpu().numpy() + 1j * out_b[:2047].cpu().numpy(), wa[:2047] + 1j * wb[:2047])
         common.update(steps=args.steps, warmup=args.warmup)
         line = dict(common, **{
             "metric": "grid pipeline frames/s (ft_apply_complex_pipeline_device: harmonic quantizer + 3x3 local "
@@ -354,7 +433,9 @@ def main():
             "config": {"workload": "ft pipeline on the complex STFT grid, 65533 time bins x 513 freq bins, "
                                    "quantizer 55 Hz / step 0.05 / mix 0.5 / decay 8, contrast 3x3 gain 0.5, "
                                    "freq step 44100/1024, no resampling (one pass)",
-                       "frames": frames, "bins": bins, "cells": cells},
+                       "frames": frames, "bins": bins, "cells": cells,
+                       "rel_l2_oracle": {"planes": ft_err, "time_rows_checked": 2047,
+                                         "oracle": "oracle/cpu_ref.cpp ref_ft_pipeline"}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("ft_fused_65533x513"),
                          "algorithmic_bytes_per_launch": alg_bytes, "kernel": "ft_fused_kernel<false>"}})
@@ -370,9 +451,9 @@ def main():
                 el = time.perf_counter() - t0
                 if el >= args.cpu_seconds:
                     break
-            line["cpu_baseline"] = {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
-                                    "sample": f"{done} frames (2048 x 513 slices of the same grid) through the "
-                                              f"restated pipeline, 1 thread, {el:.1f} s"}
+            line["cpu_baseline"] = dict({"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
+                                         "sample": f"{done} frames (2048 x 513 slices of the same grid) through the "
+                                                   f"restated pipeline, 1 thread, {el:.1f} s"}, **host_info())
     elif args.workload == "c2c_large":
         n, B = 1 << args.lg, 16
         fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
@@ -391,19 +472,38 @@ def main():
         per_step = gpu_s / args.steps
         torch.cuda.synchronize()
         rt_err = float(torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x))
-        # each of the four passes per step (two per transform) reads and writes B*N complex64
+        # forward vs the FP64 DFT, every transform
+        xs, ys = x.cpu().numpy(), y.cpu().numpy()
+        num = den = 0.0
+        for b in range(B):
+            want = np.fft.fft(xs[b].astype(np.complex128))
+            num += float(np.sum(np.abs(ys[b] - want) ** 2))
+            den += float(np.sum(np.abs(want) ** 2))
+        fwd_err = (num / den) ** 0.5
+        from oracle import fp64
+        # algorithmic bytes: each transform reads and writes B*N complex64 once; the two
+        # passes per transform move twice that, which is not algorithmic (a two-pass
+        # design caps at 50% of HBM peak)
         pass_bytes = 2 * 8 * n * B
-        achieved = 4 * pass_bytes / per_step / 1e9
+        alg_bytes = 2 * pass_bytes  # forward + inverse per step
+        achieved = alg_bytes / per_step / 1e9
         line = {"metric": f"C2C round trips/s, N=2^{args.lg} batch 16", "value": B * args.steps / gpu_s,
                 "unit": "round trips/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": per_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
                 "config": {"workload": f"C2C forward + inverse N=2^{args.lg} x 16, two LDS passes per transform",
                            "n": n, "batch": B, "roundtrip_rel_l2": rt_err,
-                           "variant": ff.variants(fwd)[max(args.variant, 0)]},
+                           "rel_l2_fp64": {"forward": fwd_err, "transforms_checked": B,
+                                           "tolerance": fp64.tolerance(n)},
+                           "variant": ff.variants(fwd)[ff.default_variant(fwd, B)]},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_large_2e{args.lg}"),
-                             "algorithmic_bytes_per_launch": pass_bytes}}
+                             "algorithmic_bytes_per_launch": pass_bytes,
+                             "algorithmic_bytes_per_step": alg_bytes,
+                             "pass_traffic_GBs": 4 * pass_bytes / per_step / 1e9,
+                             "note": "per step: forward + inverse, four HBM passes; frac counts each transform's "
+                                     "compulsory read + write once (cap 0.5 for two passes); pass_traffic_GBs is "
+                                     "the rate of all four passes' reads and writes"}}
     elif args.workload == "stft_host":
         n, W, H, L = 1024, 1024, 256, 1 << 24
         p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,
@@ -466,19 +566,41 @@ def main():
         fwd_ms, adj_ms = fwd_s / args.steps * 1e3, adj_s / args.steps * 1e3
         fwd_bytes = B * L * 4 + B * frames * bins * 8
         adj_bytes = B * frames * bins * 4 + B * L * 8
+        # FP64: forward planes and the gradient of the timed adjoint call, two signals
+        from oracle import fp64
+        win = fp64.build_window(1, W)
+        torch.mul(mag - ymag, 2.0, out=gmag)
+        adj()
+        torch.cuda.synchronize()
+        fe = [0.0, 0.0, 0.0, 0.0]
+        for sgl in (0, B - 1):
+            xs = x[sgl].cpu().numpy()
+            A, Bp = fp64.value_planes64(xs, n, W, H, frames, win, "polar")
+            got_m, got_p = mag[sgl].cpu().numpy(), phase[sgl].cpu().numpy()
+            fe[0] += float(np.sum(np.abs(got_m * np.exp(1j * got_p) - A * np.exp(1j * Bp)) ** 2))
+            fe[1] += float(np.sum(A ** 2))
+            want = fp64.stft_adjoint64(xs, n, W, H, frames, win, "magnitude", gmag[sgl].cpu().numpy())
+            fe[2] += float(np.sum((gx[sgl].cpu().numpy() - want) ** 2))
+            fe[3] += float(np.sum(want ** 2))
+        fwd_err, grad_err = (fe[0] / fe[1]) ** 0.5, (fe[2] / fe[3]) ** 0.5
         line = {"metric": "differentiable STFT: (fwd + adjoint) / fwd time, and signals/s", "value": B / (step_s / args.steps),
                 "unit": "signals/s (whole autograd step)", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": step_s / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
                 "vs_baseline": None, "dtype": "f32", "data": "synthetic: noise + 8 sinusoids, seeds 1000.. and 2000..",
                 "config": {"workload": "32 signals x 2^18 samples, Hann 2048, hop 512, polar planes, 509 frames x 1025 bins",
                            "fwd_kernel_ms": fwd_ms, "adjoint_kernel_ms": adj_ms,
-                           "fwd_plus_adjoint_over_fwd": (fwd_ms + adj_ms) / fwd_ms},
+                           "fwd_plus_adjoint_over_fwd": (fwd_ms + adj_ms) / fwd_ms,
+                           "rel_l2_fp64": {"forward_polar": fwd_err, "gradient": grad_err, "signals_checked": 2,
+                                           "tolerance_forward": fp64.tolerance(n),
+                                           "tolerance_gradient": 4 * fp64.tolerance(n)}},
                 "roofline": {"bound": "hbm", "achieved": adj_bytes / (adj_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                             "traffic": None, "algorithmic_bytes_per_launch": adj_bytes, "kernel": "stft_adjoint_kernel",
+                             "traffic": read_traffic("stft_grad_adjoint_n2048"), "algorithmic_bytes_per_launch": adj_bytes,
+                             "kernel": "stft_adjoint_kernel + stft_halo_kernel",
                              "forward": {"achieved": fwd_bytes / (fwd_ms * 1e-3) / 1e9,
                                          "frac": fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                         "algorithmic_bytes_per_launch": fwd_bytes}}}
+                                         "algorithmic_bytes_per_launch": fwd_bytes,
+                                         "traffic": read_traffic("stft_grad_forward_n2048")}}}
     elif args.workload == "ragged":
         from fftfree_amd.cffi_fft import RaggedBatch
         from fftfree_amd.workloads import ragged_requests
@@ -493,6 +615,23 @@ def main():
         graph_s, wall = timed(rb.execute, args.steps, args.warmup, torch)
         plain_s, _ = timed(rb.execute_plain, args.steps, args.warmup, torch)
         per, per_plain = graph_s / args.steps, plain_s / args.steps
+        # FP64 of every 8th request's polar planes, all their frames (graph output)
+        from oracle import fp64
+        rb.execute()
+        torch.cuda.synchronize()
+        acc = [0.0, 0.0]
+        worst = 0.0
+        for i in range(0, len(reqs), 8):
+            L, W, H = reqs[i]
+            f = rb.frames[i]
+            mg, ph = (t[:f].cpu().numpy() for t in (rb.outputs[i][0], rb.outputs[i][1]))
+            A, Bp = fp64.value_planes64(pcms[i].cpu().numpy(), W, W, H, f, fp64.build_window(1, W), "polar")
+            e = float(np.sum(np.abs(mg * np.exp(1j * ph) - A * np.exp(1j * Bp)) ** 2))
+            d = float(np.sum(A ** 2))
+            acc[0] += e
+            acc[1] += d
+            worst = max(worst, (e / d) ** 0.5 / fp64.tolerance(W))
+        ragged_err = (acc[0] / acc[1]) ** 0.5
         line = {"metric": "ragged batched STFT: frames/s in one graph launch, speedup vs per-request launches",
                 "value": frames / per, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": per * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -501,9 +640,12 @@ def main():
                                        "Hann, R2C half spectrum, polar planes, pad_mode 1",
                            "frames": frames, "samples": sum(L for L, W, H in reqs),
                            "plain_ms_per_step": per_plain * 1e3, "speedup_vs_plain": per_plain / per,
-                           "autotune_s": tune_s, "plan": rb.describe()},
+                           "autotune_s": tune_s, "plan": rb.describe(),
+                           "rel_l2_fp64": {"planes_polar": ragged_err, "requests_checked": len(range(0, len(reqs), 8)),
+                                           "worst_request_over_tolerance": worst,
+                                           "tolerance": "2e-7 * log2(N) per request"}},
                 "roofline": {"bound": "hbm", "achieved": alg_bytes / per / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": alg_bytes / per / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                             "frac": alg_bytes / per / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic("stft_ragged_512"),
                              "algorithmic_bytes_per_launch": alg_bytes, "kernel": "stft_ragged_kernel"}}
         rb.close()
     elif args.workload == "copy":
@@ -535,14 +677,19 @@ def main():
         per_launch = gpu_s / args.steps
         alg_bytes = 2 * 8 * n * B
         achieved = alg_bytes / per_launch / 1e9
+        from oracle import fp64
+        fwd_err = fp64.rel_l2(y.cpu().numpy(), fp64.fft64(x.cpu().numpy()))
         line = {"metric": f"C2C transforms/s, N=1024 batch {B}", "value": B * args.steps / gpu_s,
                 "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": per_launch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
-                "config": {"workload": f"C2C forward N=1024 x {B}, Stockham", "variant": ff.variants(p)[
-                    max(args.variant, 0)]},
+                "config": {"workload": f"C2C forward N=1024 x {B}, Stockham",
+                           "variant": ff.variants(p)[ff.default_variant(p, B)],
+                           "rel_l2_fp64": {"forward": fwd_err, "transforms_checked": B,
+                                           "tolerance": fp64.tolerance(n)}},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_n1024_b{B}"),
+                             "algorithmic_bytes_per_launch": alg_bytes, "kernel": "c2c_kernel"}}
     print(json.dumps(line), flush=True)
     ff.close()
 

@@ -89,6 +89,7 @@ SIGNATURES = {
     "fft_set_variant": (c_int, [c_void_p, c_int]),
     "fft_variant_count": (c_int, [c_void_p]),
     "fft_variant_name": (c_int, [c_void_p, c_int, c_char_p, c_size_t]),
+    "fft_variant_default": (c_int, [c_void_p, c_size_t]),
 }
 
 _lib = None

@@ -166,6 +166,10 @@ class CffiFft:
     def set_variant(self, plan: FftPlan, variant: int) -> bool:
         return bool(self.lib.fft_set_variant(plan.handle, variant))
 
+    def default_variant(self, plan: FftPlan, frames: int) -> int:
+        """Index of the variant a launch of `frames` frames runs (fft_variant_default)."""
+        return int(self.lib.fft_variant_default(plan.handle, frames))
+
     def variants(self, plan: FftPlan):
         names = []
         buf = ctypes.create_string_buffer(128)

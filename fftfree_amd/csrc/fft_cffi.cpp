@@ -372,8 +372,6 @@ FftContext* make_context(size_t n, int threads, int lanes, int inverse, int kern
   return c.release();
 }
 
-// Which kernel family fft_set_variant tunes: the fused STFT kernel when the context's
-// STFT runs fused, else the complex transform (single-launch LDS or two-pass large).
 // Which kernel table fft_set_variant / fft_variant_* address: the fused STFT tiles for
 // real-input and real-output contexts, the complex-transform kernels for C2C contexts
 // (whose batched complex entries run those) and for contexts the fused STFT cannot take.
@@ -1655,6 +1653,20 @@ int fft_variant_count(void* handle) {
 int fft_variant_name(void* handle, int variant, char* buf, size_t buflen) {
   if (!handle || !buf || buflen == 0) return 0;
   return fftfree::variant_desc(static_cast<FftContext*>(handle), variant, buf, buflen) ? 1 : 0;
+}
+
+int fft_variant_default(void* handle, size_t frames) {
+  if (!handle) return -1;
+  auto* c = static_cast<FftContext*>(handle);
+  if (fftfree::stft_variants(c)) {
+    if (c->plan->stft_variant >= 0) return c->plan->stft_variant;
+    if (c->N < 16) return 0;  // the direct-DFT kernel
+    fftfree::StftArgs a;
+    a.n = c->N;
+    a.frames = frames;
+    return fftfree::default_stft_variant(a);
+  }
+  return c->plan->c2c_variant >= 0 ? c->plan->c2c_variant : 0;
 }
 
 void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode, int reduce_magnitude,

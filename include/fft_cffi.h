@@ -272,6 +272,12 @@ size_t fft_plan_metadata(void* handle, int kind, float* out, size_t capacity);
 int fft_set_variant(void* handle, int variant);
 int fft_variant_count(void* handle);
 int fft_variant_name(void* handle, int variant, char* buf, size_t buflen);
+/* additive. The variant index a launch of `frames` frames (STFT contexts) or of a batch
+ * (complex contexts) runs on this context: the one fft_set_variant chose, else the
+ * default for that shape on the current device; -1 for a NULL handle. The batched STFT
+ * of a C2C context always runs its default STFT variant: fft_set_variant on a C2C
+ * context tunes the complex-transform kernels. */
+int fft_variant_default(void* handle, size_t frames);
 
 #ifdef __cplusplus
 }
