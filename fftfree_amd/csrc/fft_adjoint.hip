@@ -31,6 +31,8 @@
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 
+#include <mutex>
+
 namespace fftfree {
 using namespace dev;
 
@@ -517,8 +519,10 @@ hipError_t signals_launch(const SignalArgs& a, int mode, hipStream_t s) {
 // Sizes the grid to the chip: each workgroup runs `steps` steps, chosen so that the
 // workgroups of all signals fill the resident capacity (CUs x workgroups per CU) about
 // once. Longer runs cost fewer warm-up frames; more workgroups hide more latency.
+// Plans the adjoint launch of `a` (steps per run, runs per signal, halo slots) and
+// launches it; with `need`, only reports the halo floats that launch would use.
 template <int L, int E, int ACCM, int... Rs>
-hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
+hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s, size_t* need) {
   constexpr int FPS = 256 / (L / E);
   const void* kern = nullptr;
   switch (mode) {
@@ -549,6 +553,10 @@ hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
   if (grid == 0) return hipSuccess;
   if (grid > 0x7fffffffull) return hipErrorInvalidValue;
   const bool halos = halo > 0 && a.tiles > 1;
+  if (need) {
+    *need = halos ? grid * 2 * halo : 0;
+    return hipSuccess;
+  }
   if (halos && (!a.halo || grid * 2 * halo > a.halo_capacity)) return hipErrorInvalidValue;
   void* args[] = {&a};
   e = hipLaunchKernel(kern, dim3((unsigned)grid), dim3(256), args, 0, s);
@@ -562,16 +570,42 @@ hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s) {
 
 // The smallest accumulator ring (2N, 4N or 8N samples) that holds one step's span.
 template <int L, int E, int... Rs>
-hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s) {
-  if (span <= 4 * L) return adjoint_launch_acc<L, E, 2, Rs...>(a, mode, s);
-  if (span <= 8 * L) return adjoint_launch_acc<L, E, 4, Rs...>(a, mode, s);
-  return adjoint_launch_acc<L, E, 8, Rs...>(a, mode, s);
+hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s, size_t* need) {
+  if (span <= 4 * L) return adjoint_launch_acc<L, E, 2, Rs...>(a, mode, s, need);
+  if (span <= 8 * L) return adjoint_launch_acc<L, E, 4, Rs...>(a, mode, s, need);
+  return adjoint_launch_acc<L, E, 8, Rs...>(a, mode, s, need);
 }
 
 }  // namespace
 
 // Frames per adjoint step for each N (256 / threads per frame).
 int adjoint_fps(int n) { return n == 2048 ? 2 : 4; }
+
+namespace {
+// Scratch of the allocation-free adjoint entry (fft_stft_adjoint_device_noalloc): its
+// halo slots, kNoAllocHaloFloats floats (16 MiB), one copy on every GPU the library runs
+// on, held until the process ends. Every call on a GPU shares that GPU's copy.
+constexpr size_t kNoAllocHaloFloats = size_t(4) << 20;
+__device__ __align__(256) float scratch[kNoAllocHaloFloats];
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+float* noalloc_halo_scratch(size_t* capacity) {
+  // `scratch` names a host-side stand-in: the device address comes from
+  // hipGetSymbolAddress, looked up once per device with that device current
+  static std::mutex mu;
+  static float* per_device[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!per_device[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    per_device[dev] = static_cast<float*>(p);
+  }
+  *capacity = kNoAllocHaloFloats;
+  return per_device[dev];
+}
 
 size_t adjoint_scratch(int n, int window, int hop, size_t signals, size_t pcm_len) {
   if (hop <= 0 || window <= hop) return 0;
@@ -600,7 +634,7 @@ hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s) {
   }
 }
 
-hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s) {
+hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need) {
   const int n = a.n;
   if (!signals_supported(n, a.window, a.hop, true) || a.len == 0 || !a.grad) return hipErrorInvalidValue;
   const int fps = adjoint_fps(n);
@@ -608,9 +642,9 @@ hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s) {
   // with no phase gradient, polar V = Ga X/|X| is the magnitude map (fewer registers)
   if (mode == kOutPolar && !a.g_phase) mode = kOutMagnitude;
   switch (n) {
-    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, span, mode, s);
-    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, span, mode, s);
-    default: return adjoint_launch<1024, 8, 8, 8, 8, 2>(a, span, mode, s);
+    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, span, mode, s, halo_need);
+    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, span, mode, s, halo_need);
+    default: return adjoint_launch<1024, 8, 8, 8, 8, 2>(a, span, mode, s, halo_need);
   }
 }
 

@@ -907,6 +907,55 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
   }
 }
 
+size_t fft_stft_adjoint_device_noalloc(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
+                                       const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm,
+                                       int pad_mode, void* stream) {
+  if (!handle || !d_pcm || !d_grad_pcm || signals == 0 || pcm_len == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (!signals_ok(c, true)) return 0;
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, 0);
+  if (frames == 0) return 0;
+  try {
+    // no allocation here: the context's device state (stream, twiddles, window) must
+    // exist already, and the halo slots come from the library's static scratch
+    if (!c->gpu_ready || c->windows_dirty || !c->plan->W) {
+      fftfree::report("fft_stft_adjoint_device_noalloc",
+                      "the context has no device state yet: make one allocating device call on it first "
+                      "(e.g. fft_stft_signals_device)");
+      return 0;
+    }
+    fftfree::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    fftfree::SignalArgs a = signal_args(c, d_pcm, signals, pcm_len, frames);
+    a.g_mag = d_grad_real;
+    a.g_phase = d_grad_imag;
+    a.grad = d_grad_pcm;
+    const int mode = c->store_polar ? 3 : c->reduce_magnitude ? 2 : 0;  // polar, magnitude, complex
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    size_t need = 0, capacity = 0;
+    fftfree::hip_check(fftfree::launch_stft_adjoint(a, mode, s, &need), "plan adjoint");
+    if (need > 0) {
+      float* scr = fftfree::noalloc_halo_scratch(&capacity);
+      if (!scr) throw fftfree::HipFailure("hipGetSymbolAddress(scratch) failed");
+      if (need > capacity) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "this shape needs %zu floats of halo scratch; the allocation-free entry holds %zu",
+                      need, capacity);
+        fftfree::report("fft_stft_adjoint_device_noalloc", msg);
+        return 0;
+      }
+      a.halo = scr;
+      a.halo_capacity = capacity;
+    }
+    fftfree::hip_check(fftfree::launch_stft_adjoint(a, mode, s), "launch_stft_adjoint");
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stft_adjoint_device_noalloc", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
 size_t fft_execute_c2c_batched_device(void* handle, const float* d_in, float* d_out, size_t batch, void* stream) {
   if (!handle || !d_in || !d_out || batch == 0) return 0;
   auto* c = static_cast<FftContext*>(handle);

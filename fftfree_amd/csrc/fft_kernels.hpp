@@ -179,7 +179,11 @@ bool large_variant_desc(size_t n, int variant, char* buf, size_t len);
 // (k = 2 for N = 2048, else 4).
 bool signals_supported(int n, int window, int hop, bool adjoint);
 hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s);
-hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s);
+// With halo_need, launches nothing and reports the halo floats the launch would use.
+hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need = nullptr);
+// Device address (on the current device) and size in floats of the static halo scratch
+// of the allocation-free adjoint entry; nullptr if the lookup fails.
+float* noalloc_halo_scratch(size_t* capacity);
 
 // Variant tables for the autotuner.
 struct VariantInfo {

@@ -225,6 +225,19 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
                                const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,
                                void* stream);
 
+/* No counterpart in the reference: an allocation-free variant of fft_stft_adjoint_device,
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and results; it calls
+ * no hipMalloc, hipFree or allocating routine. Its halo scratch is a static 16 MiB device
+ * array per GPU (4M floats), shared by every call on that GPU and kept until the process
+ * ends: calls that overlap on one GPU race. A shape that needs more halo scratch is
+ * refused (0, with a message on stderr). The context must already hold its device state:
+ * make one allocating device call on it first (e.g. fft_stft_signals_device, which a
+ * forward/backward pair makes anyway); otherwise it is refused. */
+size_t fft_stft_adjoint_device_noalloc(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
+                                       const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm,
+                                       int pad_mode, void* stream);
+
 /* additive. One request of a ragged batch (device pointers): pcm_len samples, framed with
  * window = N (a power of two from 256 to 4096) and hop; output planes of frames*(N/2+1)
  * floats each, as fft_execute_batched writes them for a R2C half-spectrum context (any
