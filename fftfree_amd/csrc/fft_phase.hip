@@ -92,7 +92,9 @@ __global__ void __launch_bounds__(256) phase_log_kernel(const float* __restrict_
     const int i = (int)(idx - f * (size_t)n);
     int k = i;
     if (half && i > n / 2) k = n - i;  // spec[N-k] = spec[k] (fft_cffi.cpp:2119-2126)
-    const bool set = !half || i != n / 2 || (n % 2) == 0;
+    // half spectrum: DC always, bins 1 .. N/2-1 and their mirrors, N/2 when N is even;
+    // the rest stays 0 (fft_cffi.cpp:2113-2126)
+    const bool set = !half || i == 0 || i < n / 2 || i > n - n / 2 || (i == n / 2 && (n % 2) == 0);
     const float m = mag[f * (size_t)bins + k];
     cols[idx] = make_float2(set ? logf(fmaxf(1e-20f, m)) : 0.0f, 0.0f);
   }

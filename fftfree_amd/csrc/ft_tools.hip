@@ -333,8 +333,9 @@ __global__ void __launch_bounds__(256) ft_fused_kernel(const float* in_a, const 
         for (int a = t_lo; a <= t_hi; ++a)
           for (int b = f_lo; b <= f_hi; ++b) sum += mq[a * ec + b];
       }
-      // the hardware reciprocal (1 ulp) in place of an IEEE divide
-      const float mean = sum * __builtin_amdgcn_rcpf((float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1)));
+      // an IEEE divide, as the reference's build divides (local_mean /= count,
+      // ft_tools.hpp:433-437): the divisor changes per cell, so -ffast-math keeps it a divide
+      const float mean = sum / (float)((t_hi - t_lo + 1) * (f_hi - f_lo + 1));
       mc = contrast_adjust(c, mq0, mean, use_gamma, inv_gamma);
     }
     const size_t at = (size_t)t * fb + fo;

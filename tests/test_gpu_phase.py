@@ -47,7 +47,8 @@ def test_phase_linear_vs_oracle(lib, case):
     assert np.all(np.abs(re - rr) <= tol) and np.all(np.abs(im - ri) <= tol)
 
 
-@pytest.mark.parametrize("case", [(1024, True, 300), (256, True, 64), (512, False, 40), (8192, True, 12)])
+@pytest.mark.parametrize("case", [(1024, True, 300), (256, True, 64), (512, False, 40), (8192, True, 12),
+                                  (1, True, 5), (2, True, 7), (1, False, 3)])
 def test_phase_minimum_vs_oracle(lib, case):
     N, half, F = case
     bins = N // 2 + 1 if half else N
@@ -56,6 +57,10 @@ def test_phase_minimum_vs_oracle(lib, case):
     assert got == F
     rr, ri = ref.phase_infer(mag, N, 0, half=half, mode=1)
     assert fp64.rel_l2(re + 1j * im, rr + 1j * ri) <= 4 * fp64.tolerance(N)
+    if N == 1:
+        # one bin: log m, a 1-point transform each way, exp -> (m, 0); the reference always
+        # sets the DC bin (fft_cffi.cpp:2113-2114)
+        assert np.allclose(re, mag, rtol=1e-6) and not im.any()
 
 
 def test_phase_minimum_one_pole_on_device(lib):

@@ -99,28 +99,27 @@ def test_ragged_autotune_and_graph_replay(ff):
 
 def test_ragged_config5_batch(ff):
     """512 requests, lengths log-uniform in [2^12, 2^20], window from {256..4096}, hop
-    W/4, R2C half spectrum, polar, pad_mode 1: exact frame counts for all, FP64 oracle on
-    sampled frames of a spread of requests."""
+    W/4, R2C half spectrum, polar, pad_mode 1: exact frame counts for all, and every frame
+    of every request against the FP64 oracle (framed as the reference frames: pcm * win
+    rounded to float32, zero padding past the end)."""
     import torch
     reqs, pcms, rb = make_batch(ff, 512, seed=5, lo=12, hi=20)
     rb.execute()
     torch.cuda.synchronize()
     for (L, W, H), f in zip(reqs, rb.frames):
         assert f == fp64.frame_count(L, W, W, H, 0, 1)
-    rng = np.random.default_rng(0)
-    for i in rng.choice(512, 12, replace=False):
-        L, W, H = reqs[i]
-        pcm = pcms[i].cpu().numpy()
-        pick = np.sort(rng.choice(rb.frames[i], min(8, rb.frames[i]), replace=False))
-        win = fp64.build_window(1, W)
-        frames = np.zeros((len(pick), W))
-        for k, f in enumerate(pick):
-            seg = pcm[f * H:f * H + W].astype(np.float64)
-            frames[k, :seg.size] = seg * win[:seg.size]
-        X = np.fft.rfft(frames, axis=-1)
+    worst = 0.0
+    for i, (L, W, H) in enumerate(reqs):
+        f = rb.frames[i]
+        x = fp64.frame_signal(pcms[i].cpu().numpy(), W, W, H, f, fp64.build_window(1, W))
+        X = np.fft.rfft(x.astype(np.float64), axis=-1)
         X[:, 0] = X[:, 0].real
         X[:, -1] = X[:, -1].real
-        mag = rb.outputs[i][0].cpu().numpy()[pick]
-        ph = rb.outputs[i][1].cpu().numpy()[pick]
-        assert fp64.rel_l2(mag * np.exp(1j * ph), X) <= fp64.tolerance(W), i
+        mag = rb.outputs[i][0].cpu().numpy()
+        ph = rb.outputs[i][1].cpu().numpy()
+        err = fp64.rel_l2(mag * np.exp(1j * ph), X)
+        worst = max(worst, err / fp64.tolerance(W))
+        assert err <= fp64.tolerance(W), (i, L, W, err)
+        assert np.array_equal(rb.outputs[i][2].cpu().numpy().view(np.uint32), mag.view(np.uint32))  # mag = |X| plane
+    print(f"ragged 512 requests: worst rel_l2 / tolerance = {worst:.3f}", flush=True)
     rb.close()
