@@ -501,9 +501,10 @@ def main():
                              "algorithmic_bytes_per_launch": pass_bytes,
                              "algorithmic_bytes_per_step": alg_bytes,
                              "pass_traffic_GBs": 4 * pass_bytes / per_step / 1e9,
-                             "note": "per step: forward + inverse, four HBM passes; frac counts each transform's "
-                                     "compulsory read + write once (cap 0.5 for two passes); pass_traffic_GBs is "
-                                     "the rate of all four passes' reads and writes"}}
+                             "note": "per step: forward + inverse, each a pass A (columns) and a pass B (rows) "
+                                     "launch per chunk of ~32 MiB, chunks alternating over two streams; frac counts "
+                                     "each transform's compulsory read + write once (cap 0.5 for two passes); "
+                                     "pass_traffic_GBs is the rate of all four passes' reads and writes"}}
     elif args.workload == "stft_host":
         n, W, H, L = 1024, 1024, 256, 1 << 24
         p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,

@@ -10,6 +10,13 @@
 // product of two entries of the plan's FP64-built table (a few KB of it stay hot):
 // W_N^m = W_N^{m with its low S bits cleared} * W_N^{m mod 2^S}.
 //
+// The default kernels (col_line_kernel / row_line_kernel) give each line one wave of 64
+// lanes (16 values each), so the passes inside a line need no workgroup barrier, and read
+// the per-pass twiddles from an LDS table rather than registers: about 110 VGPRs, so two
+// 8-line workgroups of N1 = 1024 share a CU. The host runs the batch in chunks whose
+// pass-A output (about 32 MiB) is still in the caches when pass B reads it, alternating
+// two streams so that one chunk's pass A overlaps the other's pass B.
+//
 // The reference runs every N through lgN radix-2 stages in place (eigen_fft.hpp:2902-3057
 // Cooley-Tukey, 3472-3532 Stockham); the sum computed is the same DFT.
 #include "fft_device.hpp"
@@ -204,6 +211,264 @@ __global__ void __launch_bounds__(C*(L / E)) row_pass_kernel(LargeArgs a) {
   }
 }
 
+// ---- line kernels with LDS twiddle tables ------------------------------------------
+//
+// One line (a column of pass A, a row of pass B) is held by T threads, E = L/T values
+// each; C lines per workgroup. Exchanges between passes go through the line's LDS
+// buffer; with T = 64 a line is one wave and needs no workgroup barrier. The pass
+// twiddles come from per-workgroup LDS tables, not registers: pass P (span Ns > 1,
+// radix R) has table[m*(R-1) + r-1] = exp(DIR 2 pi i m r / (Ns R)), m < Ns. So the
+// kernels fit 128 VGPRs and two workgroups share a CU.
+
+// Line stride of the C-line staging buffers, in float2: the padded line plus 8, so that
+// stride = 8 (mod 32). The transposing accesses (lane = column c + C * row) then put the C
+// lines 16 dwords apart in the banks; with a stride that is a multiple of 32 every column
+// of a row hits the same bank pair.
+template <int L>
+constexpr int staged_stride() {
+  constexpr int p = padded_len<L>();
+  return p + ((8 - p % 32) + 32) % 32;
+}
+static_assert(staged_stride<1024>() % 32 == 8 && staged_stride<2048>() % 32 == 8, "stride 8 mod 32");
+
+// Twiddle-table floats of the passes of Rs (pass 0, span 1, has none).
+template <int Ns, int R, int... Rest>
+constexpr int line_tw_size() {
+  constexpr int own = Ns > 1 ? Ns * (R - 1) : 0;
+  if constexpr (sizeof...(Rest) == 0) return own;
+  else return own + line_tw_size<Ns * R, Rest...>();
+}
+
+// HT: one table tab[j] = exp(DIR 2 pi i j / L), j < L/2, for every pass, with
+// W_L^{j + L/2} = -W_L^j: half the LDS of the per-pass tables, a few more VALU per twiddle.
+template <int L, bool HT, int... Rs>
+constexpr int line_tab_size() {
+  if constexpr (HT) return L / 2;
+  else return line_tw_size<1, Rs...>();
+}
+
+template <int L, int DIR>
+__device__ __forceinline__ void fill_half_table(float2* tab, const float2* __restrict__ tw, int stride, int tid,
+                                                int nt) {
+  for (int i = tid; i < L / 2; i += nt) tab[i] = load_tw<DIR>(tw, i * stride);
+}
+
+template <int L>
+__device__ __forceinline__ float2 half_table_tw(const float2* tab, int idx) {
+  static_assert((L & (L - 1)) == 0, "power of two");
+  const float2 w = tab[idx & (L / 2 - 1)];
+  const unsigned sg = (unsigned)(idx & (L / 2)) << (31 - __builtin_ctz(L / 2));
+  return make_float2(__uint_as_float(__float_as_uint(w.x) ^ sg), __uint_as_float(__float_as_uint(w.y) ^ sg));
+}
+
+template <int L, int DIR, int Ns, int OFF, int R, int... Rest>
+struct LineTw {
+  // fill every table; tw is exp(-2 pi i m / (L*stride))
+  __device__ static __forceinline__ void fill(float2* tab, const float2* __restrict__ tw, int stride, int tid,
+                                              int nt) {
+    if constexpr (Ns > 1) {
+      for (int i = tid; i < Ns * (R - 1); i += nt) {
+        const int m = i / (R - 1), r = i - m * (R - 1) + 1;
+        tab[OFF + i] = load_tw<DIR>(tw, m * (L / R / Ns) * stride * r);
+      }
+    }
+    if constexpr (sizeof...(Rest) > 0)
+      LineTw<L, DIR, Ns * R, OFF + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>::fill(tab, tw, stride, tid, nt);
+  }
+};
+
+// The passes of one line; result in last-pass order v[q*Rl + r] = X[t + q*T + r*L/Rl].
+template <int L, int E, int DIR, bool HT, int Ns, int OFF, int R, int... Rest>
+struct LineStockham {
+  static constexpr int T = L / E;
+  template <class Sync>
+  __device__ static __forceinline__ void run(float2* v, int t, float2* buf, const float2* tab, Sync&& sync) {
+#pragma unroll
+    for (int q = 0; q < E / R; ++q) {
+      float2* u = v + q * R;
+      if constexpr (Ns > 1) {
+        if constexpr (HT) {
+          const int base = ((t + q * T) & (Ns - 1)) * (L / R / Ns);
+#pragma unroll
+          for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], half_table_tw<L>(tab, base * r));
+        } else {
+          const float2* w = tab + OFF + ((t + q * T) & (Ns - 1)) * (R - 1);
+#pragma unroll
+          for (int r = 1; r < R; ++r) u[r] = c_mul(u[r], w[r - 1]);
+        }
+      }
+      reg_dft<R, DIR>(u);
+    }
+    if constexpr (sizeof...(Rest) > 0) {
+#pragma unroll
+      for (int q = 0; q < E / R; ++q) {
+        const int bb = t + q * T;
+        const int k = bb & (Ns - 1);
+        const int o = (bb - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pidx(o + r * Ns)] = v[q * R + r];
+      }
+      sync();
+      constexpr int Rn = FirstOf<Rest...>::value;
+#pragma unroll
+      for (int q = 0; q < E / Rn; ++q) {
+        const int bb = t + q * T;
+#pragma unroll
+        for (int r = 0; r < Rn; ++r) v[q * Rn + r] = buf[pidx(bb + r * (L / Rn))];
+      }
+      sync();
+      LineStockham<L, E, DIR, HT, Ns * R, OFF + (Ns > 1 ? Ns * (R - 1) : 0), Rest...>::run(v, t, buf, tab, sync);
+    }
+  }
+};
+
+// Pass A with LDS twiddles: C adjacent columns (length L = N1, stride N2) per workgroup.
+// The inter-pass twiddle W_N^{n2 k1} with k1 = t + T*j is W_N^{n2 t} * W_N^{n2 T j}, the
+// second factor the same for the whole line; each factor is read as the product of the
+// table's hot subsets (big_twiddle), so the whole N-entry table is never swept. Pass A's
+// output is read back by pass B: ordinary stores.
+template <int L, int E, int C, bool HT, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E)) col_line_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int LP = staged_stride<L>();
+  constexpr int TWN = line_tab_size<L, HT, Rs...>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  static_assert(T % 64 == 0, "whole waves per line");
+  __shared__ float2 lds[C][LP];
+  __shared__ float2 tab[TWN > 0 ? TWN : 1];
+
+  const int tid = threadIdx.x;
+  const int groups = a.n2 / C;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t b = (size_t)(tile / groups);
+  const size_t c0 = (size_t)(tile - (int)b * groups) * C;
+  const size_t N2 = (size_t)a.n2;
+  const float2* __restrict__ src = a.in + b * a.n + c0;
+  float2* __restrict__ dst = a.out + b * a.n + c0;
+
+  if constexpr (HT) fill_half_table<L, DIR>(tab, a.tw, (int)(a.n / L), tid, NT);
+  else LineTw<L, DIR, 1, 0, Rs...>::fill(tab, a.tw, (int)(a.n / L), tid, NT);
+  {  // coalesced column load: thread = (column tid % C, rows tid / C + k * NT / C)
+    const int col = tid % C;
+    const int row0 = tid / C;
+    float2 x[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = src[(size_t)(row0 + k * (NT / C)) * N2 + col];
+#pragma unroll
+    for (int k = 0; k < E; ++k) lds[col][pidx(row0 + k * (NT / C))] = pre1(x[k], a.hooks.pre_mode);
+  }
+  __syncthreads();
+
+  const int line = __builtin_amdgcn_readfirstlane(tid / T);
+  const int t = tid - line * T;
+  float2* buf = &lds[line][0];
+  auto sync = [&]() {
+    if constexpr (T == 64) wave_lds_fence();
+    else __syncthreads();
+  };
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
+  sync();  // every value of the line read before the first in-place scatter
+  LineStockham<L, E, DIR, HT, 1, 0, Rs...>::run(v, t, buf, tab, sync);
+  const size_t n2 = c0 + (size_t)line;
+  const float2 wt = big_twiddle<DIR>(a, n2 * (size_t)t);
+  sync();  // the last pass's reads of buf are done before the natural-order writes
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int j = q + r * (E / Rl);  // k1 = t + T*j
+      const float2 wj = big_twiddle<DIR>(a, n2 * (size_t)(T * j));
+      buf[pidx(t + T * j)] = c_mul(v[q * Rl + r], c_mul(wt, wj));
+    }
+  __syncthreads();
+  {
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = row0 + k * (NT / C);
+      dst[(size_t)j * N2 + col] = lds[col][pidx(j)];
+    }
+  }
+}
+
+// Pass B with LDS twiddles: C adjacent rows (length L = N2, contiguous) per workgroup;
+// output (row k1, k2) goes to k1 + N1*k2 through LDS as runs of C values. NTS: streaming
+// stores; otherwise ordinary ones, which let L2 merge the partial lines of neighbouring
+// workgroups.
+template <int L, int E, int C, bool HT, bool NTS, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E)) row_line_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int LP = staged_stride<L>();
+  constexpr int TWN = line_tab_size<L, HT, Rs...>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  static_assert(T % 64 == 0, "whole waves per line");
+  __shared__ float2 lds[C][LP];
+  __shared__ float2 tab[TWN > 0 ? TWN : 1];
+
+  const int tid = threadIdx.x;
+  const int groups = a.n1 / C;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const size_t b = (size_t)(tile / groups);
+  const size_t r0 = (size_t)(tile - (int)b * groups) * C;
+  const size_t N1 = (size_t)a.n1;
+  const int line = __builtin_amdgcn_readfirstlane(tid / T);
+  const int t = tid - line * T;
+  float2* buf = &lds[line][0];
+  const float2* __restrict__ row = a.in + b * a.n + (r0 + line) * (size_t)L;
+
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) v[q * R0 + r] = row[t + q * T + r * (L / R0)];
+  if constexpr (HT) fill_half_table<L, DIR>(tab, a.tw, (int)(a.n / L), tid, NT);
+  else LineTw<L, DIR, 1, 0, Rs...>::fill(tab, a.tw, (int)(a.n / L), tid, NT);
+  __syncthreads();
+  auto sync = [&]() {
+    if constexpr (T == 64) wave_lds_fence();
+    else __syncthreads();
+  };
+  LineStockham<L, E, DIR, HT, 1, 0, Rs...>::run(v, t, buf, tab, sync);
+  const size_t k1 = r0 + (size_t)line;
+  sync();
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int k2 = t + q * T + r * (L / Rl);
+      buf[pidx(k2)] = post_large(v[q * Rl + r], k1 + N1 * (size_t)k2, a.n, a.hooks);
+    }
+  __syncthreads();
+  float2* __restrict__ dst = a.out + b * a.n + r0;
+  {
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int k2 = row0 + k * (NT / C);
+      const float2 z = lds[col][pidx(k2)];
+      if constexpr (NTS) {
+        float* o = reinterpret_cast<float*>(dst + (size_t)k2 * N1 + col);
+        __builtin_nontemporal_store(z.x, o);
+        __builtin_nontemporal_store(z.y, o + 1);
+      } else {
+        dst[(size_t)k2 * N1 + col] = z;
+      }
+    }
+  }
+}
+
 using LineFn = hipError_t (*)(const LargeArgs&, int lines, hipStream_t);
 
 template <int L, int E, int C, int DIR, int... Rs>
@@ -221,6 +486,21 @@ hipError_t row_launch(const LargeArgs& a, int lines, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int L, int E, int C, bool HT, int DIR, int... Rs>
+hipError_t col_line_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((col_line_kernel<L, E, C, HT, DIR, Rs...>), dim3((unsigned)((lines / C) * a.batch)),
+                     dim3(C * (L / E)), 0, s, a);
+  return hipGetLastError();
+}
+template <int L, int E, int C, bool HT, bool NTS, int DIR, int... Rs>
+hipError_t row_line_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((row_line_kernel<L, E, C, HT, NTS, DIR, Rs...>), dim3((unsigned)((lines / C) * a.batch)),
+                     dim3(C * (L / E)), 0, s, a);
+  return hipGetLastError();
+}
+
 struct LineEntry {
   int l, e, c;
   const char* radices;
@@ -231,19 +511,28 @@ struct LineEntry {
 #define FFTFREE_LINE(L, E, C, NAME, ...)                                                           \
   {L, E, C, NAME, &col_launch<L, E, C, -1, __VA_ARGS__>, &col_launch<L, E, C, 1, __VA_ARGS__>,    \
    &row_launch<L, E, C, -1, __VA_ARGS__>, &row_launch<L, E, C, 1, __VA_ARGS__>}
+#define FFTFREE_LINE_T(L, E, C, HT, NTS, NAME, ...)                                                  \
+  {L, E, C, NAME, &col_line_launch<L, E, C, HT, -1, __VA_ARGS__>, &col_line_launch<L, E, C, HT, 1, __VA_ARGS__>, \
+   &row_line_launch<L, E, C, HT, NTS, -1, __VA_ARGS__>, &row_line_launch<L, E, C, HT, NTS, 1, __VA_ARGS__>}
 const LineEntry kLines[] = {
     FFTFREE_LINE(64, 8, 32, "8x8", 8, 8),
     FFTFREE_LINE(128, 16, 32, "16x8", 16, 8),
     FFTFREE_LINE(256, 16, 16, "16x16", 16, 16),
     FFTFREE_LINE(512, 8, 8, "8x8x8", 8, 8, 8),
+    FFTFREE_LINE_T(1024, 16, 8, false, true, "16x16x4 lds-tw", 16, 16, 4),
+    FFTFREE_LINE_T(1024, 16, 8, true, true, "16x16x4 lds-half-tw", 16, 16, 4),
+    FFTFREE_LINE_T(1024, 16, 16, false, true, "16x16x4 lds-tw", 16, 16, 4),
     FFTFREE_LINE(1024, 16, 8, "16x16x4", 16, 16, 4),
     FFTFREE_LINE(1024, 8, 8, "8x8x4x4", 8, 8, 4, 4),
     FFTFREE_LINE(1024, 4, 4, "4x4x4x4x4", 4, 4, 4, 4, 4),
+    FFTFREE_LINE_T(2048, 16, 8, false, true, "16x16x8 lds-tw", 16, 16, 8),
+    FFTFREE_LINE_T(2048, 16, 4, true, false, "16x16x8 lds-half-tw wb", 16, 16, 8),
     FFTFREE_LINE(2048, 16, 4, "16x16x8", 16, 16, 8),
     FFTFREE_LINE(2048, 8, 4, "8x8x8x4", 8, 8, 8, 4),
     FFTFREE_LINE(4096, 16, 4, "16x16x16", 16, 16, 16),
 };
 #undef FFTFREE_LINE
+#undef FFTFREE_LINE_T
 
 int line_count(int l) {
   int c = 0;

@@ -74,9 +74,22 @@ Plan::Plan(int n, bool inv, const PlanRuntimeConfig& cfg) : N(n), inverse(inv) {
 }
 
 Plan::~Plan() {
-  if (W || scratch) (void)hipSetDevice(device);
+  if (W || scratch || aux_stream) (void)hipSetDevice(device);
+  if (aux_stream) (void)hipStreamDestroy(aux_stream);
+  if (aux_fork) (void)hipEventDestroy(aux_fork);
+  if (aux_join) (void)hipEventDestroy(aux_join);
   if (W) (void)hipFree(W);
   if (scratch) (void)hipFree(scratch);
+}
+
+hipStream_t Plan::ensure_aux_stream() const {
+  auto* self = const_cast<Plan*>(this);
+  if (!aux_stream) {
+    hip_check(hipEventCreateWithFlags(&self->aux_fork, hipEventDisableTiming), "hipEventCreate(fork)");
+    hip_check(hipEventCreateWithFlags(&self->aux_join, hipEventDisableTiming), "hipEventCreate(join)");
+    hip_check(hipStreamCreateWithFlags(&self->aux_stream, hipStreamNonBlocking), "hipStreamCreate(aux)");
+  }
+  return aux_stream;
 }
 
 float2* Plan::ensure_scratch(size_t elems) const {
@@ -173,15 +186,35 @@ void gpu_execute_axis(const Plan& P, const AxisLayout& layout, bool cooley_tukey
       hip_check(launch_c2c(a, P.inverse ? 1 : 0, P.c2c_variant, P.stream()), "launch_c2c");
       return;
     }
-    // two-pass large transform, in chunks that bound the scratch (<= 512 MiB)
-    const size_t per = std::max<size_t>(1, std::min<size_t>(65535, (size_t(64) << 20) / static_cast<size_t>(P.N)));
-    for (size_t c = b; c < e; c += per) {
+    // Two-pass large transform in chunks of transforms. A chunk's pass-A output (the
+    // scratch) is read back by its pass B straight away, so a chunk's scratch is kept
+    // small enough to stay in the caches (about 32 MiB); chunks alternate between the
+    // plan's stream and a second one, each with its own scratch half, so one chunk's
+    // pass A overlaps the other's pass B and neither waits for a launch's tail.
+    const size_t chunk_elems = size_t(4) << 20;
+    const size_t n = static_cast<size_t>(P.N);
+    const size_t per = std::max<size_t>(1, std::min<size_t>(65535, chunk_elems / n));
+    const size_t chunks = (e - b + per - 1) / per;
+    const bool two = chunks > 1;
+    float2* tmp = P.ensure_scratch((two ? 2 : 1) * per * n);
+    hipStream_t s0 = P.stream();
+    hipStream_t s1 = two ? P.ensure_aux_stream() : s0;
+    if (two) {
+      hip_check(hipEventRecord(P.aux_fork, s0), "hipEventRecord(fork)");
+      hip_check(hipStreamWaitEvent(s1, P.aux_fork, 0), "hipStreamWaitEvent(fork)");
+    }
+    for (size_t i = 0; i < chunks; ++i) {
+      const size_t c = b + i * per;
       const size_t cnt = std::min(per, e - c);
-      float2* tmp = P.ensure_scratch(cnt * static_cast<size_t>(P.N));
-      a.in = src + c * static_cast<size_t>(P.N);
-      a.out = layout.base + c * static_cast<size_t>(P.N);
+      a.in = src + c * n;
+      a.out = layout.base + c * n;
       a.batch = cnt;
-      hip_check(launch_c2c_large(a, P.inverse ? 1 : 0, P.c2c_variant, tmp, P.stream()), "launch_c2c_large");
+      hip_check(launch_c2c_large(a, P.inverse ? 1 : 0, P.c2c_variant, tmp + (i & 1) * per * n, (i & 1) ? s1 : s0),
+                "launch_c2c_large");
+    }
+    if (two) {
+      hip_check(hipEventRecord(P.aux_join, s1), "hipEventRecord(join)");
+      hip_check(hipStreamWaitEvent(s0, P.aux_join, 0), "hipStreamWaitEvent(join)");
     }
   });
 }

@@ -122,6 +122,8 @@ struct Plan {
   float2* W = nullptr;             // device table exp(-2 pi i m / N), m < N, FP64-built
   float2* scratch = nullptr;       // device scratch for N > 4096 (pass-A output), grown on use
   size_t scratch_elems = 0;
+  hipStream_t aux_stream = nullptr;  // second stream of the large-N chunk pipeline (default priority)
+  hipEvent_t aux_fork = nullptr, aux_join = nullptr;
   int stft_variant = -1;           // kernel variant (autotuner), -1 = default
   int c2c_variant = -1;
 
@@ -132,6 +134,8 @@ struct Plan {
 
   // Uploads the twiddle table on first use (lazy, so host-only queries need no GPU).
   void ensure_device();
+  // The second stream and its fork/join events, created on first use on the plan's device.
+  hipStream_t ensure_aux_stream() const;
   // Scratch of at least `elems` complex values (large-N two-pass transforms).
   float2* ensure_scratch(size_t elems) const;
 
