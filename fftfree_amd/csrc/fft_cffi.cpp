@@ -24,6 +24,7 @@
 
 #include "../../include/fft_cffi.h"
 #include "fft_kernels.hpp"
+#include "host_pipeline.hpp"
 #include "mt19937.hpp"
 #include "plan.hpp"
 
@@ -108,8 +109,33 @@ struct FftContext {
   size_t h_words_cap = 0;
   hipEvent_t words_copied = nullptr;  // the last copy out of h_words
   bool windows_dirty = true;
+  // host-pointer pipeline (created on first use): upload and download streams, pinned
+  // bounce slots, one event per slot and stage, and the host copy threads
+  hipStream_t up_stream = nullptr, down_stream = nullptr;
+  PinnedBuffer h_in, h_out;
+  hipEvent_t ev_in[2] = {nullptr, nullptr};
+  hipEvent_t ev_kernel[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_out[3] = {nullptr, nullptr, nullptr};
+  std::unique_ptr<CopyPool> copy_pool;
 
   ~FftContext() {
+    if (up_stream || down_stream) {
+      (void)hipSetDevice(device);
+      if (up_stream) (void)hipStreamSynchronize(up_stream);
+      if (down_stream) (void)hipStreamSynchronize(down_stream);
+      if (stream) (void)hipStreamSynchronize(stream);
+      for (hipEvent_t e : ev_in)
+        if (e) (void)hipEventDestroy(e);
+      for (hipEvent_t e : ev_kernel)
+        if (e) (void)hipEventDestroy(e);
+      for (hipEvent_t e : ev_out)
+        if (e) (void)hipEventDestroy(e);
+      if (up_stream) (void)hipStreamDestroy(up_stream);
+      if (down_stream) (void)hipStreamDestroy(down_stream);
+    }
+    h_in.release();
+    h_out.release();
+    copy_pool.reset();
     d_awin.release();
     d_swin.release();
     d_pcm.release();
@@ -351,6 +377,125 @@ void run_stft_frames(FftContext* c, const StftArgs& base, size_t f0, size_t nf, 
                             im ? im + b * bins : nullptr, mag ? mag + b * bins : nullptr, s),
               "launch_planes");
   }
+}
+
+// After an error part-way: wait for every stage so no copy is left writing into the
+// caller's or the context's buffers.
+void drain_pipeline(FftContext* c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->up_stream) (void)hipStreamSynchronize(c->up_stream);
+  if (c->down_stream) (void)hipStreamSynchronize(c->down_stream);
+}
+
+// Streams, events and copy threads of the host-pointer pipeline.
+void ensure_pipeline(FftContext* c) {
+  if (c->up_stream) return;
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  for (hipEvent_t& e : c->ev_in) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  for (hipEvent_t& e : c->ev_kernel) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  for (hipEvent_t& e : c->ev_out) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipStreamCreateWithFlags(&c->down_stream, hipStreamNonBlocking), "hipStreamCreate(down)");
+  hip_check(hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking), "hipStreamCreate(up)");
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  c->copy_pool = std::make_unique<CopyPool>((int)std::min(7u, hw > 1 ? hw - 1 : 0u));
+}
+
+// fft_execute_batched on host pointers (fft_cffi.cpp:713-804): `frames` frames of the STFT
+// of pcm[0, pcm_len) into the three host planes, in chunks of about 24 MiB of planes.
+// Chunk i: its new samples are copied into pinned input slot i % 2 and uploaded; its
+// kernel writes device plane slot i % 3; the planes are downloaded into pinned slot i % 3;
+// two chunks later they are copied into the caller's planes. Returns the frames whose
+// planes reached the caller (all of them unless an error is thrown part-way).
+size_t stft_host_pipelined(FftContext* c, const float* pcm, size_t pcm_len, size_t frames, float* o_re, float* o_im,
+                           float* o_mag, size_t* produced) {
+  constexpr size_t K = 3;
+  ensure_pipeline(c);
+  const size_t bins = bins_of(c);
+  const size_t W = (size_t)(c->window ? c->window : c->N);
+  const size_t H = (size_t)(c->hop ? c->hop : (int)W);
+  const size_t plane_bytes = bins * sizeof(float);
+  // A multiple of every kernel variant's tile (4 to 32 frames), so chunk edges fall on
+  // tile edges of the single launch over all frames and the planes are bit-identical.
+  constexpr size_t kTileMultiple = 192;
+  size_t chunk = (size_t(24) << 20) / (3 * plane_bytes);
+  chunk = std::max(kTileMultiple, chunk / kTileMultiple * kTileMultiple);
+  chunk = std::min(chunk, frames);
+  const size_t chunks = (frames + chunk - 1) / chunk;
+  // samples [0, end(i)) are needed by chunks 0..i
+  auto sample_end = [&](size_t i) {
+    const size_t last = std::min(frames, (i + 1) * chunk) - 1;
+    return std::min(pcm_len, last * H + W);
+  };
+  size_t max_piece = 0;
+  for (size_t i = 0, prev = 0; i < chunks; ++i) {
+    const size_t e = std::max(prev, sample_end(i));
+    max_piece = std::max(max_piece, e - prev);
+    prev = e;
+  }
+  // each plane of a slot starts on a 256-byte boundary, as the kernels' vector stores want
+  const size_t plane_stride = (chunk * bins + 63) / 64 * 64;
+  const size_t slot_floats = 3 * plane_stride;
+  c->d_pcm.reserve(pcm_len * sizeof(float));
+  c->d_planes.reserve(K * slot_floats * sizeof(float));
+  c->h_in.reserve(std::max<size_t>(1, 2 * max_piece) * sizeof(float));
+  c->h_out.reserve(K * slot_floats * sizeof(float));
+  const StftArgs base = stft_args(c, c->d_pcm.as<float>(), pcm_len);
+  float* const d_slots = c->d_planes.as<float>();
+  float* const h_slots = c->h_out.as<float>();
+
+  auto copy_out = [&](size_t j) {
+    hip_check(hipEventSynchronize(c->ev_out[j % K]), "download");
+    const size_t f0 = j * chunk, nf = std::min(chunk, frames - f0);
+    const float* src = h_slots + (j % K) * slot_floats;
+    const CopyPool::Copy copies[3] = {{o_re + f0 * bins, src, nf * plane_bytes},
+                                      {o_im + f0 * bins, src + plane_stride, nf * plane_bytes},
+                                      {o_mag + f0 * bins, src + 2 * plane_stride, nf * plane_bytes}};
+    c->copy_pool->run(copies, 3);
+    *produced = f0 + nf;
+  };
+  size_t staged = 0;  // samples uploaded so far
+  bool in_used[2] = {false, false};
+  for (size_t i = 0; i < chunks; ++i) {
+    const size_t f0 = i * chunk, nf = std::min(chunk, frames - f0);
+    const size_t end = std::max(staged, sample_end(i));
+    if (end > staged) {
+      const size_t in_slot = i % 2;
+      // the upload that last read this pinned slot (chunk i - 2) has finished
+      if (in_used[in_slot]) hip_check(hipEventSynchronize(c->ev_in[in_slot]), "upload");
+      in_used[in_slot] = true;
+      float* stage = c->h_in.as<float>() + in_slot * max_piece;
+      const CopyPool::Copy in{stage, pcm + staged, (end - staged) * sizeof(float)};
+      c->copy_pool->run(&in, 1);
+      hip_check(hipMemcpyAsync(c->d_pcm.as<float>() + staged, stage, (end - staged) * sizeof(float),
+                               hipMemcpyHostToDevice, c->up_stream),
+                "H2D pcm");
+      hip_check(hipEventRecord(c->ev_in[in_slot], c->up_stream), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(c->stream, c->ev_in[in_slot], 0), "hipStreamWaitEvent");
+      staged = end;
+    }
+    const size_t k = i % K;
+    float* d_re = d_slots + k * slot_floats;
+    // the download that last read this device slot (chunk i - K) has finished
+    if (i >= K) hip_check(hipStreamWaitEvent(c->stream, c->ev_out[k], 0), "hipStreamWaitEvent");
+    run_stft_frames(c, base, f0, nf, d_re, d_re + plane_stride, d_re + 2 * plane_stride, c->stream);
+    hip_check(hipEventRecord(c->ev_kernel[k], c->stream), "hipEventRecord");
+    // pinned slot k was copied out (chunk i - K) before this iteration began
+    hip_check(hipStreamWaitEvent(c->down_stream, c->ev_kernel[k], 0), "hipStreamWaitEvent");
+    float* h_re = h_slots + k * slot_floats;
+    if (nf == chunk) {
+      hip_check(hipMemcpyAsync(h_re, d_re, slot_floats * sizeof(float), hipMemcpyDeviceToHost, c->down_stream),
+                "D2H planes");
+    } else {
+      for (int p = 0; p < 3; ++p)
+        hip_check(hipMemcpyAsync(h_re + p * plane_stride, d_re + p * plane_stride, nf * plane_bytes,
+                                 hipMemcpyDeviceToHost, c->down_stream),
+                  "D2H planes");
+    }
+    hip_check(hipEventRecord(c->ev_out[k], c->down_stream), "hipEventRecord");
+    if (i + 1 >= K) copy_out(i + 1 - K);
+  }
+  for (size_t j = chunks >= K - 1 ? chunks - (K - 1) : 0; j < chunks; ++j) copy_out(j);
+  return *produced;
 }
 
 FftContext* make_context(size_t n, int threads, int lanes, int inverse, int kernel, int radix,
@@ -770,33 +915,13 @@ size_t fft_execute_batched(void* handle, const float* pcm, size_t pcm_len, float
   size_t produced = 0;
   try {
     fftfree::ensure_gpu(c);
-    const size_t bins = fftfree::bins_of(c);
-    c->d_pcm.reserve(pcm_len * sizeof(float));
-    fftfree::hip_check(hipMemcpyAsync(c->d_pcm.ptr, pcm, pcm_len * sizeof(float), hipMemcpyHostToDevice, c->stream),
-                       "H2D pcm");
-    const fftfree::StftArgs base = fftfree::stft_args(c, c->d_pcm.as<float>(), pcm_len);
-    // device planes in chunks of <= 768 MiB (three planes)
-    const size_t per_frame = 3 * bins * sizeof(float);
-    const size_t chunk = std::max<size_t>(1, std::min<size_t>(frames, (size_t(768) << 20) / per_frame));
-    c->d_planes.reserve(chunk * per_frame);
-    float* d_re = c->d_planes.as<float>();
-    float* d_im = d_re + chunk * bins;
-    float* d_mag = d_im + chunk * bins;
-    for (size_t f0 = 0; f0 < frames; f0 += chunk) {
-      const size_t nf = std::min(chunk, frames - f0);
-      fftfree::run_stft_frames(c, base, f0, nf, d_re, d_im, d_mag, c->stream);
-      const size_t bytes = nf * bins * sizeof(float);
-      fftfree::hip_check(hipMemcpyAsync(out_real + f0 * bins, d_re, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
-      fftfree::hip_check(hipMemcpyAsync(out_imag + f0 * bins, d_im, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
-      fftfree::hip_check(hipMemcpyAsync(out_mag + f0 * bins, d_mag, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
-      fftfree::hip_check(hipStreamSynchronize(c->stream), "sync");
-      produced = f0 + nf;
-    }
-    return produced;
+    return fftfree::stft_host_pipelined(c, pcm, pcm_len, frames, out_real, out_imag, out_mag, &produced);
   } catch (const std::exception& e) {
+    fftfree::drain_pipeline(c);
     fftfree::report("fft_execute_batched", e.what());
     return produced;
   } catch (...) {
+    fftfree::drain_pipeline(c);
     return produced;
   }
 }
