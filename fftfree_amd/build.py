@@ -14,7 +14,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfft_cffi.so"
-SOURCES = ["fft_stft.hip", "fft_kernels.hip", "fft_large.hip", "fft_adjoint.hip", "fft_ragged.hip", "fft_istft.hip", "fft_phase.hip",
+SOURCES = ["fft_stft.hip", "fft_stft_lines.hip", "fft_stft_mf.hip", "fft_kernels.hip", "fft_large.hip", "fft_adjoint.hip", "fft_ragged.hip", "fft_istft.hip", "fft_phase.hip",
            "ft_tools.hip", "plan.cpp", "fft_cffi.cpp", "ft_cffi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1,0 +1,21 @@
+// fft_stft_lines.hip -- instantiations of the line-aligned N = 1024 STFT kernel
+// (stft_r2c_lines_kernel, fft_stft_impl.hpp) and their rows of the variant table.
+#include "fft_stft_impl.hpp"
+
+namespace fftfree {
+namespace {
+const StftEntry kLines[] = {
+    FFTFREE_STFT_LN(0, 512, 8, 8, 3, 3, "8x8x8 lines wg=3 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(1, 512, 8, 8, 0, 3, "8x8x8 lines wg=4 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(2, 512, 8, 8, 3, 0, "8x8x8 lines-xcd-rounds wg=3 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(6, 512, 8, 2, 0, 3, "8x8x8 small-batch lines k=2", 8, 8, 8),
+    FFTFREE_STFT_LN(7, 512, 8, 1, 0, 3, "8x8x8 small-batch lines k=1", 8, 8, 8),
+};
+}  // namespace
+
+const StftEntry* stft_lines_entries(int* count) {
+  *count = (int)(sizeof(kLines) / sizeof(kLines[0]));
+  return kLines;
+}
+
+}  // namespace fftfree

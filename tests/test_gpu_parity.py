@@ -202,24 +202,9 @@ def test_stft_edge_cases(ff, case):
     assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
 
 
-def test_config2_full_size_every_frame(ff):
-    """Batched R2C STFT, Hann 1024, hop 256, half + magnitude, 2^24 samples: frame count
-    exact, every one of the 65533 frames of the magnitude plane against the FP64 oracle
-    (framed with pcm * win rounded to float32, in chunks), re = mag and im = 0 bit for
-    bit, and the host entry equal to the device entry."""
-    import torch
-    n, W, H = 1024, 1024, 256
-    pcm = noise_plus_sines(1 << 24, seed=2)
-    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=True, half_spectrum=True,
-                     apply_windows=True, analysis_window=1, pad_mode=0)
-    frames = ff.frame_count(p, pcm.size, 0)
-    assert frames == 65533
-    d_pcm = torch.from_numpy(pcm).cuda()
-    out = torch.empty((3, frames, n // 2 + 1), device="cuda", dtype=torch.float32)
-    assert ff.stft_device(p, d_pcm, out[0], out[1], out[2], pad_mode=0) == frames
-    torch.cuda.synchronize()
-    re, im, mg = (out[i].cpu().numpy() for i in range(3))
-    win = fp64.build_window(1, W)
+def _every_frame_rel_l2(pcm, mg, n, W, H, frames, win):
+    """Relative L2 of a magnitude plane over every frame against the FP64 oracle, framed
+    with pcm * win rounded to float32, in chunks of 4096 frames."""
     num = den = 0.0
     for f0 in range(0, frames, 4096):
         f1 = min(frames, f0 + 4096)
@@ -227,13 +212,47 @@ def test_config2_full_size_every_frame(ff):
         want = np.abs(np.fft.rfft(x.astype(np.float64), axis=-1))
         num += float(np.sum((mg[f0:f1].astype(np.float64) - want) ** 2))
         den += float(np.sum(want ** 2))
-    err = (num / den) ** 0.5
-    print(f"config 2 every frame: rel_l2 = {err:.3e} (tolerance {fp64.tolerance(n):.1e})", flush=True)
-    assert err <= fp64.tolerance(n)
-    assert np.array_equal(re.view(np.uint32), mg.view(np.uint32)) and not im.any()
-    # the device entry and the host entry produce the same bits
+    return (num / den) ** 0.5
+
+
+def test_config2_full_size_every_frame(ff):
+    """Batched R2C STFT, Hann 1024, hop 256, half + magnitude, 2^24 samples: frame count
+    exact, every one of the 65533 frames of the magnitude plane against the FP64 oracle,
+    re = mag and im = 0 bit for bit, for both plane layouts a caller can hand over:
+    three separate 16-byte aligned planes (the line-aligned kernel) and one (3, frames,
+    bins) tensor whose second and third planes start 4 bytes past a 16-byte boundary
+    (the general fused kernel, which rounds differently). The host entry, whose staging
+    slots are aligned, equals the aligned device launch bit for bit."""
+    import torch
+    n, W, H = 1024, 1024, 256
+    pcm = noise_plus_sines(1 << 24, seed=2)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=True, half_spectrum=True,
+                     apply_windows=True, analysis_window=1, pad_mode=0)
+    frames = ff.frame_count(p, pcm.size, 0)
+    assert frames == 65533
+    bins = n // 2 + 1
+    assert (frames * bins * 4) % 16 == 4  # the packed layout really is misaligned
+    d_pcm = torch.from_numpy(pcm).cuda()
+    win = fp64.build_window(1, W)
+    layouts = {
+        "aligned": [torch.empty((frames, bins), device="cuda", dtype=torch.float32) for _ in range(3)],
+        "packed": list(torch.empty((3, frames, bins), device="cuda", dtype=torch.float32).unbind(0)),
+    }
+    mags = {}
+    for name, out in layouts.items():
+        assert ff.stft_device(p, d_pcm, out[0], out[1], out[2], pad_mode=0) == frames
+        torch.cuda.synchronize()
+        re, im, mg = (o.cpu().numpy() for o in out)
+        err = _every_frame_rel_l2(pcm, mg, n, W, H, frames, win)
+        print(f"config 2 every frame ({name} planes): rel_l2 = {err:.3e} (tolerance {fp64.tolerance(n):.1e})",
+              flush=True)
+        assert err <= fp64.tolerance(n)
+        assert np.array_equal(re.view(np.uint32), mg.view(np.uint32)) and not im.any()
+        mags[name] = mg
+    # the host entry and the aligned device launch produce the same bits
     got, hre, him, hmg = ff.stft_frames(p, pcm[: 1 << 20], pad_mode=0)
-    assert np.array_equal(hmg, mg[:got])
+    assert got == 4093
+    assert np.array_equal(hmg, mags["aligned"][:got])
 
 
 def test_device_entry_matches_host_entry_bitwise(ff):
