@@ -31,7 +31,8 @@ minimum phase, timed alongside); ft_pipeline (ft_apply_complex_pipeline_device o
 default workload's complex planes with the reference tool's default quantizer and 3x3
 contrast, no resampling: the one-pass kernel); copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
-WRITE_SIZE counters).
+WRITE_SIZE counters); write_probe (the default workload's reads and plane writes with no
+transform, under several store forms and launch shapes: the floor memory sets).
 """
 from __future__ import annotations
 
@@ -53,7 +54,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "phase", "ft_pipeline",
-                                                     "c2c1024", "c2c_large", "stft_grad", "ragged", "copy"])
+                                                     "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -169,6 +170,65 @@ def stft_grid(ff, torch):
     common = {"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
               "data": "synthetic: STFT planes (Hann 1024, hop 256) of 0.1*U[-1,1] noise + 8 sinusoids, seed 2"}
     return n, H, frames, bins, frames * bins, re, im, mag, out_a, out_b, common
+
+
+def write_probe(args, torch):
+    """The default workload's memory traffic with no transform: fft_probe_planes reads each
+    32-frame tile's sample span (32*256 + 768 floats) and writes its region of the three
+    planes (32*513 floats each) for the 2047 tiles of 65533 frames, under several store
+    forms and launch shapes. The fastest full pattern is the floor memory alone sets for
+    the STFT kernel; the single-plane and read-only rows show the parts."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "fftfree_amd", "libfft_probe.so"))
+    lib.fft_probe_planes.restype = ctypes.c_int
+    lib.fft_probe_planes.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_void_p]
+    tiles, tile_hop, span4, tile4 = 2047, 32 * 256, (32 * 256 + 768) // 4, 32 * 513 // 4
+    pcm = torch.rand(tiles * tile_hop + 4 * span4, device="cuda")
+    planes = [torch.empty(tiles * tile4 * 4, device="cuda") for _ in range(3)]
+    NT, INTER, NOR, NOW, BUF, POL = 1, 2, 4, 8, 16, 32
+    cases = []
+    for grid, wpc in ((768, 3), (1024, 4), (2047, 0), (512, 2)):
+        for flags, what in ((BUF | NT, "buffer nt, plane after plane"), (BUF | NT | INTER, "buffer nt, planes interleaved"),
+                            (NT, "global nt, plane after plane"), (0, "global, plane after plane")):
+            cases.append((grid, wpc, flags, what, 3))
+    # cache-policy operand of the buffer stores: bit 0 sc0, bit 1 nt, bit 4 sc1
+    for grid, wpc in ((768, 3), (1024, 4), (512, 2)):
+        for pol in (0, 1, 2, 3, 16, 17, 18, 19):
+            cases.append((grid, wpc, BUF | INTER | POL | (pol << 8), f"buffer policy {pol}, planes interleaved", 3))
+    cases.append((2047, 0, BUF | NT | NOR, "buffer nt, no reads", 3))
+    cases.append((768, 3, BUF | NT | NOR, "buffer nt, no reads", 3))
+    cases.append((2047, 0, BUF | NT, "buffer nt, plane 0 only", 1))
+    cases.append((2047, 0, NOW, "reads only", 0))
+    stream = torch.cuda.current_stream().cuda_stream
+    rows = []
+    for grid, wpc, flags, what, nplanes in cases:
+        ptrs = [planes[i].data_ptr() if i < nplanes else None for i in range(3)]
+
+        def step():
+            rc = lib.fft_probe_planes(pcm.data_ptr(), tile_hop, span4, ptrs[0], ptrs[1], ptrs[2], tile4, tiles, grid, wpc,
+                                      flags, stream)
+            assert rc == 0, rc
+
+        gpu_s, _ = timed(step, args.steps, args.warmup, torch)
+        per = gpu_s / args.steps
+        rd = 0 if flags & NOR else tiles * span4 * 16
+        wr = 0 if flags & NOW else nplanes * tiles * tile4 * 16
+        rows.append({"grid": grid, "wpc": wpc, "stores": what, "planes": nplanes, "us": per * 1e6,
+                     "read_bytes": rd, "write_bytes": wr, "GBps": (rd + wr) / per / 1e9})
+        print(f"probe grid={grid} wpc={wpc} {what} planes={nplanes}: {per * 1e6:.2f} us "
+              f"{(rd + wr) / per / 1e9:.0f} GB/s", flush=True)
+    full = [r for r in rows if r["planes"] == 3 and r["read_bytes"] > 0]
+    best = min(full, key=lambda r: r["us"])
+    alg = 470526940
+    return {"metric": "HBM floor of the N=1024 STFT's memory pattern (probe, no transform)",
+            "value": best["GBps"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best["us"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "uniform samples, constant plane words",
+            "config": {"workload": "2047 tiles x (8960 samples read + 3 x 32 x 513 floats written)",
+                       "best": best, "stft_algorithmic_bytes": alg,
+                       "stft_floor_frac": alg / (best["us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, "probes": rows}}
 
 
 def main():
@@ -666,6 +726,8 @@ def main():
                              "frac": alg_bytes / per / 1e9 / HBM_PEAK_GBS, "traffic": read_traffic("stft_ragged_512"),
                              "algorithmic_bytes_per_launch": alg_bytes, "kernel": "stft_ragged_kernel"}}
         rb.close()
+    elif args.workload == "write_probe":
+        line = write_probe(args, torch)
     elif args.workload == "copy":
         nbytes = 512 << 20
         x = torch.ones(nbytes // 4, device="cuda")

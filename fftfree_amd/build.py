@@ -14,6 +14,9 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfft_cffi.so"
+# bandwidth probes for bench.py's write_probe workload (measurement only, not the library)
+PROBE_SRC = "probe.hip"
+PROBE_LIB = PKG / "libfft_probe.so"
 SOURCES = ["fft_stft.hip", "fft_stft_lines.hip", "fft_stft_mf.hip", "fft_kernels.hip", "fft_large.hip", "fft_adjoint.hip", "fft_ragged.hip", "fft_istft.hip", "fft_phase.hip",
            "ft_tools.hip", "plan.cpp", "fft_cffi.cpp", "ft_cffi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -42,18 +45,25 @@ def build(verbose: bool = False) -> Path:
     out = PKG / "_build"
     out.mkdir(exist_ok=True)
     objs = [out / (name + ".o") for name in SOURCES]
+    probe_obj = out / (PROBE_SRC + ".o")
     # translation units are independent: compile them side by side
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as pool:
-        for f in [pool.submit(_compile, CSRC / name, obj, verbose) for name, obj in zip(SOURCES, objs)]:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES) + 1, os.cpu_count() or 1)) as pool:
+        jobs = [pool.submit(_compile, CSRC / name, obj, verbose) for name, obj in zip(SOURCES, objs)]
+        jobs.append(pool.submit(_compile, CSRC / PROBE_SRC, probe_obj, verbose))
+        for f in jobs:
             f.result()
-    objs = [str(o) for o in objs]
-    tmp = LIB.with_suffix(".so.tmp")
+    _link([str(probe_obj)], PROBE_LIB, verbose)
+    _link([str(o) for o in objs], LIB, verbose)
+    return LIB
+
+
+def _link(objs, lib: Path, verbose: bool) -> None:
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    tmp.replace(LIB)
-    return LIB
+    tmp.replace(lib)
 
 
 if __name__ == "__main__":

@@ -493,14 +493,18 @@ __global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
 // OPT bit 0 deals tiles in launch order (consecutive tiles on different XCDs) instead
 // of contiguous runs per XCD; bit 1 makes pass p of an iteration write plane p's
 // rounds, so each wave has one write stream at a time. Both together measured fastest
-// at N = 1024 (87.7 us against 92 us for neither, 65533 frames).
+// at N = 1024 (65533 frames: 87-88 us against 92 us for neither). Bit 2 keeps the window
+// taps, the split twiddles and the second pass's twiddles in registers instead of LDS:
+// about a quarter less LDS traffic per frame for more registers (at most 3 workgroups
+// per CU); it measured no faster. The default runs 4 workgroups per CU (87 us; 3 per
+// CU: 88.5 us, 2 per CU: 95 us), tiles of 32 frames (64-frame tiles: 92 us).
 // Whole tiles of regular frames (every frame inside the PCM) run as above; they need the
 // window to span N, an even hop, 8-byte aligned samples and 16-byte aligned window and
 // planes. The frames after the last whole tile (a few dozen) go to workgroup 0,
 // dispatched first, which walks them P at a time with per-frame bounds, zero fill and
 // direct stores.
 template <int L, int E, int K, int OPT, int MODE, int DIR, int... Rs>
-__global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
+__global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(StftArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
   constexpr int NT = 256;
@@ -514,19 +518,21 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
   constexpr bool kB = mode_has_b<MODE>();
   constexpr int RUN = P * BINS;        // floats per plane per iteration
   constexpr int RING = ring_floats(RUN + 32);
+  constexpr bool kRegTabs = (OPT & 4) != 0;
   static_assert(T <= 64 && (64 % T) == 0, "one transform per wave or less");
   static_assert(JF * T == L / 2, "T must divide L/2");
   static_assert(RUN % 4 == 0, "iteration runs must be whole 16-byte words");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
   __shared__ float2 lds[P][LP];
   __shared__ __attribute__((aligned(16))) float ring[kB ? 2 : 1][RING];
-  __shared__ __attribute__((aligned(16))) float2 wtab[L];  // window taps x 1/2, as (x[2n], x[2n+1]) pairs
-  __shared__ float2 stw[L / 2 + 1];                        // split twiddles, p = 0 .. L/2
-  // second-pass twiddles: they depend on t mod R0 only, so they are reloaded from LDS
+  // window taps x 1/2, as (x[2n], x[2n+1]) pairs
+  __shared__ __attribute__((aligned(16))) float2 wtab[kRegTabs ? 2 : L];
+  __shared__ float2 stw[kRegTabs ? 1 : L / 2 + 1];  // split twiddles, p = 0 .. L/2
+  // second-pass twiddles: they depend on t mod R0 only, so (LDS tables) they are reloaded
   // before that pass of every frame instead of living in registers across the kernel
   constexpr int R1 = SecondOf<Rs...>::value;
   constexpr int NS1 = R0;  // the second pass's input span
-  __shared__ float2 tw1[NS1][R1 - 1];
+  __shared__ float2 tw1[kRegTabs ? 1 : NS1][R1 - 1];
 
   const int tid = threadIdx.x;
   // with one transform per wave the slot is wave-uniform: a scalar, so that every
@@ -547,24 +553,42 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
   float2 twr[E * NP];
   StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, 2);
 
-  // split and second-pass twiddles into LDS (read per frame there, so that four
-  // workgroups fit per CU)
-  for (int p = tid; p <= L / 2; p += NT) stw[p] = load_tw<DIR>(a.tw, p);
-  if (tid < NS1 * (R1 - 1)) {
-    const int m = tid / (R1 - 1), r = tid % (R1 - 1) + 1;
-    tw1[m][r - 1] = load_tw<DIR>(a.tw, m * (L / R1 / NS1) * 2 * r);
-  }
-  // window taps times 1/2 into LDS (1/2 when windows are off), one 16-byte load per
-  // thread; held there rather than in registers so that four workgroups fit per CU
-  static_assert(L % (2 * NT) == 0, "whole 16-byte words per thread");
+  // window taps x 1/2 and split twiddles in registers (bit 2), else in LDS tables
+  float2 wreg[kRegTabs ? E : 1];
+  float2 wsp[kRegTabs ? JF : 1];
+  float2 wmid = make_float2(0.0f, 0.0f);
+  if constexpr (kRegTabs) {
+    const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win ? win : reinterpret_cast<const float*>(a.tw));
 #pragma unroll
-  for (int c = 0; c < L / (2 * NT); ++c) {
-    const int w = tid + c * NT;  // 16-byte word: taps 4w .. 4w+3
-    // unconditional load (the twiddle table stands in for a missing window), then a
-    // select: a conditional load would become a branch and a wait
-    const f32x4 x = reinterpret_cast<const f32x4*>(win ? win : reinterpret_cast<const float*>(a.tw))[w];
-    const f32x4 h = {0.5f, 0.5f, 0.5f, 0.5f};
-    reinterpret_cast<f32x4*>(&wtab[0])[w] = win ? x * h : h;
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const float2 x = w2[t + q * T + r * (L / R0)];
+        wreg[q * R0 + r] = win ? make_float2(0.5f * x.x, 0.5f * x.y) : make_float2(0.5f, 0.5f);
+      }
+#pragma unroll
+    for (int j = 0; j < JF; ++j) wsp[j] = load_tw<DIR>(a.tw, t + j * T);
+    wmid = load_tw<DIR>(a.tw, L / 2);
+  } else {
+    // split and second-pass twiddles into LDS (read per frame there, so that four
+    // workgroups fit per CU)
+    for (int p = tid; p <= L / 2; p += NT) stw[p] = load_tw<DIR>(a.tw, p);
+    if (tid < NS1 * (R1 - 1)) {
+      const int m = tid / (R1 - 1), r = tid % (R1 - 1) + 1;
+      tw1[m][r - 1] = load_tw<DIR>(a.tw, m * (L / R1 / NS1) * 2 * r);
+    }
+    // window taps times 1/2 into LDS (1/2 when windows are off), one 16-byte load per
+    // thread; held there rather than in registers so that four workgroups fit per CU
+    static_assert(L % (2 * NT) == 0, "whole 16-byte words per thread");
+#pragma unroll
+    for (int c = 0; c < L / (2 * NT); ++c) {
+      const int w = tid + c * NT;  // 16-byte word: taps 4w .. 4w+3
+      // unconditional load (the twiddle table stands in for a missing window), then a
+      // select: a conditional load would become a branch and a wait
+      const f32x4 x = reinterpret_cast<const f32x4*>(win ? win : reinterpret_cast<const float*>(a.tw))[w];
+      const f32x4 h = {0.5f, 0.5f, 0.5f, 0.5f};
+      reinterpret_cast<f32x4*>(&wtab[0])[w] = win ? x * h : h;
+    }
   }
   __syncthreads();
 
@@ -581,12 +605,16 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
         const int e = q * R0 + r;
-        const float2 w = wtab[t + q * T + r * (L / R0)];
+        float2 w;
+        if constexpr (kRegTabs)
+          w = wreg[e];
+        else
+          w = wtab[t + q * T + r * (L / R0)];
         v[e] = make_float2(cur[e].x * w.x, cur[e].y * w.y);
       }
     after_window();
     WaveStockham<L, E, DIR, 1, 0, Rs...>::run(v, t, buf, twr, [&](int pass) {
-      if (pass == 0) {  // the second pass's twiddles (slots E .. of twr), fresh from LDS
+      if (!kRegTabs && pass == 0) {  // the second pass's twiddles (slots E .. of twr), fresh from LDS
 #pragma unroll
         for (int q = 0; q < E / R1; ++q)
 #pragma unroll
@@ -607,7 +635,12 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
       const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
       const float2 e = c_add(zp, zc);
       const float2 d = c_sub(zp, zc);
-      const float2 wo = c_mul(stw[p], make_float2(d.y, -d.x));
+      float2 wp;
+      if constexpr (kRegTabs)
+        wp = wsp[j];
+      else
+        wp = stw[p];
+      const float2 wo = c_mul(wp, make_float2(d.y, -d.x));
       float2 X1 = c_add(e, wo);
       float2 X2 = c_conj(c_sub(e, wo));
       if (p == 0) {  // bins 0 and L are real
@@ -622,15 +655,20 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
       const float2 zc = c_conj(zm);
       const float2 e = c_add(zm, zc);
       const float2 d = c_sub(zm, zc);
-      const float2 X = c_add(e, c_mul(stw[L / 2], make_float2(d.y, -d.x)));
+      float2 wm;
+      if constexpr (kRegTabs)
+        wm = wmid;
+      else
+        wm = stw[L / 2];
+      const float2 X = c_add(e, c_mul(wm, make_float2(d.y, -d.x)));
       if (t == 0) emit(L / 2, X);
     }
     __syncthreads();  // tiles: the ring holds this iteration; buf is free for the next frame
   };
 
-  if (tile < 0) {
-    // Frames [a.lead, a.frames), P at a time (the loop is uniform over the workgroup):
-    // per-frame bounds, zero fill past the PCM, direct plane stores.
+  // Frames [a.lead, a.frames), P at a time (the loop is uniform over the workgroup):
+  // per-frame bounds, zero fill past the PCM, direct plane stores.
+  auto rest_frames = [&]() {
     for (size_t fb = a.lead; fb < a.frames; fb += P) {
       const size_t f = fb + slot;
       const bool active = f < a.frames;
@@ -664,14 +702,17 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
             if (omg) omg[k] = plane_value<MODE, 2>(A, B);
           });
     }
+  };
+
+  if (tile < 0) {  // workgroup 0: the frames after the last whole tile
+    rest_frames();
     return;
   }
 
-  const size_t f0 = (size_t)tile * (size_t)(K * P);  // first frame of the tile
-  const size_t region = f0 * (size_t)BINS;             // tile's first float in each plane
-  // samples of iteration it's frame for this slot, first-pass order (8-byte loads: the
-  // host guarantees an even hop and 8-byte aligned samples)
-  auto load_frame = [&](int it, float2* dst) {
+  // samples of iteration it's frame for this slot in tile tl, first-pass order (8-byte
+  // loads: the host guarantees an even hop and 8-byte aligned samples)
+  auto load_frame = [&](long tl, int it, float2* dst) {
+    const size_t f0 = (size_t)tl * (size_t)(K * P);
     const float2* __restrict__ src =
         reinterpret_cast<const float2*>(a.pcm + (f0 + (size_t)(it * P + slot) + a.frame_offset) * (size_t)hop);
 #pragma unroll
@@ -680,15 +721,23 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
       for (int r = 0; r < R0; ++r) dst[q * R0 + r] = src[t + q * T + r * (L / R0)];
   };
 
-  // The tile's plane regions as buffer descriptors; store round c of iteration it
-  // writes the region's floats lo + 4*(tid + c*NT), whole 16-byte words; lanes past
-  // the iteration's range (only in its last round) get the dropped offset.
+  // A tile's plane regions as buffer descriptors; store round c of iteration it writes
+  // the region's floats lo + 4*(tid + c*NT), whole 16-byte words; lanes past the
+  // iteration's range (only in its last round) get the dropped offset.
   constexpr int kTileBytes = K * RUN * 4;
-  const __amdgpu_buffer_rsrc_t r0 = plane_rsrc(a.o_re, region, kTileBytes);
-  const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(a.o_im, region, kTileBytes);
-  const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(a.o_mag, region, kTileBytes);
+  struct Regions {
+    __amdgpu_buffer_rsrc_t r[3];
+  };
+  auto regions = [&](long tl) {
+    const size_t region = (size_t)tl * (size_t)(K * P) * (size_t)BINS;  // tile's first float in each plane
+    Regions g;
+    g.r[0] = plane_rsrc(a.o_re, region, kTileBytes);
+    g.r[1] = plane_rsrc(a.o_im, region, kTileBytes);
+    g.r[2] = plane_rsrc(a.o_mag, region, kTileBytes);
+    return g;
+  };
   // WHICH < 0: all three planes; else that plane only
-  auto store_round = [&](int it, int c, int which) {
+  auto store_round = [&](const Regions& rg, int it, int c, int which) {
     const int lo = lines_lo<RUN, K>(it), n4 = (lines_hi<RUN, K>(it) - lo) / 4;
     const int j = tid + c * NT;
     const int g = lo + 4 * j;
@@ -706,13 +755,30 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
       o2[u] = plane_value<MODE, 2>(A4[u], B4[u]);
     }
     const int off = (c * NT + NT <= n4 || j < n4) ? 4 * g : kDropOffset;
-    if (which < 0 || which == 0) store_nt16(o0, r0, off);
-    if (which < 0 || which == 1) store_nt16(o1, r1, off);
-    if (which < 0 || which == 2) store_nt16(o2, r2, off);
+    if (which < 0 || which == 0) store_nt16(o0, rg.r[0], off);
+    if (which < 0 || which == 1) store_nt16(o1, rg.r[1], off);
+    if (which < 0 || which == 2) store_nt16(o2, rg.r[2], off);
+  };
+  // iteration it's store rounds during pass `pass` of the next iteration
+  auto stores_in_pass = [&](const Regions& rg, int it, int pass) {
+    constexpr int C_PREV = lines_max_rounds<RUN, K, NT>();
+    const int rounds = lines_rounds<RUN, K, NT>(it);
+    if constexpr ((OPT & 2) && NP == 3) {
+      // OPT bit 1: pass p writes every round of plane p (one write stream at a time)
+#pragma unroll
+      for (int c = 0; c < C_PREV; ++c)
+        if (c < rounds) store_round(rg, it, c, pass);
+    } else {
+      const int per = (rounds + NP - 1) / NP;
+#pragma unroll
+      for (int c = 0; c < C_PREV; ++c)
+        if (c < rounds && c / per == pass) store_round(rg, it, c, -1);
+    }
   };
 
+  const Regions rg = regions(tile);
   float2 cur[E];
-  load_frame(0, cur);
+  load_frame(tile, 0, cur);
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     // bin k of slot s lands in the ring at float (i*P + s)*BINS + k
@@ -720,23 +786,10 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
     spectrum(
         cur,
         [&] {
-          if (i + 1 < K) load_frame(i + 1, cur);
+          if (i + 1 < K) load_frame(tile, i + 1, cur);
         },
         [&](int pass) {  // the previous iteration's store rounds, spread over the passes
-          if (i == 0) return;
-          constexpr int C_PREV = lines_max_rounds<RUN, K, NT>();
-          const int rounds = lines_rounds<RUN, K, NT>(i - 1);
-          if constexpr ((OPT & 2) && NP == 3) {
-            // OPT bit 1: pass p writes every round of plane p (one write stream at a time)
-#pragma unroll
-            for (int c = 0; c < C_PREV; ++c)
-              if (c < rounds) store_round(i - 1, c, pass);
-          } else {
-            const int per = (rounds + NP - 1) / NP;
-#pragma unroll
-            for (int c = 0; c < C_PREV; ++c)
-              if (c < rounds && c / per == pass) store_round(i - 1, c, -1);
-          }
+          if (i > 0) stores_in_pass(rg, i - 1, pass);
         },
         [&](int k, float2 X) {
           float A, B;
@@ -749,7 +802,7 @@ __global__ void __launch_bounds__(256, 4) stft_r2c_lines_kernel(StftArgs a) {
   {
     constexpr int rounds = lines_rounds<RUN, K, NT>(K - 1);
 #pragma unroll
-    for (int c = 0; c < rounds; ++c) store_round(K - 1, c, -1);
+    for (int c = 0; c < rounds; ++c) store_round(rg, K - 1, c, -1);
   }
 }
 
@@ -849,6 +902,14 @@ hipError_t stft_mf_launch_impl(const StftArgs& a, hipStream_t s) {
   return stft_launch_impl<L, E, P, 1, DIR, Rs...>(rest, s);
 }
 
+// Resident workgroups of a multi-frame variant (its magnitude, half-spectrum kernel).
+template <int L, int E, int K, int WPC, int OPT, int... Rs>
+size_t lines_slots() {
+  const size_t slots =
+      resident_slots(reinterpret_cast<const void*>(&stft_r2c_lines_kernel<L, E, K, OPT, kOutMagnitude, -1, Rs...>));
+  const size_t cap = (size_t)device_cus() * (WPC > 0 ? WPC : 1024);
+  return cap < slots ? cap : slots;
+}
 // Line-aligned launch: whole tiles of regular frames plus a short remainder go to one
 // stft_r2c_lines_kernel launch; anything it cannot take goes to stft_r2c_kernel. Full
 // spectra (bins = N) go to the tile-preload multi-frame kernel (their rows are whole
@@ -903,14 +964,6 @@ hipError_t stft_lines_launch_impl(const StftArgs& a, hipStream_t s) {
   return stft_launch_impl<L, E, P, 1, DIR, Rs...>(rest, s);
 }
 
-// Resident workgroups of a multi-frame variant (its magnitude, half-spectrum kernel).
-template <int L, int E, int K, int WPC, int OPT, int... Rs>
-size_t lines_slots() {
-  const size_t slots =
-      resident_slots(reinterpret_cast<const void*>(&stft_r2c_lines_kernel<L, E, K, OPT, kOutMagnitude, -1, Rs...>));
-  const size_t cap = (size_t)device_cus() * (WPC > 0 ? WPC : 1024);
-  return cap < slots ? cap : slots;
-}
 template <int L, int E, int K, int PL, int... Rs>
 size_t mf_slots() {
   return resident_slots(reinterpret_cast<const void*>(&stft_r2c_mf_kernel<L, E, K, PL, kOutMagnitude, 1, -1, Rs...>));

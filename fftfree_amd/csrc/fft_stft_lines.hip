@@ -5,11 +5,13 @@
 namespace fftfree {
 namespace {
 const StftEntry kLines[] = {
-    FFTFREE_STFT_LN(0, 512, 8, 8, 3, 3, "8x8x8 lines wg=3 k=8", 8, 8, 8),
-    FFTFREE_STFT_LN(1, 512, 8, 8, 0, 3, "8x8x8 lines wg=4 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(0, 512, 8, 8, 0, 3, "8x8x8 lines wg=4 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(1, 512, 8, 8, 3, 3, "8x8x8 lines wg=3 k=8", 8, 8, 8),
     FFTFREE_STFT_LN(2, 512, 8, 8, 3, 0, "8x8x8 lines-xcd-rounds wg=3 k=8", 8, 8, 8),
     FFTFREE_STFT_LN(6, 512, 8, 2, 0, 3, "8x8x8 small-batch lines k=2", 8, 8, 8),
     FFTFREE_STFT_LN(7, 512, 8, 1, 0, 3, "8x8x8 small-batch lines k=1", 8, 8, 8),
+    FFTFREE_STFT_LN(8, 512, 8, 8, 3, 7, "8x8x8 lines regs wg=3 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(9, 512, 8, 8, 3, 1, "8x8x8 lines mixed wg=3 k=8", 8, 8, 8),
 };
 }  // namespace
 

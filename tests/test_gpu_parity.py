@@ -350,3 +350,45 @@ def test_stft_multiframe_remainders(ff, frames, half):
     ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, fp64.build_window(1, W), False, 1, False, False, bool(half))
     assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= fp64.tolerance(n)
     assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("mode,samples", [((1, 0), 1 << 24), ((0, 0), 30000 * 256 + 768 + 100), ((0, 1), 26000 * 256 + 1000)])
+def test_lines_variants_same_bits_at_scale(ff, mode, samples):
+    """Every N=1024 line-aligned variant (tile order, stores per pass, register or LDS
+    tables, workgroups per CU, small-batch tile sizes, persistent grids that walk several
+    tiles per workgroup with the store pipeline running across tile edges) runs the same
+    per-frame arithmetic, so at sizes where persistent workgroups take two or three tiles
+    each, all three planes must equal the first variant's bit for bit; that one is checked
+    against the FP64 oracle on a frame sample."""
+    import torch
+    reduce_mag, polar = mode
+    n, W, H = 1024, 1024, 256
+    pcm = noise_plus_sines(samples, seed=29)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=bool(reduce_mag),
+                     store_polar=bool(polar), half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=1)
+    frames = ff.frame_count(p, pcm.size, 1)
+    bins = p.bins
+    d = torch.from_numpy(pcm).cuda()
+    names = ff.variants(p)
+    lines = [vi for vi, name in enumerate(names) if " lines" in name]
+    assert len(lines) >= 2 and lines[0] == 0
+    ref = None
+    for vi in lines:
+        assert ff.set_variant(p, vi)
+        planes = [torch.full((frames, bins), float("nan"), device="cuda") for _ in range(3)]
+        assert ff.stft_device(p, d, *planes, pad_mode=1) == frames
+        torch.cuda.synchronize()
+        got = [x.cpu().numpy() for x in planes]
+        if ref is None:
+            ref = got
+            sel = np.unique(np.concatenate([np.arange(64), np.linspace(0, frames - 1, 200).astype(int),
+                                            np.arange(frames - 40, frames)]))
+            win = fp64.build_window(1, W)
+            x = np.concatenate([fp64.frame_signal(pcm[f * H:f * H + W], n, W, H, 1, win) for f in sel])
+            ore, oim, omg = fp64.post_planes(fp64.fft64(x.astype(np.float64)), n, 1, reduce_mag, polar, True)
+            assert spectra_close(got[0][sel], got[1][sel], ore, oim, polar, fp64.tolerance(n))
+            assert fp64.rel_l2(got[2][sel], omg) <= fp64.tolerance(n)
+        else:
+            for i in range(3):
+                assert np.array_equal(got[i].view(np.uint32), ref[i].view(np.uint32)), (names[vi], i)
+    ff.set_variant(p, -1)
