@@ -81,6 +81,8 @@ SIGNATURES = {
                                            c_int, c_void_p]),
     "fft_stft_adjoint_device_noalloc": (c_size_t, [c_void_p, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p,
                                                    c_void_p, c_int, c_void_p]),
+    "fft_stft_adjoint_saved_device": (c_size_t, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_void_p,
+                                                 c_void_p, c_void_p, c_int, c_void_p]),
     "fft_ragged_create": (c_void_p, [ctypes.POINTER(RaggedRequest), c_size_t, c_int, c_int, c_int, c_int, c_int]),
     "fft_ragged_autotune": (c_int, [c_void_p, c_int]),
     "fft_ragged_execute": (c_int, [c_void_p, c_void_p]),

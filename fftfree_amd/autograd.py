@@ -3,10 +3,13 @@
 ``stft_polar(x, ...)`` maps a (signals, samples) float32 CUDA tensor to the magnitude
 and phase planes (signals, frames, N/2+1) that a R2C half-spectrum, store_polar context
 writes (fft_cffi.cpp:811-825 after the post hook, eigen_fft.hpp:2535-2596). Its backward
-is the hand-written adjoint kernel (fft_stft_adjoint_device): it recomputes each frame's
-spectrum from the saved samples (nothing else is saved), forms
-V = dL/d|X| * X/|X| + i dL/d(arg X) * X/|X|^2, runs the real inverse DFT of the half
-spectrum, applies the analysis window and overlap-adds in a fixed order.
+is the hand-written adjoint kernel run on the forward's own outputs
+(fft_stft_adjoint_saved_device): with the saved |X| and arg X it forms
+V = dL/d|X| * e^{i arg X} + i dL/d(arg X) * e^{i arg X}/|X| (0 where |X| = 0), runs the
+real inverse DFT of the half spectrum, applies the analysis window and overlap-adds in a
+fixed order; no forward transform is recomputed and the samples are not kept.
+fft_stft_adjoint_device, which recomputes each frame's spectrum from the samples, is the
+same gradient for callers that keep only the samples.
 
 The phase-recovery and weight-training tools of the reference (phase_infer.hpp,
 train_mipmap_weights.py) optimise through exactly this map.
@@ -53,16 +56,17 @@ class StftPolarFunction(torch.autograd.Function):
         mag = torch.empty((x.shape[0], frames, bins), device=x.device, dtype=torch.float32)
         phase = torch.empty_like(mag)
         ff.stft_signals_device(plan, x, mag, phase, None)
-        ctx.save_for_backward(x)
+        ctx.save_for_backward(mag, phase)
         ctx.plan = plan
+        ctx.samples = x.shape[1]
         return mag, phase
 
     @staticmethod
     def backward(ctx, grad_mag, grad_phase):
-        (x,) = ctx.saved_tensors
+        mag, phase = ctx.saved_tensors
         gm = grad_mag.contiguous() if grad_mag is not None else None
         gp = grad_phase.contiguous() if grad_phase is not None else None
-        grad = _lib().stft_adjoint_device(ctx.plan, x, gm, gp)
+        grad = _lib().stft_adjoint_saved_device(ctx.plan, ctx.samples, mag, phase, gm, gp)
         return grad, None
 
 

@@ -283,6 +283,27 @@ class CffiFft:
             raise RuntimeError("fft_stft_adjoint_device failed")
         return out
 
+    def stft_adjoint_saved_device(self, plan: FftPlan, samples: int, plane_re=None, plane_im=None, grad_re=None,
+                                  grad_im=None, out=None, pad_mode: int = 0):
+        """fft_stft_adjoint_saved_device: gradient of the samples from the forward's own
+        planes (|X|, arg X for a polar context; unused for complex planes) and the
+        gradients of those planes ((signals, frames, bins) tensors or None); `samples` per
+        signal. Returns out, a (signals, samples) tensor."""
+        import torch
+        ref = next(t for t in (grad_re, grad_im, plane_re, plane_im, out) if t is not None)
+        signals = ref.shape[0]
+        out = torch.empty((signals, samples), device=ref.device, dtype=torch.float32) if out is None else out
+        assert out.shape == (signals, samples) and out.is_contiguous()
+        for t in (plane_re, plane_im, grad_re, grad_im):
+            assert t is None or (t.is_contiguous() and t.shape[0] == signals)
+        ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        got = int(self.lib.fft_stft_adjoint_saved_device(plan.handle, ptr(plane_re), ptr(plane_im), signals, samples,
+                                                         ptr(grad_re), ptr(grad_im), ctypes.c_void_p(out.data_ptr()),
+                                                         pad_mode, self._stream()))
+        if got == 0:
+            raise RuntimeError("fft_stft_adjoint_saved_device failed")
+        return out
+
 
 class RaggedBatch:
     """A ragged batch over torch CUDA tensors (fft_ragged_*): requests are

@@ -22,6 +22,10 @@
 // (the mirror image of the forward split). The inverse runs as conj(DFT(conj Y)) on the
 // forward engine with the forward twiddles.
 //
+// From saved planes (SAVED): the forward's own output planes give V directly (polar:
+// V = e^{i arg X} (Ga + i Gb/|X|), 0 where |X| = 0; complex: V = Ga + i Gb), so no frame
+// is transformed forward and the samples are not read; the rest is the same.
+//
 // Overlap-add runs in a fixed order, so the result is deterministic. A workgroup owns a
 // run of consecutive frame positions of one signal and accumulates its frames' windowed
 // r_f, in ascending frame order, in an LDS ring; each sample is written once. The W-H
@@ -234,7 +238,7 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
 // The first W-H samples of the run go to the left halo slot, and after the last step
 // the samples past the run that its frames touch go to the right halo slot;
 // stft_halo_kernel completes them.
-template <int L, int E, int MODE, int ACCM, int... Rs>
+template <int L, int E, int MODE, int ACCM, bool SAVED, int... Rs>
 __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
@@ -262,9 +266,12 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   const int tix = (int)(tile - sig * a.tiles);  // run index within the signal
   const long long f0 = (long long)tix * FPS * steps;
   const long long fend = f0 + FPS * steps;
-  const float* __restrict__ x = a.pcm + sig * a.len;
+  const float* __restrict__ x = SAVED ? nullptr : a.pcm + sig * a.len;  // saved planes: samples unused
   const float* __restrict__ ga = a.g_mag ? a.g_mag + sig * a.frames * BINS : nullptr;
   const float* __restrict__ gb = a.g_phase ? a.g_phase + sig * a.frames * BINS : nullptr;
+  // saved planes (SAVED, polar): the forward's |X| and arg X of this signal
+  const float* __restrict__ pmg = SAVED && MODE == kOutPolar ? a.p_mag + sig * a.frames * BINS : nullptr;
+  const float* __restrict__ pph = SAVED && MODE == kOutPolar ? a.p_phase + sig * a.frames * BINS : nullptr;
   float* __restrict__ gx = a.grad + sig * a.len;
   const long long frames = (long long)a.frames;
   const long long len = (long long)a.len;
@@ -278,16 +285,75 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   for (int j = 0; j < JF; ++j) wsp[j] = load_tw<-1>(a.tw, t + j * T);
   const float2 wmid = load_tw<-1>(a.tw, L / 2);
   stage_window<N>(wl, a.win);
+  if constexpr (SAVED) {
+    // the forward's zero padding: taps past W reach no sample
+    for (int i = a.window + tid; i < N; i += 256) wl[i] = 0.0f;
+  }
   for (int i = tid; i < ACC; i += 256) acc[i] = 0.0f;
   __syncthreads();
 
   const int halo = W > (int)hop ? W - (int)hop : 0;  // samples shared with the previous run
   auto frame_valid = [&](long long f) { return f >= 0 && f < frames; };
-  // samples of the first step's frame; later steps' samples are prefetched a step ahead
-  float2 nxt[E];
+  // samples of the first step's frame (recomputing) or its plane values (saved); later
+  // steps' are prefetched a step ahead
+  float2 nxt[SAVED ? 1 : E];
+  constexpr bool kPolarSaved = SAVED && MODE == kOutPolar;
+  float sa[SAVED ? 2 * JF + 1 : 1], sb[SAVED ? 2 * JF + 1 : 1];
+  float sm[kPolarSaved ? 2 * JF + 1 : 1], sp[kPolarSaved ? 2 * JF + 1 : 1];
+  const bool has_a = ga != nullptr, has_b = gb != nullptr;
+  // plane values of frame f (index clamped so the loads are unconditional; a missing
+  // gradient plane reads the other one, and its values are not used; the launcher never
+  // runs this kernel with both missing)
+  auto fetch_saved = [&](long long f) {
+    const size_t row = (size_t)(f < 0 ? 0 : (f >= frames ? frames - 1 : f)) * BINS;
+    const float* __restrict__ pa = (has_a ? ga : gb) + row;
+    const float* __restrict__ pb = (has_b ? gb : ga) + row;
+#pragma unroll
+    for (int j = 0; j < JF; ++j) {
+      const int p = t + j * T;
+      sa[2 * j] = pa[p];
+      sa[2 * j + 1] = pa[L - p];
+      sb[2 * j] = pb[p];
+      sb[2 * j + 1] = pb[L - p];
+    }
+    sa[2 * JF] = pa[L / 2];
+    sb[2 * JF] = pb[L / 2];
+    if constexpr (kPolarSaved) {
+#pragma unroll
+      for (int j = 0; j < JF; ++j) {
+        const int p = t + j * T;
+        sm[2 * j] = pmg[row + p];
+        sm[2 * j + 1] = pmg[row + L - p];
+        sp[2 * j] = pph[row + p];
+        sp[2 * j + 1] = pph[row + L - p];
+      }
+      sm[2 * JF] = pmg[row + L / 2];
+      sp[2 * JF] = pph[row + L / 2];
+    }
+  };
+  // V of plane values (ga, gb, |X|, arg X); bins 0 and L take no Gb term
+  auto v_saved = [&](int i, bool real_bin) -> float2 {
+    const float g1 = has_a ? sa[i] : 0.0f;
+    const float g2 = has_b && !real_bin ? sb[i] : 0.0f;
+    if constexpr (kPolarSaved) {
+      const float m = sm[i];
+      if (m == 0.0f) return make_float2(0.0f, 0.0f);
+      // hardware sine and cosine: arg X lies in [-pi, pi], where their error (a few 1e-7
+      // of the unit direction) stays far inside the gradient's bound
+      float sn, cs;
+      __sincosf(sp[i], &sn, &cs);
+      const float q = g2 / m;
+      return make_float2(cs * g1 - sn * q, sn * g1 + cs * q);
+    } else {
+      return make_float2(g1, g2);
+    }
+  };
   {
     const long long f = f0 + slot;
-    fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(f) ? (size_t)(f * hop) : 0, W, frame_valid(f));
+    if constexpr (SAVED)
+      fetch_saved(f);
+    else
+      fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(f) ? (size_t)(f * hop) : 0, W, frame_valid(f));
   }
   // Final samples of the previous step wait in the ring until the next step has used its
   // gradients; they are stored then, a whole transform ahead of the next wait for loads
@@ -313,50 +379,8 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
     const long long f = fbase + slot;
     const bool valid = frame_valid(f);  // invalid groups run on zeros (barriers stay uniform)
 
-    // 1. r_f into buf
-    float2 v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = nxt[e];
-    window_frame<L, E, R0>(v, t, wl);
-    // Wait here for the samples (and the stores issued with them, a transform ago), so
-    // that the gradient loads below are not part of that wait: s_waitcnt vmcnt(0)
-    // expcnt(7) lgkmcnt(15).
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    __builtin_amdgcn_sched_barrier(0);
-    // upstream gradients of this frame's bins, requested before the transform (a clamped
-    // row: an invalid frame's spectrum is zero, or its r_f is never added)
-    const size_t row = (size_t)(valid ? f : (f < 0 ? 0 : frames - 1)) * BINS;
-    float gAp[JF], gAq[JF], gBp[JF], gBq[JF];
-#pragma unroll
-    for (int j = 0; j < JF; ++j) {
-      const int p = t + j * T;
-      gAp[j] = ga ? ga[row + p] : 0.0f;
-      gAq[j] = ga ? ga[row + L - p] : 0.0f;
-      gBp[j] = gb ? gb[row + p] : 0.0f;
-      gBq[j] = gb ? gb[row + L - p] : 0.0f;
-    }
-    const float gAm = ga ? ga[row + L / 2] : 0.0f;
-    const float gBm = gb ? gb[row + L / 2] : 0.0f;
-    float2* fb;
-    StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
-    store_natural<L, E, Rl>(v, t, buf);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < JF; ++j) {
-      const int p = t + j * T;
-      const float2 zp = buf[pidx(p)];
-      const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
-      const float2 e = c_scale(c_add(zp, zc), 0.5f);
-      const float2 d = c_sub(zp, zc);
-      const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
-      float2 X1 = c_add(e, wo);
-      float2 X2 = c_conj(c_sub(e, wo));
-      if (p == 0) {  // bins 0 and L are real
-        X1.y = 0.0f;
-        X2.y = 0.0f;
-      }
-      const float2 Vp = grad_spectrum<MODE>(X1, gAp[j], gBp[j], p == 0);
-      const float2 Vq = grad_spectrum<MODE>(X2, gAq[j], gBq[j], p == 0);  // V_{L-p}
+    // conj(Y_p) and conj(Y_{L-p}) into buf from V_p and V_{L-p} (thread t owns the pair)
+    auto put_pair = [&](int j, int p, float2 Vp, float2 Vq) {
       if (p == 0) {
         buf[pidx(0)] = make_float2(Vp.x + Vq.x, -(Vp.x - Vq.x));
       } else {
@@ -373,14 +397,73 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
         buf[pidx(p)] = c_conj(Yp);
         buf[pidx(L - p)] = c_conj(Yq);
       }
-    }
-    if (t == 0) {  // p = L/2: e = i, so Y = Re V + i * i * (i Im V) = conj(V): conj(Y) = V
-      const float2 zm = buf[pidx(L / 2)];
-      const float2 zc = c_conj(zm);
-      const float2 e = c_scale(c_add(zm, zc), 0.5f);
-      const float2 d = c_sub(zm, zc);
-      const float2 Xm = c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x)));
-      buf[pidx(L / 2)] = grad_spectrum<MODE>(Xm, gAm, gBm, false);
+    };
+    // 1. r_f into buf
+    float2 v[E];
+    float2* fb;
+    if constexpr (SAVED) {
+      // V from the prefetched plane values, then Y as below (invalid frames: V = 0)
+      const float keep = valid ? 1.0f : 0.0f;
+#pragma unroll
+      for (int j = 0; j < JF; ++j) {
+        const int p = t + j * T;
+        const float2 Vp = c_scale(v_saved(2 * j, p == 0), keep);
+        const float2 Vq = c_scale(v_saved(2 * j + 1, p == 0), keep);  // V_{L-p}
+        put_pair(j, p, Vp, Vq);
+      }
+      if (t == 0) buf[pidx(L / 2)] = c_scale(v_saved(2 * JF, false), keep);
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = nxt[e];
+      window_frame<L, E, R0>(v, t, wl);
+      // Wait here for the samples (and the stores issued with them, a transform ago), so
+      // that the gradient loads below are not part of that wait: s_waitcnt vmcnt(0)
+      // expcnt(7) lgkmcnt(15).
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      __builtin_amdgcn_sched_barrier(0);
+      // upstream gradients of this frame's bins, requested before the transform (a clamped
+      // row: an invalid frame's spectrum is zero, or its r_f is never added)
+      const size_t row = (size_t)(valid ? f : (f < 0 ? 0 : frames - 1)) * BINS;
+      float gAp[JF], gAq[JF], gBp[JF], gBq[JF];
+#pragma unroll
+      for (int j = 0; j < JF; ++j) {
+        const int p = t + j * T;
+        gAp[j] = ga ? ga[row + p] : 0.0f;
+        gAq[j] = ga ? ga[row + L - p] : 0.0f;
+        gBp[j] = gb ? gb[row + p] : 0.0f;
+        gBq[j] = gb ? gb[row + L - p] : 0.0f;
+      }
+      const float gAm = ga ? ga[row + L / 2] : 0.0f;
+      const float gBm = gb ? gb[row + L / 2] : 0.0f;
+      StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+      store_natural<L, E, Rl>(v, t, buf);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < JF; ++j) {
+        const int p = t + j * T;
+        const float2 zp = buf[pidx(p)];
+        const float2 zc = c_conj(buf[pidx((L - p) & (L - 1))]);
+        const float2 e = c_scale(c_add(zp, zc), 0.5f);
+        const float2 d = c_sub(zp, zc);
+        const float2 wo = c_mul(wsp[j], make_float2(0.5f * d.y, -0.5f * d.x));
+        float2 X1 = c_add(e, wo);
+        float2 X2 = c_conj(c_sub(e, wo));
+        if (p == 0) {  // bins 0 and L are real
+          X1.y = 0.0f;
+          X2.y = 0.0f;
+        }
+        const float2 Vp = grad_spectrum<MODE>(X1, gAp[j], gBp[j], p == 0);
+        const float2 Vq = grad_spectrum<MODE>(X2, gAq[j], gBq[j], p == 0);  // V_{L-p}
+        put_pair(j, p, Vp, Vq);
+      }
+      if (t == 0) {  // p = L/2: e = i, so Y = Re V + i * i * (i Im V) = conj(V): conj(Y) = V
+        const float2 zm = buf[pidx(L / 2)];
+        const float2 zc = c_conj(zm);
+        const float2 e = c_scale(c_add(zm, zc), 0.5f);
+        const float2 d = c_sub(zm, zc);
+        const float2 Xm = c_add(e, c_mul(wmid, make_float2(0.5f * d.y, -0.5f * d.x)));
+        buf[pidx(L / 2)] = grad_spectrum<MODE>(Xm, gAm, gBm, false);
+      }
     }
     // every gradient load has been used (lane 0 of a frame also used the middle bin's):
     // waiting here costs nothing and keeps the stores below out of later waits
@@ -401,9 +484,12 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
         const float2 y = v[q * Rl + r];
         buf[pidx(t + q * T + r * (L / Rl))] = make_float2(y.x, -y.y);
       }
-    if (step + 1 < steps) {  // the next step's samples, in flight during steps 2 and 3
+    if (step + 1 < steps) {  // the next step's samples or plane values, in flight during steps 2 and 3
       const long long fn = f + FPS;
-      fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(fn) ? (size_t)(fn * hop) : 0, W, frame_valid(fn));
+      if constexpr (SAVED)
+        fetch_saved(fn);
+      else
+        fetch_frame<L, E, R0>(nxt, t, x, a.len, frame_valid(fn) ? (size_t)(fn * hop) : 0, W, frame_valid(fn));
     }
     __syncthreads();
 
@@ -482,6 +568,11 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
   }
 }
 
+// dL/dx = 0 (an adjoint call with no upstream gradient).
+__global__ void __launch_bounds__(256) zero_grad_kernel(float* __restrict__ out, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) out[i] = 0.0f;
+}
+
 // Halo samples of every run boundary: previous run's partial sum + this run's. One
 // workgroup per boundary, boundaries of all signals in turn.
 __global__ void __launch_bounds__(256) stft_halo_kernel(SignalArgs a, int halo, size_t run_len) {
@@ -522,16 +613,32 @@ hipError_t signals_launch(const SignalArgs& a, int mode, hipStream_t s) {
 // Plans the adjoint launch of `a` (steps per run, runs per signal, halo slots) and
 // launches it; with `need`, only reports the halo floats that launch would use.
 template <int L, int E, int ACCM, int... Rs>
-hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s, size_t* need) {
+hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s, size_t* need, bool saved) {
   constexpr int FPS = 256 / (L / E);
   const void* kern = nullptr;
-  switch (mode) {
-    case kOutComplex: kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutComplex, ACCM, Rs...>); break;
-    case kOutMagnitude:
-      kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutMagnitude, ACCM, Rs...>);
-      break;
-    case kOutPolar: kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutPolar, ACCM, Rs...>); break;
-    default: return hipErrorInvalidValue;
+  if (saved) {
+    switch (mode) {
+      case kOutComplex:
+        kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutComplex, ACCM, true, Rs...>);
+        break;
+      case kOutPolar:
+        kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutPolar, ACCM, true, Rs...>);
+        break;
+      default: return hipErrorInvalidValue;  // magnitude planes hold no direction of X
+    }
+  } else {
+    switch (mode) {
+      case kOutComplex:
+        kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutComplex, ACCM, false, Rs...>);
+        break;
+      case kOutMagnitude:
+        kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutMagnitude, ACCM, false, Rs...>);
+        break;
+      case kOutPolar:
+        kern = reinterpret_cast<const void*>(&stft_adjoint_kernel<L, E, kOutPolar, ACCM, false, Rs...>);
+        break;
+      default: return hipErrorInvalidValue;
+    }
   }
   int dev = 0, cus = 0, occ = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -570,10 +677,10 @@ hipError_t adjoint_launch_acc(SignalArgs a, int mode, hipStream_t s, size_t* nee
 
 // The smallest accumulator ring (2N, 4N or 8N samples) that holds one step's span.
 template <int L, int E, int... Rs>
-hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s, size_t* need) {
-  if (span <= 4 * L) return adjoint_launch_acc<L, E, 2, Rs...>(a, mode, s, need);
-  if (span <= 8 * L) return adjoint_launch_acc<L, E, 4, Rs...>(a, mode, s, need);
-  return adjoint_launch_acc<L, E, 8, Rs...>(a, mode, s, need);
+hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s, size_t* need, bool saved) {
+  if (span <= 4 * L) return adjoint_launch_acc<L, E, 2, Rs...>(a, mode, s, need, saved);
+  if (span <= 8 * L) return adjoint_launch_acc<L, E, 4, Rs...>(a, mode, s, need, saved);
+  return adjoint_launch_acc<L, E, 8, Rs...>(a, mode, s, need, saved);
 }
 
 }  // namespace
@@ -634,17 +741,32 @@ hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s) {
   }
 }
 
-hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need) {
+hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need, bool saved) {
   const int n = a.n;
   if (!signals_supported(n, a.window, a.hop, true) || a.len == 0 || !a.grad) return hipErrorInvalidValue;
+  if (saved && mode == kOutPolar && (!a.p_mag || !a.p_phase)) return hipErrorInvalidValue;
+  if (saved && !a.g_mag && !a.g_phase) {
+    // no upstream gradient: the saved-plane kernel would have no plane to read its
+    // gradient values from, and dL/dx is 0
+    if (halo_need) {
+      *halo_need = 0;
+      return hipSuccess;
+    }
+    const size_t total = a.signals * a.len;
+    size_t grid = (total + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(zero_grad_kernel, dim3((unsigned)grid), dim3(256), 0, s, a.grad, total);
+    return hipGetLastError();
+  }
   const int fps = adjoint_fps(n);
   const int span = a.window + fps * a.hop;
-  // with no phase gradient, polar V = Ga X/|X| is the magnitude map (fewer registers)
-  if (mode == kOutPolar && !a.g_phase) mode = kOutMagnitude;
+  // recomputing, with no phase gradient: polar V = Ga X/|X| is the magnitude map (fewer
+  // registers)
+  if (!saved && mode == kOutPolar && !a.g_phase) mode = kOutMagnitude;
   switch (n) {
-    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, span, mode, s, halo_need);
-    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, span, mode, s, halo_need);
-    default: return adjoint_launch<1024, 8, 8, 8, 8, 2>(a, span, mode, s, halo_need);
+    case 512: return adjoint_launch<256, 4, 4, 4, 4, 4>(a, span, mode, s, halo_need, saved);
+    case 1024: return adjoint_launch<512, 8, 8, 8, 8>(a, span, mode, s, halo_need, saved);
+    default: return adjoint_launch<1024, 8, 8, 8, 8, 2>(a, span, mode, s, halo_need, saved);
   }
 }
 

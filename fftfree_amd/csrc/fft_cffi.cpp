@@ -1032,6 +1032,49 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
   }
 }
 
+size_t fft_stft_adjoint_saved_device(void* handle, const float* d_plane_real, const float* d_plane_imag,
+                                     size_t signals, size_t pcm_len, const float* d_grad_real,
+                                     const float* d_grad_imag, float* d_grad_pcm, int pad_mode, void* stream) {
+  if (!handle || !d_grad_pcm || signals == 0 || pcm_len == 0) return 0;
+  auto* c = static_cast<FftContext*>(handle);
+  if (!signals_ok(c, true)) return 0;
+  if (c->reduce_magnitude && !c->store_polar) {
+    fftfree::report("fft_stft_adjoint_saved_device",
+                    "magnitude planes do not hold the direction of X: use fft_stft_adjoint_device");
+    return 0;
+  }
+  const bool polar = c->store_polar;
+  if (polar && (!d_plane_real || !d_plane_imag)) {
+    fftfree::report("fft_stft_adjoint_saved_device", "polar planes |X| and arg X are required");
+    return 0;
+  }
+  const size_t frames = fftfree::frame_count(c, pcm_len, pad_mode, 0);
+  if (frames == 0) return 0;
+  try {
+    fftfree::ensure_gpu(c);
+    fftfree::SignalArgs a = signal_args(c, nullptr, signals, pcm_len, frames);
+    a.g_mag = d_grad_real;
+    a.g_phase = d_grad_imag;
+    a.p_mag = polar ? d_plane_real : nullptr;
+    a.p_phase = polar ? d_plane_imag : nullptr;
+    a.grad = d_grad_pcm;
+    const size_t halo_floats = fftfree::adjoint_scratch(c->N, a.window, a.hop, signals, pcm_len);
+    if (halo_floats > 0) {
+      c->d_halo.reserve(halo_floats * sizeof(float));
+      a.halo = c->d_halo.as<float>();
+      a.halo_capacity = c->d_halo.bytes / sizeof(float);
+    }
+    fftfree::hip_check(fftfree::launch_stft_adjoint(a, polar ? 3 : 0, static_cast<hipStream_t>(stream), nullptr, true),
+                       "launch_stft_adjoint(saved)");
+    return frames;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_stft_adjoint_saved_device", e.what());
+    return 0;
+  } catch (...) {
+    return 0;
+  }
+}
+
 size_t fft_stft_adjoint_device_noalloc(void* handle, const float* d_pcm, size_t signals, size_t pcm_len,
                                        const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm,
                                        int pad_mode, void* stream) {

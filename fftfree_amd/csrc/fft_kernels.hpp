@@ -66,6 +66,10 @@ struct SignalArgs {
   float* o_mag = nullptr;
   const float* g_mag = nullptr;    // adjoint: gradient of plane re (|X| when polar/magnitude)
   const float* g_phase = nullptr;  // adjoint: gradient of plane im (arg X when polar), may be nullptr
+  // adjoint from the forward's saved polar planes (launch_stft_adjoint with `saved`):
+  // V = e^{i p_phase} (g_mag + i g_phase / p_mag), 0 where p_mag = 0; pcm is not read
+  const float* p_mag = nullptr;
+  const float* p_phase = nullptr;
   float* grad = nullptr;           // adjoint output: dL/dpcm, signals*len floats
   int tiles = 0;                   // set by the launchers: workgroups per signal
   int steps = 0;                   // adjoint: steps per workgroup
@@ -179,8 +183,12 @@ bool large_variant_desc(size_t n, int variant, char* buf, size_t len);
 // (k = 2 for N = 2048, else 4).
 bool signals_supported(int n, int window, int hop, bool adjoint);
 hipError_t launch_stft_signals(SignalArgs a, int mode, hipStream_t s);
-// With halo_need, launches nothing and reports the halo floats the launch would use.
-hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need = nullptr);
+// With halo_need, launches nothing and reports the halo floats the launch would use. With
+// `saved`, V comes from the gradients and the forward's saved planes (complex: V = g_mag +
+// i g_phase; polar: p_mag, p_phase) and no frame is transformed forward; magnitude mode is
+// refused there.
+hipError_t launch_stft_adjoint(SignalArgs a, int mode, hipStream_t s, size_t* halo_need = nullptr,
+                               bool saved = false);
 // Device address (on the current device) and size in floats of the static halo scratch
 // of the allocation-free adjoint entry; nullptr if the lookup fails.
 float* noalloc_halo_scratch(size_t* capacity);

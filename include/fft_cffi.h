@@ -225,6 +225,19 @@ size_t fft_stft_adjoint_device(void* handle, const float* d_pcm, size_t signals,
                                const float* d_grad_real, const float* d_grad_imag, float* d_grad_pcm, int pad_mode,
                                void* stream);
 
+/* additive. The same adjoint from the forward's own output planes instead of the samples:
+ * d_plane_real / d_plane_imag are the first two planes fft_stft_signals_device wrote for
+ * this context and signals (|X| and arg X when store_polar; not read for complex planes,
+ * may be NULL there). V = dL/d|X| e^{i arg X} + i dL/d(arg X) e^{i arg X} / |X| (0 where
+ * |X| = 0) or dL/dre + i dL/dim, then one inverse real DFT per frame, the analysis window
+ * and the overlap-add: no forward transform is recomputed. A magnitude-only context is
+ * refused (its planes do not hold the direction of X). The same window/hop limits as
+ * fft_stft_adjoint_device apply. Either gradient may be NULL (zero). Deterministic (fixed
+ * summation order). Returns frames per signal, 0 on error. */
+size_t fft_stft_adjoint_saved_device(void* handle, const float* d_plane_real, const float* d_plane_imag,
+                                     size_t signals, size_t pcm_len, const float* d_grad_real,
+                                     const float* d_grad_imag, float* d_grad_pcm, int pad_mode, void* stream);
+
 /* No counterpart in the reference: an allocation-free variant of fft_stft_adjoint_device,
  * for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc). Same arguments and results; it calls
