@@ -85,6 +85,8 @@ SIGNATURES = {
                                                  c_void_p, c_void_p, c_int, c_void_p]),
     "fft_ragged_create": (c_void_p, [ctypes.POINTER(RaggedRequest), c_size_t, c_int, c_int, c_int, c_int, c_int]),
     "fft_ragged_autotune": (c_int, [c_void_p, c_int]),
+    "fft_ragged_set_engine": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "fft_ragged_engine_count": (c_int, []),
     "fft_ragged_execute": (c_int, [c_void_p, c_void_p]),
     "fft_ragged_execute_plain": (c_int, [c_void_p, c_void_p]),
     "fft_ragged_describe": (c_int, [c_void_p, c_char_p, c_size_t]),

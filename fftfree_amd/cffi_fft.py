@@ -348,6 +348,10 @@ class RaggedBatch:
         if not self.ff.lib.fft_ragged_autotune(self.handle, repeats):
             raise RuntimeError("fft_ragged_autotune failed")
 
+    def set_engine(self, n: int, engine: int, blocks_per_item: int = 2) -> bool:
+        """fft_ragged_set_engine: False when the engine does not run window size n."""
+        return bool(self.ff.lib.fft_ragged_set_engine(self.handle, n, engine, blocks_per_item))
+
     def execute(self) -> None:
         if not self.ff.lib.fft_ragged_execute(self.handle, CffiFft._stream()):
             raise RuntimeError("fft_ragged_execute failed")

@@ -546,20 +546,27 @@ namespace fftfree {
 namespace {
 
 // A ragged batch: one context per distinct (window, hop) supplies windows, twiddles and
-// the plane sink; every request becomes a StftArgs block in device memory; the work list
-// (items of tiles, sorted by cost, split into 8 cost-balanced XCD chunks) and the graph
-// that holds the one persistent launch are rebuilt when the autotuner changes the
-// per-N tile shape or tiles per item.
+// the plane sink; every request becomes a StftArgs block in device memory. Per window size
+// N the plan holds an engine (a tile shape, or the line-aligned engine) and the blocks per
+// work item; the work items of each N are sorted by cost and split into 8 cost-balanced
+// XCD chunks, and the graph holds one persistent launch per N. The autotuner re-times the
+// choices on the batch itself.
 struct RaggedPlan {
   std::vector<fft_ragged_request> reqs;
   std::vector<void*> ctxs;           // FftContext handles
   std::vector<int> req_ctx;
-  std::vector<StftArgs> args;
+  std::vector<StftArgs> args;        // as the per-request launches take them
   int mode = 0;
-  int family = 0;                    // ragged kernel: 0 = E=8 shapes, 1 = E=16 shapes
-  int grids[2] = {0, 0};             // resident workgroups of each kernel
-  std::map<int, int> cfg_of_n;       // N -> ragged tile shape
-  std::map<int, int> tiles_of_n;     // N -> tiles per work item
+  std::map<int, int> cfg_of_n;       // N -> engine (ragged_cfg id)
+  std::map<int, int> tiles_of_n;     // N -> blocks per work item
+  std::map<int, std::vector<size_t>> reqs_of_n;
+  struct Launch {
+    int cfg;
+    size_t chunk_off;  // its 9 chunk bounds (absolute item indices)
+    int grid;
+  };
+  std::vector<Launch> launches;      // of the staged work lists
+  std::map<int, int> grid_of_cfg;
   DeviceBuffer d_args, d_items, d_chunks;
   size_t n_items = 0;
   int device = 0;
@@ -578,29 +585,56 @@ struct RaggedPlan {
     for (void* c : ctxs) fft_free(c);
   }
 
-  // Items for the requests in `only` (all when empty) with the current choices; returns
-  // them with the chunk boundaries.
-  void build(const std::vector<size_t>& only, std::vector<RaggedItem>& items, std::vector<unsigned>& chunks) const {
+  int grid(int cfg) {
+    auto it = grid_of_cfg.find(cfg);
+    if (it != grid_of_cfg.end()) return it->second;
+    int g = 0;
+    hip_check(ragged_grid(mode, cfg, &g), "ragged_grid");
+    g = std::max(g, 8);
+    grid_of_cfg[cfg] = g;
+    return g;
+  }
+
+  // The line-aligned engine `cfg` takes every request of N whole (each qualifies and its
+  // frames after the last tile fit the rest block).
+  bool lines_ok(int n, int cfg) const {
+    for (size_t r : reqs_of_n.at(n)) {
+      size_t lead = 0, blocks = 0, take = 0;
+      if (!ragged_lines_layout(cfg, args[r], &lead, &blocks, &take) || take != args[r].frames) return false;
+    }
+    return true;
+  }
+
+  // Blocks of request r under engine cfg, and the frames (cost) of one block.
+  size_t blocks_of(size_t r, int cfg, size_t* lead) const {
+    *lead = 0;
+    if (ragged_cfg_lines(cfg)) {
+      size_t blocks = 0, take = 0;
+      ragged_lines_layout(cfg, args[r], lead, &blocks, &take);
+      return blocks;
+    }
+    const int fpw = ragged_cfg(cfg)->fpw;
+    return (args[r].frames + fpw - 1) / fpw;
+  }
+
+  // Work items of the requests `rs` (all of one N) under that N's current choices.
+  void build_n(int n, const std::vector<size_t>& rs, std::vector<RaggedItem>& items,
+               std::vector<unsigned>& chunks) const {
     items.clear();
     std::vector<double> cost;
-    auto add = [&](size_t r) {
-      const int n = reqs[r].window;
-      const int cfg = cfg_of_n.at(n);
-      const int fpw = ragged_cfg(cfg)->fpw;
-      const unsigned per = (unsigned)tiles_of_n.at(n);
-      const size_t tiles = (reqs[r].frames + fpw - 1) / fpw;
-      const double tile_cost = (double)fpw * (4.0 * reqs[r].hop + 12.0 * (n / 2 + 1));
-      for (size_t t0 = 0; t0 < tiles; t0 += per) {
-        const unsigned cnt = (unsigned)std::min<size_t>(per, tiles - t0);
+    const int cfg = cfg_of_n.at(n);
+    const unsigned per = (unsigned)tiles_of_n.at(n);
+    for (size_t r : rs) {
+      size_t lead = 0;
+      const size_t blocks = blocks_of(r, cfg, &lead);
+      const double block_cost = (double)ragged_cfg(cfg)->fpw * (4.0 * reqs[r].hop + 12.0 * (n / 2 + 1));
+      for (size_t t0 = 0; t0 < blocks; t0 += per) {
+        const unsigned cnt = (unsigned)std::min<size_t>(per, blocks - t0);
         items.push_back(RaggedItem{(unsigned)r, (unsigned)cfg, (unsigned)t0, cnt});
-        cost.push_back(tile_cost * cnt);
+        cost.push_back(block_cost * cnt);
       }
-    };
-    if (only.empty())
-      for (size_t r = 0; r < reqs.size(); ++r) add(r);
-    else
-      for (size_t r : only) add(r);
-    // costliest first (stable, so the tiles of a request stay in order)
+    }
+    // costliest first (stable, so the blocks of a request stay in order)
     std::vector<size_t> order(items.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
@@ -623,36 +657,70 @@ struct RaggedPlan {
     chunks[8] = (unsigned)items.size();
   }
 
-  void upload(const std::vector<RaggedItem>& items, const std::vector<unsigned>& chunks) {
-    d_items.reserve(std::max<size_t>(1, items.size()) * sizeof(RaggedItem));
-    if (!items.empty())
-      hip_check(hipMemcpy(d_items.ptr, items.data(), items.size() * sizeof(RaggedItem), hipMemcpyHostToDevice),
+  // Upload the requests (with each one's `lead` for its N's engine) and the work lists of
+  // the sizes `ns` (all of their requests); the launches of run() follow.
+  void stage(const std::vector<int>& ns) {
+    std::vector<StftArgs> dev_args = args;
+    for (auto& kv : reqs_of_n) {
+      const int cfg = cfg_of_n.at(kv.first);
+      if (!ragged_cfg_lines(cfg)) continue;
+      for (size_t r : kv.second) blocks_of(r, cfg, &dev_args[r].lead);
+    }
+    d_args.reserve(dev_args.size() * sizeof(StftArgs));
+    hip_check(hipMemcpy(d_args.ptr, dev_args.data(), dev_args.size() * sizeof(StftArgs), hipMemcpyHostToDevice),
+              "upload requests");
+    std::vector<RaggedItem> all;
+    std::vector<unsigned> all_chunks;
+    launches.clear();
+    for (int n : ns) {
+      std::vector<RaggedItem> items;
+      std::vector<unsigned> chunks;
+      build_n(n, reqs_of_n.at(n), items, chunks);
+      if (items.empty()) continue;
+      const int cfg = cfg_of_n.at(n);
+      launches.push_back(Launch{cfg, all_chunks.size(), grid(cfg)});
+      for (unsigned& c : chunks) c += (unsigned)all.size();
+      all.insert(all.end(), items.begin(), items.end());
+      all_chunks.insert(all_chunks.end(), chunks.begin(), chunks.end());
+    }
+    d_items.reserve(std::max<size_t>(1, all.size()) * sizeof(RaggedItem));
+    if (!all.empty())
+      hip_check(hipMemcpy(d_items.ptr, all.data(), all.size() * sizeof(RaggedItem), hipMemcpyHostToDevice),
                 "upload items");
-    d_chunks.reserve(chunks.size() * sizeof(unsigned));
-    hip_check(hipMemcpy(d_chunks.ptr, chunks.data(), chunks.size() * sizeof(unsigned), hipMemcpyHostToDevice),
-              "upload chunks");
-    n_items = items.size();
+    d_chunks.reserve(std::max<size_t>(1, all_chunks.size()) * sizeof(unsigned));
+    if (!all_chunks.empty())
+      hip_check(hipMemcpy(d_chunks.ptr, all_chunks.data(), all_chunks.size() * sizeof(unsigned),
+                          hipMemcpyHostToDevice),
+                "upload chunks");
+    n_items = all.size();
   }
 
-  void launch(hipStream_t s) const {
-    hip_check(launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(), mode,
-                                 family, grids[family], s),
-              "launch_stft_ragged");
+  hipError_t run(hipStream_t s) const {
+    for (const Launch& l : launches) {
+      // chunk bounds are absolute item indices, so every launch takes the whole item list
+      const hipError_t e = launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(),
+                                              d_chunks.as<unsigned>() + l.chunk_off, mode, l.cfg, l.grid, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
   }
 
-  // Rebuild the full work list and capture its launch in a graph.
+  std::vector<int> sizes() const {
+    std::vector<int> ns;
+    for (auto& kv : reqs_of_n) ns.push_back(kv.first);
+    std::sort(ns.rbegin(), ns.rend());  // the costliest transforms first
+    return ns;
+  }
+
+  // Stage every size and capture the launches in a graph.
   void commit() {
-    std::vector<RaggedItem> items;
-    std::vector<unsigned> chunks;
-    build({}, items, chunks);
-    upload(items, chunks);
+    stage(sizes());
     if (exec) hip_check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
     if (graph) hip_check(hipGraphDestroy(graph), "hipGraphDestroy");
     exec = nullptr;
     graph = nullptr;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipError_t le = launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(), d_chunks.as<unsigned>(),
-                                       mode, family, grids[family], stream);
+    const hipError_t le = run(stream);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(stream, &g);
     hip_check(le, "launch_stft_ragged (capture)");
@@ -661,58 +729,54 @@ struct RaggedPlan {
     hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   }
 
-  // Per kernel family: for every N, time the persistent launch over that N's requests
-  // with the family's tile shape and each tiles-per-item choice, keep the fastest; then
-  // time the whole batch per family and keep the faster family.
-  float time_launches(const std::vector<size_t>& only, int repeats, hipEvent_t e0, hipEvent_t e1) {
-    std::vector<RaggedItem> items;
-    std::vector<unsigned> chunks;
-    build(only, items, chunks);
-    upload(items, chunks);
-    launch(stream);  // warm
-    hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-    for (int i = 0; i < repeats; ++i) launch(stream);
-    hip_check(hipEventRecord(e1, stream), "hipEventRecord");
-    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-    return ms;
+  // Median time (ms) of `repeats` timed runs of the staged launches, after one warm-up.
+  float median_ms(int repeats, hipEvent_t e0, hipEvent_t e1) {
+    hip_check(run(stream), "launch_stft_ragged");
+    std::vector<float> t;
+    for (int i = 0; i < repeats; ++i) {
+      hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+      hip_check(run(stream), "launch_stft_ragged");
+      hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
   }
 
+  // Per N: time every engine of N (its tile shapes; the line-aligned engine when the
+  // requests qualify) at 1, 2, 4 and 8 blocks per item on that N's requests, by the median
+  // of `repeats` runs; a candidate replaces the current choice only when it is more than
+  // 3% faster (timings of near-equal choices otherwise flip with noise).
   void tune(int repeats) {
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
-    std::map<int, int> best_cfg[2], best_tiles[2];
-    float family_ms[2] = {0, 0};
-    for (int fam = 0; fam < 2; ++fam) {
-      family = fam;
-      for (auto& kv : cfg_of_n) {
-        const int n = kv.first;
-        int shape = -1;
-        for (int cfg = 0; cfg < ragged_cfg_count(); ++cfg)
-          if (2 * ragged_cfg(cfg)->l == n && ragged_cfg_family(cfg) == fam) shape = cfg;
-        cfg_of_n[n] = shape;
-        std::vector<size_t> only;
-        for (size_t r = 0; r < reqs.size(); ++r)
-          if (reqs[r].window == n) only.push_back(r);
-        float best = 1e30f;
+    for (int n : sizes()) {
+      std::vector<int> engines;
+      for (int cfg = 0; cfg < ragged_cfg_count(); ++cfg)
+        if (2 * ragged_cfg(cfg)->l == n && (!ragged_cfg_lines(cfg) || lines_ok(n, cfg))) engines.push_back(cfg);
+      stage({n});
+      float best = median_ms(repeats, e0, e1);
+      int best_cfg = cfg_of_n[n], best_per = tiles_of_n[n];
+      for (int cfg : engines)
         for (int per : {1, 2, 4, 8}) {
+          if (cfg == best_cfg && per == best_per) continue;
+          cfg_of_n[n] = cfg;
           tiles_of_n[n] = per;
-          const float ms = time_launches(only, repeats, e0, e1);
-          if (ms < best) {
+          stage({n});
+          const float ms = median_ms(repeats, e0, e1);
+          if (ms < best * 0.97f) {
             best = ms;
-            best_tiles[fam][n] = per;
+            best_cfg = cfg;
+            best_per = per;
           }
         }
-        tiles_of_n[n] = best_tiles[fam][n];
-        best_cfg[fam][n] = shape;
-      }
-      family_ms[fam] = time_launches({}, repeats, e0, e1);
+      cfg_of_n[n] = best_cfg;
+      tiles_of_n[n] = best_per;
     }
-    family = family_ms[1] < family_ms[0] ? 1 : 0;
-    cfg_of_n = best_cfg[family];
-    tiles_of_n = best_tiles[family];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
@@ -1921,25 +1985,24 @@ void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode
       a.o_im = q.out_imag;
       a.o_mag = q.out_mag;
       p->args.push_back(a);
-      p->cfg_of_n[q.window] = -1;
-    }
-    // defaults before tuning: the E=8 family and 2 tiles per item
-    for (auto& kv : p->cfg_of_n) {
-      for (int cfg = 0; cfg < fftfree::ragged_cfg_count(); ++cfg)
-        if (2 * fftfree::ragged_cfg(cfg)->l == kv.first && fftfree::ragged_cfg_family(cfg) == 0) kv.second = cfg;
-      p->tiles_of_n[kv.first] = 2;
+      p->reqs_of_n[q.window].push_back(i);
     }
     p->mode = store_polar ? 3 : reduce_magnitude ? 2 : 0;
     fftfree::hip_check(hipGetDevice(&p->device), "hipGetDevice");
     fftfree::hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
-    for (int fam = 0; fam < 2; ++fam) {
-      fftfree::hip_check(fftfree::ragged_grid(p->mode, fam, &p->grids[fam]), "ragged_grid");
-      if (p->grids[fam] < 8) p->grids[fam] = 8;
+    // defaults before tuning: the line-aligned engine where it takes every request of N,
+    // else the E=8 tile shape; 2 blocks per item
+    for (auto& kv : p->reqs_of_n) {
+      const int n = kv.first;
+      int pick = -1;
+      for (int cfg = 0; cfg < fftfree::ragged_cfg_count(); ++cfg) {
+        if (2 * fftfree::ragged_cfg(cfg)->l != n) continue;
+        if (fftfree::ragged_cfg_lines(cfg) ? p->lines_ok(n, cfg) : fftfree::ragged_cfg(cfg)->e == 8) pick = cfg;
+        if (pick >= 0 && fftfree::ragged_cfg_lines(pick)) break;
+      }
+      p->cfg_of_n[n] = pick;
+      p->tiles_of_n[n] = 2;
     }
-    p->d_args.reserve(p->args.size() * sizeof(fftfree::StftArgs));
-    fftfree::hip_check(hipMemcpy(p->d_args.ptr, p->args.data(), p->args.size() * sizeof(fftfree::StftArgs),
-                                 hipMemcpyHostToDevice),
-                       "upload requests");
     p->commit();
     return p.release();
   } catch (const std::exception& e) {
@@ -1963,6 +2026,26 @@ int fft_ragged_autotune(void* ragged, int repeats) {
     return 0;
   }
 }
+
+int fft_ragged_set_engine(void* ragged, int n, int engine, int blocks_per_item) {
+  if (!ragged || blocks_per_item < 1) return 0;
+  auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
+  const fftfree::VariantInfo* v = fftfree::ragged_cfg(engine);
+  if (!v || 2 * v->l != n || !p->reqs_of_n.count(n)) return 0;
+  try {
+    fftfree::hip_check(hipSetDevice(p->device), "hipSetDevice");
+    if (fftfree::ragged_cfg_lines(engine) && !p->lines_ok(n, engine)) return 0;
+    p->cfg_of_n[n] = engine;
+    p->tiles_of_n[n] = blocks_per_item;
+    p->commit();
+    return 1;
+  } catch (const std::exception& e) {
+    fftfree::report("fft_ragged_set_engine", e.what());
+    return 0;
+  }
+}
+
+int fft_ragged_engine_count(void) { return fftfree::ragged_cfg_count(); }
 
 int fft_ragged_execute(void* ragged, void* stream) {
   if (!ragged) return 0;
@@ -1989,12 +2072,11 @@ int fft_ragged_execute_plain(void* ragged, void* stream) {
 int fft_ragged_describe(void* ragged, char* buf, size_t buflen) {
   if (!ragged || !buf || buflen == 0) return 0;
   auto* p = static_cast<fftfree::RaggedPlan*>(ragged);
-  std::string out = "kernel=" + std::string(p->family ? "E16" : "E8") + " grid=" + std::to_string(p->grids[p->family]) +
-                    " items=" + std::to_string(p->n_items);
+  std::string out = "launches=" + std::to_string(p->launches.size()) + " items=" + std::to_string(p->n_items);
   for (auto& kv : p->cfg_of_n) {
     const fftfree::VariantInfo* v = fftfree::ragged_cfg(kv.second);
-    out += " N" + std::to_string(kv.first) + ":" + v->radices + "/fpw" + std::to_string(v->fpw) + "/x" +
-           std::to_string(p->tiles_of_n[kv.first]);
+    out += " N" + std::to_string(kv.first) + ":" + v->radices + "/fpb" + std::to_string(v->fpw) + "/x" +
+           std::to_string(p->tiles_of_n[kv.first]) + "/grid" + std::to_string(p->grid(kv.second));
   }
   std::snprintf(buf, buflen, "%s", out.c_str());
   return 1;

@@ -201,8 +201,9 @@ struct VariantInfo {
   const char* radices;
 };
 
-// Ragged batches: one work item = `ntiles` consecutive tiles (FPW frames each) of
-// request `req`, run with tile shape `cfg` (ragged_cfg).
+// Ragged batches: one work item = `ntiles` consecutive blocks of request `req`, run by
+// engine `cfg` (ragged_cfg): a tile engine (blocks = tiles of FPW frames) or a
+// line-aligned engine (blocks as in ragged_lines_layout).
 struct RaggedItem {
   unsigned req;
   unsigned cfg;
@@ -210,15 +211,19 @@ struct RaggedItem {
   unsigned ntiles;
 };
 int ragged_cfg_count();
-const VariantInfo* ragged_cfg(int id);  // l = N/2; 256 threads = fpw * l / e
-int ragged_cfg_family(int id);          // the ragged kernel (0: E=8 shapes, 1: E=16) that runs it
-// Resident workgroups of a ragged kernel on the current device (CUs x occupancy).
-hipError_t ragged_grid(int mode, int family, int* grid);
-// One persistent launch over every item: workgroups of XCD x (blockIdx % 8 == x) stride
-// through items[chunks[x] .. chunks[x+1]). Forward R2C, half spectrum, output mode
-// kOutComplex / kOutMagnitude / kOutPolar.
+const VariantInfo* ragged_cfg(int id);  // l = N/2; fpw = frames per block
+bool ragged_cfg_lines(int id);          // a line-aligned engine
+// Blocks of request `a` under line-aligned engine `cfg`: frames in whole tiles (`lead`,
+// to be set in the request's StftArgs), blocks, frames covered (`take`); false when the
+// request does not qualify (window != N, odd hop, misaligned buffers).
+bool ragged_lines_layout(int cfg, const StftArgs& a, size_t* lead, size_t* blocks, size_t* take);
+// Resident workgroups of engine `cfg`'s kernel on the current device (CUs x occupancy).
+hipError_t ragged_grid(int mode, int cfg, int* grid);
+// One persistent launch of engine `cfg` over its items: workgroups of XCD x
+// (blockIdx % 8 == x) stride through items[chunks[x] .. chunks[x+1]). Forward R2C, half
+// spectrum, output mode kOutComplex / kOutMagnitude / kOutPolar.
 hipError_t launch_stft_ragged(const StftArgs* reqs, const RaggedItem* items, const unsigned* chunks, int mode,
-                              int family, int grid, hipStream_t s);
+                              int cfg, int grid, hipStream_t s);
 int stft_variant_count(int n);
 const VariantInfo* stft_variant(int n, int idx);
 int c2c_variant_count(int n);

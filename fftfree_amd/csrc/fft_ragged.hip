@@ -1,28 +1,31 @@
 // fft_ragged.hip -- ragged batched STFT: many requests with different lengths, window
-// sizes and hops in one persistent launch.
+// sizes and hops in one graph.
 //
 // Each request is an fft_execute_batched call of its own (frame rule fft_cffi.cpp:715-750,
 // framing 766-791, post hook eigen_fft.hpp:2535-2596, output copy fft_cffi.cpp:811-825),
-// described by a StftArgs block in device memory. The host cuts every request into work
-// items of `ntiles` tiles (a tile = the FPW frames one 256-thread workgroup transforms
-// at once, with the request's N, radix pattern and FPW from a configuration table),
-// sorts the items by cost and splits them into 8 contiguous, cost-balanced chunks, one
-// per XCD: the workgroups of XCD x (blockIdx % 8 == x, the dispatcher's round-robin
-// order) stride through chunk x, so the neighbouring tiles of one request, which share
-// samples, are read through the same L2. One launch, sized to the resident capacity of
-// the chip, covers every request; the host captures it in a graph.
-#include "fft_device.hpp"
-#include "fft_kernels.hpp"
-#include "fft_stft_tile.hpp"
+// described by a StftArgs block in device memory. The graph holds one persistent launch
+// per window size N in the batch, each compiled for that N alone (its own register
+// allocation, no shape dispatch). A launch runs one engine over the work items of its N:
+//  * the line-aligned engine (N = 256, 512, 1024): the body of stft_r2c_lines_kernel,
+//    whose blocks are tiles of 32 or 64 frames whose plane regions are whole 128-byte
+//    lines, plus a block for the frames after the last tile;
+//  * the tile engine (every N): StftTile, FPW frames per tile with a radix pattern.
+// The host cuts every request into work items of `ntiles` blocks, sorts the items of each
+// N by cost and splits them into 8 contiguous, cost-balanced chunks, one per XCD: the
+// workgroups of XCD x (blockIdx % 8 == x, the dispatcher's round-robin order) stride
+// through chunk x, so the neighbouring tiles of one request, which share samples, are read
+// through the same L2.
+#include "fft_stft_impl.hpp"
 
 namespace fftfree {
 using namespace dev;
 
 namespace {
 
-// Tile shapes the ragged kernel can run: 256 threads = FPW transforms of L/E threads.
-// Per N, the autotuner picks one of these and the tiles per work item.
-#define FFTFREE_RAGGED_CFGS(X)                \
+// Engines: ID, L (= N/2), E, frames per block, name, radices. IDs below kLinesFirst are
+// tile-engine shapes (FPW frames per tile); the others are line-aligned engines (K = 8
+// iterations of P = 256/(L/E) frames per tile).
+#define FFTFREE_RAGGED_TILES(X)               \
   X(0, 128, 8, 16, "8x8x2", 8, 8, 2)         \
   X(1, 128, 16, 32, "16x8", 16, 8)           \
   X(2, 256, 8, 8, "8x8x4", 8, 8, 4)          \
@@ -33,73 +36,85 @@ namespace {
   X(7, 1024, 16, 4, "16x16x4", 16, 16, 4)    \
   X(8, 2048, 8, 1, "8x8x8x4", 8, 8, 8, 4)    \
   X(9, 2048, 16, 2, "16x16x8", 16, 16, 8)
+#define FFTFREE_RAGGED_LINES(X)                          \
+  X(10, 128, 4, 64, "lines 4x4x4x2", 4, 4, 4, 2)        \
+  X(11, 256, 4, 32, "lines 4x4x4x4", 4, 4, 4, 4)        \
+  X(12, 512, 8, 32, "lines 8x8x8", 8, 8, 8)
+constexpr int kLinesFirst = 10;
+constexpr int kLinesK = 8;    // iterations per line-aligned tile
+constexpr int kLinesOpt = 3;  // tiles in launch order, one plane per pass
 
 #define FFTFREE_RAGGED_INFO(ID, L, E, FPW, NAME, ...) {L, E, FPW, NAME},
-const VariantInfo kRaggedCfgs[] = {FFTFREE_RAGGED_CFGS(FFTFREE_RAGGED_INFO)};
+const VariantInfo kRaggedCfgs[] = {FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_INFO) FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_INFO)};
 #undef FFTFREE_RAGGED_INFO
 constexpr int kRaggedCfgCount = sizeof(kRaggedCfgs) / sizeof(kRaggedCfgs[0]);
 
-// LDS of the largest tile shape (float2); every shape stages its bins inside it.
-#define FFTFREE_RAGGED_LDS(ID, L, E, FPW, NAME, ...) \
-  StftTile<L, E, FPW, 1, kOutPolar, 1, -1, __VA_ARGS__>::kLds,
-constexpr int kRaggedLdsList[] = {FFTFREE_RAGGED_CFGS(FFTFREE_RAGGED_LDS)};
-#undef FFTFREE_RAGGED_LDS
-constexpr int max_lds() {
-  int m = 0;
-  for (int v : kRaggedLdsList) m = v > m ? v : m;
-  return m;
-}
-constexpr int kRaggedLds = max_lds();
+// Items of XCD x: workgroups with blockIdx % 8 == x stride through [chunks[x], chunks[x+1]).
+#define FFTFREE_RAGGED_ITEMS(BODY)                                                        \
+  const int x = blockIdx.x & 7;                                                           \
+  const unsigned per_xcd = (gridDim.x - x + 7) >> 3; /* workgroups dealt to this XCD */   \
+  for (unsigned i = chunks[x] + (blockIdx.x >> 3); i < chunks[x + 1]; i += per_xcd) {     \
+    const RaggedItem w = items[i];                                                        \
+    for (unsigned k = 0; k < w.ntiles; ++k) {                                             \
+      BODY;                                                                               \
+      __syncthreads(); /* the next block reuses the LDS */                                \
+    }                                                                                     \
+  }
 
-// Each tile shape is its own (not inlined) routine, so that register allocation sees
-// one shape at a time.
-template <int MODE, int ID, int L, int E, int FPW, int... Rs>
-__device__ __noinline__ void ragged_tile(const StftArgs& a, size_t tile, float2* lds) {
+// One block is a call that is not inlined: register allocation then sees the block's
+// routine alone, not the routine interleaved with the item loop (inlined, the compiler
+// keeps values live across blocks and the kernels lose most of their occupancy). The
+// request's arguments are passed as their global address (a struct in registers would be
+// spilled to scratch for every call) and read with uniform loads.
+template <int MODE, int L, int E, int FPW, int... Rs>
+__device__ __noinline__ void ragged_tile_block(const StftArgs* __restrict__ ap, size_t tile, float2* lds) {
+  const StftArgs a = *ap;
   StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::run(a, tile, lds, nullptr);
 }
+template <int MODE, int L, int E, int... Rs>
+__device__ __noinline__ void ragged_lines_block(const StftArgs* __restrict__ ap, int block) {
+  const StftArgs a = *ap;
+  stft_lines_body<L, E, kLinesK, kLinesOpt, MODE, -1, Rs...>(a, block, 0);
+}
 
-// Two kernels: family 0 holds the E=8 shapes, family 1 the E=16 ones. A kernel's register
-// allocation is that of its largest shape, so keeping the lean shapes apart lets them
-// run at a higher occupancy; the autotuner picks the family for the batch.
-template <int MODE, int FAM>
-__global__ void __launch_bounds__(256) stft_ragged_kernel(const StftArgs* __restrict__ reqs,
-                                                          const RaggedItem* __restrict__ items,
-                                                          const unsigned* __restrict__ chunks) {
-  __shared__ float2 lds[kRaggedLds];
-  const int x = blockIdx.x & 7;
-  const unsigned per_xcd = (gridDim.x - x + 7) >> 3;  // workgroups dealt to this XCD
-  for (unsigned i = chunks[x] + (blockIdx.x >> 3); i < chunks[x + 1]; i += per_xcd) {
-    const RaggedItem w = items[i];
-    const StftArgs a = reqs[w.req];
-    for (unsigned k = 0; k < w.ntiles; ++k) {
-      const size_t tile = (size_t)w.tile + k;
-      switch (w.cfg) {
-#define FFTFREE_RAGGED_CASE(ID, L, E, FPW, NAME, ...)                        \
-  case ID:                                                                   \
-    if constexpr ((E == 16) == (FAM == 1))                                    \
-      ragged_tile<MODE, ID, L, E, FPW, __VA_ARGS__>(a, tile, lds);            \
-    break;
-        FFTFREE_RAGGED_CFGS(FFTFREE_RAGGED_CASE)
-#undef FFTFREE_RAGGED_CASE
-        default:
-          break;
-      }
-      __syncthreads();  // the next tile reuses the LDS
-    }
-  }
+// Tile engine, one shape per kernel.
+template <int MODE, int L, int E, int FPW, int... Rs>
+__global__ void __launch_bounds__(256) stft_ragged_tile_kernel(const StftArgs* __restrict__ reqs,
+                                                               const RaggedItem* __restrict__ items,
+                                                               const unsigned* __restrict__ chunks) {
+  __shared__ float2 lds[StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::kLds];
+  FFTFREE_RAGGED_ITEMS((ragged_tile_block<MODE, L, E, FPW, Rs...>(reqs + w.req, (size_t)w.tile + k, lds)))
+}
+
+// Line-aligned engine: block w.tile + k of request w.req (block 0 = the frames after the
+// last tile when the request has them; the host set each request's `lead`).
+template <int MODE, int L, int E, int... Rs>
+__global__ void __launch_bounds__(256) stft_ragged_lines_kernel(const StftArgs* __restrict__ reqs,
+                                                                const RaggedItem* __restrict__ items,
+                                                                const unsigned* __restrict__ chunks) {
+  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, Rs...>(reqs + w.req, (int)(w.tile + k))))
 }
 
 template <int MODE>
-const void* ragged_kernel_ptr(int family) {
-  return family ? reinterpret_cast<const void*>(&stft_ragged_kernel<MODE, 1>)
-                : reinterpret_cast<const void*>(&stft_ragged_kernel<MODE, 0>);
+const void* ragged_kernel_mode(int cfg) {
+  switch (cfg) {
+#define FFTFREE_RAGGED_TILE_CASE(ID, L, E, FPW, NAME, ...) \
+  case ID: return reinterpret_cast<const void*>(&stft_ragged_tile_kernel<MODE, L, E, FPW, __VA_ARGS__>);
+    FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_TILE_CASE)
+#undef FFTFREE_RAGGED_TILE_CASE
+#define FFTFREE_RAGGED_LINES_CASE(ID, L, E, FPW, NAME, ...) \
+  case ID: return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, __VA_ARGS__>);
+    FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LINES_CASE)
+#undef FFTFREE_RAGGED_LINES_CASE
+    default: return nullptr;
+  }
 }
 
-const void* ragged_kernel(int mode, int family) {
+const void* ragged_kernel(int mode, int cfg) {
   switch (mode) {
-    case kOutComplex: return ragged_kernel_ptr<kOutComplex>(family);
-    case kOutMagnitude: return ragged_kernel_ptr<kOutMagnitude>(family);
-    case kOutPolar: return ragged_kernel_ptr<kOutPolar>(family);
+    case kOutComplex: return ragged_kernel_mode<kOutComplex>(cfg);
+    case kOutMagnitude: return ragged_kernel_mode<kOutMagnitude>(cfg);
+    case kOutPolar: return ragged_kernel_mode<kOutPolar>(cfg);
     default: return nullptr;
   }
 }
@@ -108,23 +123,31 @@ const void* ragged_kernel(int mode, int family) {
 
 int ragged_cfg_count() { return kRaggedCfgCount; }
 const VariantInfo* ragged_cfg(int id) { return id >= 0 && id < kRaggedCfgCount ? &kRaggedCfgs[id] : nullptr; }
-int ragged_cfg_family(int id) { return id >= 0 && id < kRaggedCfgCount && kRaggedCfgs[id].e == 16 ? 1 : 0; }
+bool ragged_cfg_lines(int id) { return id >= kLinesFirst && id < kRaggedCfgCount; }
 
-hipError_t ragged_grid(int mode, int family, int* grid) {
-  const void* k = ragged_kernel(mode, family);
+bool ragged_lines_layout(int cfg, const StftArgs& a, size_t* lead, size_t* blocks, size_t* take) {
+  switch (cfg) {
+#define FFTFREE_RAGGED_LAYOUT_CASE(ID, L, E, FPW, NAME, ...) \
+  case ID: return lines_layout<L, E, kLinesK>(a, lead, blocks, take);
+    FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LAYOUT_CASE)
+#undef FFTFREE_RAGGED_LAYOUT_CASE
+    default: return false;
+  }
+}
+
+hipError_t ragged_grid(int mode, int cfg, int* grid) {
+  const void* k = ragged_kernel(mode, cfg);
   if (!k) return hipErrorInvalidValue;
-  int dev = 0, cus = 0, occ = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 256, 0);
+  int occ = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 256, 0);
   if (e != hipSuccess) return e;
-  *grid = (cus > 0 ? cus : 1) * (occ > 0 ? occ : 1);
+  *grid = device_cus() * (occ > 0 ? occ : 1);
   return hipSuccess;
 }
 
 hipError_t launch_stft_ragged(const StftArgs* reqs, const RaggedItem* items, const unsigned* chunks, int mode,
-                              int family, int grid, hipStream_t s) {
-  const void* k = ragged_kernel(mode, family);
+                              int cfg, int grid, hipStream_t s) {
+  const void* k = ragged_kernel(mode, cfg);
   if (!k || grid <= 0) return hipErrorInvalidValue;
   void* args[] = {&reqs, &items, &chunks};
   return hipLaunchKernel(k, dim3((unsigned)grid), dim3(256), args, 0, s);

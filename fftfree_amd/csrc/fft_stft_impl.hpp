@@ -503,8 +503,11 @@ __global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
 // planes. The frames after the last whole tile (a few dozen) go to workgroup 0,
 // dispatched first, which walks them P at a time with per-frame bounds, zero fill and
 // direct stores.
+// The kernel body is a device routine over a virtual block index in [0, vgrid) (tiles in
+// order, block 0 the rest when there is one), so that persistent kernels (ragged batches)
+// can run it for work items of many requests.
 template <int L, int E, int K, int OPT, int MODE, int DIR, int... Rs>
-__global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(StftArgs a) {
+__device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, int vgrid) {
   constexpr int T = L / E;
   constexpr int N = 2 * L;
   constexpr int NT = 256;
@@ -544,7 +547,7 @@ __global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(
   // workgroup 0 takes the rest (when there is one); the others walk the tiles.
   // OPT bit 0: tiles in launch order (consecutive tiles on consecutive workgroups, so on
   // different XCDs) instead of contiguous runs per XCD
-  const long tile = (long)((OPT & 1) ? (int)blockIdx.x : xcd_tile((int)blockIdx.x, (int)gridDim.x)) - has_rest;
+  const long tile = (long)((OPT & 1) ? vblock : xcd_tile(vblock, vgrid)) - has_rest;
   const int hop = a.hop;
   const float* __restrict__ win = a.win;
   constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
@@ -577,12 +580,11 @@ __global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(
       const int m = tid / (R1 - 1), r = tid % (R1 - 1) + 1;
       tw1[m][r - 1] = load_tw<DIR>(a.tw, m * (L / R1 / NS1) * 2 * r);
     }
-    // window taps times 1/2 into LDS (1/2 when windows are off), one 16-byte load per
-    // thread; held there rather than in registers so that four workgroups fit per CU
-    static_assert(L % (2 * NT) == 0, "whole 16-byte words per thread");
+    // window taps times 1/2 into LDS (1/2 when windows are off), 16-byte loads; held
+    // there rather than in registers so that four workgroups fit per CU
+    static_assert(L % 2 == 0, "whole 16-byte words of taps");
 #pragma unroll
-    for (int c = 0; c < L / (2 * NT); ++c) {
-      const int w = tid + c * NT;  // 16-byte word: taps 4w .. 4w+3
+    for (int w = tid; w < L / 2; w += NT) {  // 16-byte word: taps 4w .. 4w+3
       // unconditional load (the twiddle table stands in for a missing window), then a
       // select: a conditional load would become a branch and a wait
       const f32x4 x = reinterpret_cast<const f32x4*>(win ? win : reinterpret_cast<const float*>(a.tw))[w];
@@ -806,6 +808,36 @@ __global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(
   }
 }
 
+template <int L, int E, int K, int OPT, int MODE, int DIR, int... Rs>
+__global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(StftArgs a) {
+  stft_lines_body<L, E, K, OPT, MODE, DIR, Rs...>(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// The blocks of a single-request launch of the line-aligned kernel: whole tiles of
+// regular frames (frames fully inside the PCM) plus, when at most 2*P*K frames remain, the
+// rest in block 0. Returns whether the request qualifies (window = N, even hop, aligned
+// samples, window and planes); `lead` = frames in whole tiles, `blocks` = blocks of the
+// launch, `take` = frames the launch covers (the rest needs another path).
+template <int L, int E, int K>
+bool lines_layout(const StftArgs& a, size_t* lead, size_t* blocks, size_t* take) {
+  constexpr int N = 2 * L;
+  constexpr int P = 256 / (L / E);
+  auto al = [](const void* q, uintptr_t m) { return (((uintptr_t)q) & (m - 1)) == 0; };
+  const bool eligible = a.bins == L + 1 && a.window == N && a.hop > 0 && (a.hop % 2) == 0 && al(a.pcm, 8) &&
+                        al(a.win, 16) && al(a.o_re, 16) && al(a.o_im, 16) && al(a.o_mag, 16);
+  size_t nreg = 0;
+  if (eligible && a.pcm_len >= (size_t)N) {
+    const size_t fit = (a.pcm_len - N) / (size_t)a.hop + 1;  // frames fully inside the PCM
+    nreg = fit > a.frame_offset ? fit - a.frame_offset : 0;
+    if (nreg > a.frames) nreg = a.frames;
+    nreg -= nreg % ((size_t)P * K);  // whole tiles only
+  }
+  *take = eligible && a.frames - nreg <= 2 * (size_t)P * K ? a.frames : nreg;
+  *lead = nreg;
+  *blocks = nreg / ((size_t)P * K) + (*take > nreg ? 1 : 0);
+  return eligible;
+}
+
 template <int L, int E, int FPW, int IP, int DIR, int... Rs>
 hipError_t stft_launch_impl(const StftArgs& a, hipStream_t s) {
   const size_t grid = (a.frames + FPW - 1) / FPW;
@@ -916,26 +948,15 @@ size_t lines_slots() {
 // lines already).
 template <int L, int E, int K, int WPC, int OPT, int DIR, int... Rs>
 hipError_t stft_lines_launch_impl(const StftArgs& a, hipStream_t s) {
-  constexpr int N = 2 * L;
   constexpr int P = 256 / (L / E);
   if (a.bins != L + 1) return stft_mf_launch_impl<L, E, 6, 1, DIR, Rs...>(a, s);
-  auto al = [](const void* q, uintptr_t m) { return (((uintptr_t)q) & (m - 1)) == 0; };
-  const bool eligible = a.window == N && a.hop > 0 && (a.hop % 2) == 0 && al(a.pcm, 8) && al(a.win, 16) &&
-                        al(a.o_re, 16) && al(a.o_im, 16) && al(a.o_mag, 16);
-  size_t nreg = 0;
-  if (eligible && a.pcm_len >= (size_t)N) {
-    const size_t fit = (a.pcm_len - N) / (size_t)a.hop + 1;  // frames fully inside the PCM
-    nreg = fit > a.frame_offset ? fit - a.frame_offset : 0;
-    if (nreg > a.frames) nreg = a.frames;
-    nreg -= nreg % ((size_t)P * K);  // whole tiles only
-  }
-  // a short remainder rides along in workgroup 0 of the same launch
-  const size_t take = eligible && a.frames - nreg <= 2 * (size_t)P * K ? a.frames : nreg;
+  // whole tiles, and a short remainder riding along in workgroup 0 of the same launch
+  size_t nreg = 0, grid = 0, take = 0;
+  lines_layout<L, E, K>(a, &nreg, &grid, &take);
   if (take > 0) {
     StftArgs r = a;
     r.frames = take;
     r.lead = nreg;
-    const size_t grid = nreg / ((size_t)P * K) + (take > nreg ? 1 : 0);
     if (grid > 0x7fffffffull) return hipErrorInvalidValue;
     const dim3 g((unsigned)grid), blk(256);
 #define FFTFREE_LINES_GO(MODE)                                                                        \

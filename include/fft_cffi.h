@@ -272,11 +272,20 @@ typedef struct fft_ragged_request {
  * given here, so refill them between executions. Returns NULL on bad requests. */
 void* fft_ragged_create(fft_ragged_request* requests, size_t count, int pad_mode, int reduce_magnitude,
                         int store_polar, int apply_windows, int window_kind);
-/* additive. Times each tile shape (radix pattern and frames per workgroup) and tiles per
- * work item for every N of the batch on the batch itself (writing its outputs), keeps
- * the fastest, and rebuilds the work list and graph. Returns 1 on success. */
+/* additive. Times every engine of each N of the batch (the tile shapes: radix pattern and
+ * frames per tile; the line-aligned engine where every request of N qualifies) at 1, 2, 4
+ * and 8 blocks per work item on the batch itself (writing its outputs), by the median of
+ * `repeats` runs, keeps a new choice only when it is more than 3% faster than the current
+ * one, and rebuilds the work lists and graph. Returns 1 on success. */
 int fft_ragged_autotune(void* ragged, int repeats);
-/* additive. The whole batch as one graph launch (one persistent kernel) on `stream`. */
+/* additive. Sets the engine of window size n (an index below fft_ragged_engine_count: a
+ * tile shape, or a line-aligned engine) and the blocks per work item (>= 1), and rebuilds
+ * the graph. Returns 0 when the engine does not run size n or, for a line-aligned engine,
+ * some request of size n does not qualify (window != N, odd hop, buffers not aligned). */
+int fft_ragged_set_engine(void* ragged, int n, int engine, int blocks_per_item);
+int fft_ragged_engine_count(void);
+/* additive. The whole batch as one graph launch (one persistent launch per window size)
+ * on `stream`. */
 int fft_ragged_execute(void* ragged, void* stream);
 /* additive. The same outputs with one plain launch per request (the unbatched baseline). */
 int fft_ragged_execute_plain(void* ragged, void* stream);

@@ -123,3 +123,55 @@ def test_ragged_config5_batch(ff):
         assert np.array_equal(rb.outputs[i][2].cpu().numpy().view(np.uint32), mag.view(np.uint32))  # mag = |X| plane
     print(f"ragged 512 requests: worst rel_l2 / tolerance = {worst:.3f}", flush=True)
     rb.close()
+
+
+def test_ragged_every_engine_matches_oracle(ff):
+    """Every engine of every window size (tile shapes of each radix pattern, and the
+    line-aligned engine at N = 256, 512, 1024), forced through fft_ragged_set_engine at
+    1 and 3 blocks per work item, on a batch whose requests end in padded frames: every
+    frame of a sample of requests against the FP64 oracle."""
+    import torch
+    reqs, pcms, rb = make_batch(ff, 30, seed=8, lo=12, hi=15)
+    sizes = sorted({W for L, W, H in reqs})
+    count = ff.lib.fft_ragged_engine_count()
+    ran = {}
+    for n in sizes:
+        for engine in range(count):
+            for per in (1, 3):
+                if not rb.set_engine(n, engine, per):
+                    continue
+                ran.setdefault(n, set()).add(engine)
+                rb.execute()
+                torch.cuda.synchronize()
+                for i, (L, W, H) in enumerate(reqs):
+                    if W != n:
+                        continue
+                    check_planes(rb.outputs[i], oracle_planes(pcms[i].cpu().numpy(), W, H, rb.frames[i], "polar"),
+                                 "polar", W)
+    print(f"engines run per N: {ran}", flush=True)
+    for n in sizes:
+        assert len(ran.get(n, ())) >= (3 if n <= 1024 else 2), (n, ran.get(n))
+    rb.close()
+
+
+def test_ragged_misaligned_request_takes_the_tile_engine(ff):
+    """A request whose samples start one float past an 8-byte boundary does not qualify
+    for the line-aligned engine: its window size runs a tile shape, and the results
+    still match the oracle."""
+    import torch
+    W, H = 1024, 256
+    big = torch.from_numpy(noise_plus_sines(60001, seed=9)).cuda()
+    pcms = [big[1:], torch.from_numpy(noise_plus_sines(40000, seed=10)).cuda()]
+    rb = RaggedBatch(ff, [(p, W, H) for p in pcms])
+    assert "lines" not in rb.describe()
+    accepted = []
+    for e in range(ff.lib.fft_ragged_engine_count()):
+        if rb.set_engine(W, e, 2):  # the line-aligned engine of N = 1024 is refused
+            accepted.append(e)
+            assert "lines" not in rb.describe()
+    assert len(accepted) == 2  # the two tile shapes of N = 1024
+    rb.execute()
+    torch.cuda.synchronize()
+    for i, p in enumerate(pcms):
+        check_planes(rb.outputs[i], oracle_planes(p.cpu().numpy(), W, H, rb.frames[i], "polar"), "polar", W)
+    rb.close()
