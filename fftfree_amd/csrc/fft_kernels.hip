@@ -44,7 +44,7 @@ __device__ __forceinline__ float2 post_op(float2 X, int idx, int N, const C2CArg
   }
   if (a.post_mode >= 2 && idx < a.post_upper) {
     const float m = __builtin_sqrtf(__builtin_fmaf(X.x, X.x, X.y * X.y));
-    X = make_float2(m, a.post_mode == 3 ? atan2f(X.y, X.x) : 0.0f);
+    X = make_float2(m, a.post_mode == 3 ? fast_atan2f(X.y, X.x) : 0.0f);
   } else if (a.post_mode == 1) {
     X.y = 0.0f;
   }

@@ -41,7 +41,7 @@ __device__ __forceinline__ float2 post_large(float2 X, size_t idx, size_t N, con
   }
   if (a.post_mode >= 2 && idx < (size_t)a.post_upper) {
     const float m = __builtin_sqrtf(__builtin_fmaf(X.x, X.x, X.y * X.y));
-    X = make_float2(m, a.post_mode == 3 ? atan2f(X.y, X.x) : 0.0f);
+    X = make_float2(m, a.post_mode == 3 ? fast_atan2f(X.y, X.x) : 0.0f);
   } else if (a.post_mode == 1) {
     X.y = 0.0f;
   }

@@ -392,3 +392,34 @@ def test_lines_variants_same_bits_at_scale(ff, mode, samples):
             for i in range(3):
                 assert np.array_equal(got[i].view(np.uint32), ref[i].view(np.uint32)), (names[vi], i)
     ff.set_variant(p, -1)
+
+
+@pytest.mark.parametrize("n", [256, 1024, 4096])
+def test_polar_phase_is_atan2_of_the_complex_planes(ff, n):
+    """The polar planes' arg X (fast_atan2f) against atan2 of the complex planes of the
+    same input and kernel: equal on the axes (signed zeros: atan2(+-0, +0) = +-0,
+    atan2(+-0, -0) = +-pi, atan2(+-0, x < 0) = +-pi, atan2(y, +-0) = +-pi/2, in float)
+    and within 4e-7 rad elsewhere (the bound fast_atan2f states), silent frames (X = 0)
+    included; |X| equal bit for bit."""
+    import torch
+    W, H = n, n // 4
+    pcm = noise_plus_sines(60 * H + W, seed=31)
+    pcm[10 * H:20 * H + W] = 0.0  # whole frames of zeros
+    d = torch.from_numpy(pcm).cuda()
+    planes = {}
+    for polar in (False, True):
+        p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, store_polar=polar, half_spectrum=True,
+                         apply_windows=True, analysis_window=1, pad_mode=1)
+        frames = ff.frame_count(p, pcm.size, 1)
+        out = [torch.empty((frames, n // 2 + 1), device="cuda") for _ in range(3)]
+        assert ff.stft_device(p, d, *out, pad_mode=1) == frames
+        torch.cuda.synchronize()
+        planes[polar] = [o.cpu().numpy() for o in out]
+    re, im, mg = planes[False]
+    m, ph, _ = planes[True]
+    assert np.array_equal(m.view(np.uint32), mg.view(np.uint32))
+    want = np.arctan2(im.astype(np.float64), re.astype(np.float64))
+    axis = (re == 0) | (im == 0)
+    assert axis.sum() > 0
+    assert np.array_equal(ph[axis].view(np.uint32), want[axis].astype(np.float32).view(np.uint32))
+    assert np.max(np.abs(ph.astype(np.float64) - want)) <= 4e-7
