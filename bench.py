@@ -57,6 +57,7 @@ def parse():
                                                      "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
+    ap.add_argument("--n", type=int, default=1024, help="transform size of the c2c1024 workload (variant studies)")
     ap.add_argument("--window", type=int, default=0,
                     help="ragged workload: every request takes this window (hop W/4) instead of the mix")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -765,7 +766,7 @@ def main():
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "ones",
                 "config": {"workload": "elementwise copy, 512 MiB read + 512 MiB written per launch"}}
     else:
-        n, B = 1024, args.batch
+        n, B = args.n, args.batch
         p = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
         if args.variant >= 0:
             assert ff.set_variant(p, args.variant)
@@ -781,16 +782,16 @@ def main():
         achieved = alg_bytes / per_launch / 1e9
         from oracle import fp64
         fwd_err = fp64.rel_l2(y.cpu().numpy(), fp64.fft64(x.cpu().numpy()))
-        line = {"metric": f"C2C transforms/s, N=1024 batch {B}", "value": B * args.steps / gpu_s,
+        line = {"metric": f"C2C transforms/s, N={n} batch {B}", "value": B * args.steps / gpu_s,
                 "unit": "transforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": per_launch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                 "dtype": "f32", "data": "synthetic complex64 N(0,1), seed 1",
-                "config": {"workload": f"C2C forward N=1024 x {B}, Stockham",
-                           "variant": ff.variants(p)[ff.default_variant(p, B)],
+                "config": {"workload": f"C2C forward N={n} x {B}, Stockham",
+                           "variant": ff.variants(p)[args.variant if args.variant >= 0 else ff.default_variant(p, B)],
                            "rel_l2_fp64": {"forward": fwd_err, "transforms_checked": B,
                                            "tolerance": fp64.tolerance(n)}},
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_n1024_b{B}"),
+                             "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(f"c2c_n{n}_b{B}"),
                              "algorithmic_bytes_per_launch": alg_bytes, "kernel": "c2c_kernel"}}
     print(json.dumps(line), flush=True)
     ff.close()

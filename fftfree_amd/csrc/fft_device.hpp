@@ -379,5 +379,116 @@ struct WaveStockham {
   }
 };
 
+// Cross-lane exchanges in registers (gfx950 DPP and permlane swaps; no LDS). xlane_swap<B>
+// exchanges a register pair across lane bit B: afterwards a lane with bit B clear holds
+// (a, the partner's a) and a lane with it set (the partner's b, b), the partner being
+// lane ^ (1 << B). Lane bits 5 and 4 are one v_permlane32/16_swap per pair; bits 3 and 2
+// are two DPP row rotations that write only the lanes their bank mask selects (banks are
+// lane bits 2..3); bits 1 and 0 are DPP quad permutations and a select.
+template <int B>
+__device__ __forceinline__ void xlane_swap(float& a, float& b, int lane) {
+  const int ia = __builtin_bit_cast(int, a), ib = __builtin_bit_cast(int, b);
+  int na, nb;
+  if constexpr (B == 5 || B == 4) {
+    const auto r = B == 5 ? __builtin_amdgcn_permlane32_swap((unsigned)ia, (unsigned)ib, false, false)
+                          : __builtin_amdgcn_permlane16_swap((unsigned)ia, (unsigned)ib, false, false);
+    na = (int)r[0];
+    nb = (int)r[1];
+  } else if constexpr (B == 3) {
+    // row_ror:8 reads lane ^ 8 inside each 16-lane row; banks 2, 3 have bit 3 set
+    na = __builtin_amdgcn_update_dpp(ia, ib, 0x128, 0xf, 0xc, false);
+    nb = __builtin_amdgcn_update_dpp(ib, ia, 0x128, 0xf, 0x3, false);
+  } else if constexpr (B == 2) {
+    // row_ror:n reads lane (i - n) mod 16: lanes with bit 2 set (banks 1, 3) read i - 4
+    // (row_ror:4), the others i + 4 (row_ror:12)
+    na = __builtin_amdgcn_update_dpp(ia, ib, 0x124, 0xf, 0xa, false);
+    nb = __builtin_amdgcn_update_dpp(ib, ia, 0x12c, 0xf, 0x5, false);
+  } else {
+    static_assert(B == 0 || B == 1, "lane bits 0..5");
+    constexpr int ctrl = B == 0 ? 0xb1 : 0x4e;  // quad_perm [1,0,3,2] / [2,3,0,1]
+    const int pa = __builtin_amdgcn_mov_dpp(ia, ctrl, 0xf, 0xf, false);
+    const int pb = __builtin_amdgcn_mov_dpp(ib, ctrl, 0xf, 0xf, false);
+    const bool set = (lane >> B) & 1;
+    na = set ? pb : ia;
+    nb = set ? ib : pa;
+  }
+  a = __builtin_bit_cast(float, na);
+  b = __builtin_bit_cast(float, nb);
+}
+
+// One stage of a lane-digit / register-index transpose: bit j of the register index
+// trades places with lane bit B.
+template <int B, int J>
+__device__ __forceinline__ void xlane_stage8(float2* v, int lane) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e)
+    if (!(e & (1 << J))) {
+      xlane_swap<B>(v[e].x, v[e | (1 << J)].x, lane);
+      xlane_swap<B>(v[e].y, v[e | (1 << J)].y, lane);
+    }
+}
+// Transposes the 3-bit lane digit at lane bits S..S+2 with the index of v[0..8): lane
+// digit d, register e afterwards holds what lane digit e, register d held.
+template <int S>
+__device__ __forceinline__ void xlane_transpose8(float2* v, int lane) {
+  xlane_stage8<S, 0>(v, lane);
+  xlane_stage8<S + 1, 1>(v, lane);
+  xlane_stage8<S + 2, 2>(v, lane);
+}
+
+// The three radix-8 passes of a 512-point transform held by one wave (E = 8 values per
+// lane), with the exchanges between passes done in registers: the same butterflies,
+// twiddles and operation order as WaveStockham<512, 8, DIR, 1, 0, 8, 8, 8>, so the same
+// bits, but butterfly b of passes 1 and 2 runs on lane (b >> 3) | ((b & 7) << 3) (the
+// lane's 3-bit digits rotated), which makes each exchange one transpose of a lane digit
+// with the register index:
+//  * after pass 0, lane t register r holds element 8t + r; pass 1's butterfly
+//    b = b0 + 8 b1 needs elements b + 64r, held by lane b1 + 8r register b0: transpose
+//    lane bits 3..5 with the register index;
+//  * after pass 1, lane b1 + 8 b0 register r holds element b0 + 8r + 64 b1; pass 2's
+//    butterfly c = c0 + 8 c1 needs elements c + 64r, held by lane r + 8 c0 register c1:
+//    transpose lane bits 0..2 with the register index.
+// Twiddles: slots as load_twiddles fills them for thread index rot_lane(lane). On return
+// lane l register r holds X[rot_lane(l) + 64 r] (store_natural_rot puts it in LDS).
+// LO_LDS: the second exchange (lane bits 0..2, the dearer one in registers: DPP quad
+// permutations need a select each) goes through LDS as in WaveStockham instead, pass 2's
+// butterfly c runs on lane c (twiddle slots for thread index lane), and the result is in
+// WaveStockham's order (store_natural).
+__device__ __forceinline__ int rot_lane(int lane) { return (lane >> 3) | ((lane & 7) << 3); }
+template <int DIR, bool LO_LDS = false>
+struct WaveRegs888 {
+  template <class Hook>
+  __device__ static __forceinline__ void run(float2* v, int lane, float2* buf, const float2* twr, Hook&& hook) {
+    reg_dft<8, DIR>(v);
+    hook(0);
+    xlane_transpose8<3>(v, lane);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = c_mul(v[r], twr[8 + r - 1]);
+    reg_dft<8, DIR>(v);
+    hook(1);
+    if constexpr (LO_LDS) {
+      // pass 1's butterfly b = rot_lane(lane) wrote elements (b & 7) + 8r + 64 (b >> 3)
+      const int b = rot_lane(lane);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) buf[pidx((b & 7) + 8 * r + 64 * (b >> 3))] = v[r];
+      wave_lds_fence();
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = buf[pidx(lane + 64 * r)];
+      wave_lds_fence();
+    } else {
+      xlane_transpose8<0>(v, lane);
+    }
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = c_mul(v[r], twr[16 + r - 1]);
+    reg_dft<8, DIR>(v);
+    hook(2);
+  }
+};
+__device__ __forceinline__ void store_natural_rot(const float2* v, int lane, float2* dst) {
+  const int c = rot_lane(lane);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) dst[pidx(c + 64 * r)] = v[r];
+}
+
 }  // namespace dev
 }  // namespace fftfree

@@ -522,10 +522,17 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   constexpr int RUN = P * BINS;        // floats per plane per iteration
   constexpr int RING = ring_floats(RUN + 32);
   constexpr bool kRegTabs = (OPT & 4) != 0;
+  // OPT bit 3: the exchanges between passes in registers (DPP / permlane swaps); with bit
+  // 4 only the first one (lane bits 3..5), the second through LDS
+  constexpr bool kRegX = (OPT & 8) != 0;
+  constexpr bool kLoLds = (OPT & 16) != 0;
   static_assert(T <= 64 && (64 % T) == 0, "one transform per wave or less");
   static_assert(JF * T == L / 2, "T must divide L/2");
   static_assert(RUN % 4 == 0, "iteration runs must be whole 16-byte words");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
+  static_assert(!kRegX || (L == 512 && E == 8 && NP == 3 && R0 == 8 && Rl == 8 && SecondOf<Rs...>::value == 8),
+                "register exchanges: the 8x8x8 transform of one wave");
+  static_assert(!kLoLds || (kRegX && !kRegTabs), "bit 4 refines bit 3; pass 1's twiddles come from LDS");
   __shared__ float2 lds[P][LP];
   __shared__ __attribute__((aligned(16))) float ring[kB ? 2 : 1][RING];
   // window taps x 1/2, as (x[2n], x[2n+1]) pairs
@@ -542,6 +549,10 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   // slot-derived LDS and global address is scalar base + lane offset
   const int slot = T == 64 ? __builtin_amdgcn_readfirstlane(tid / T) : tid / T;
   const int t = tid & (T - 1);
+  // the butterfly index of pass 1 (tb1) and pass 2 (tb2), which set their twiddles: t, or
+  // with register exchanges the lane's digits rotated
+  const int tb1 = kRegX ? rot_lane(t) : t;
+  const int tb2 = kRegX && !kLoLds ? rot_lane(t) : t;
   float2* buf = &lds[slot][0];
   const int has_rest = a.frames > a.lead ? 1 : 0;
   // workgroup 0 takes the rest (when there is one); the others walk the tiles.
@@ -554,7 +565,7 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
 
   // per-thread constants: pass twiddles (registers)
   float2 twr[E * NP];
-  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, t, a.tw, 2);
+  StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, tb2, a.tw, 2);
 
   // window taps x 1/2 and split twiddles in registers (bit 2), else in LDS tables
   float2 wreg[kRegTabs ? E : 1];
@@ -615,16 +626,25 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
         v[e] = make_float2(cur[e].x * w.x, cur[e].y * w.y);
       }
     after_window();
-    WaveStockham<L, E, DIR, 1, 0, Rs...>::run(v, t, buf, twr, [&](int pass) {
+    auto hook = [&](int pass) {
       if (!kRegTabs && pass == 0) {  // the second pass's twiddles (slots E .. of twr), fresh from LDS
 #pragma unroll
         for (int q = 0; q < E / R1; ++q)
 #pragma unroll
-          for (int r = 1; r < R1; ++r) twr[E + q * (R1 - 1) + r - 1] = tw1[(t + q * T) & (NS1 - 1)][r - 1];
+          for (int r = 1; r < R1; ++r) twr[E + q * (R1 - 1) + r - 1] = tw1[(tb1 + q * T) & (NS1 - 1)][r - 1];
       }
       during(pass);
-    });
-    store_natural<L, E, Rl>(v, t, buf);
+    };
+    if constexpr (kRegX) {
+      WaveRegs888<DIR, kLoLds>::run(v, t, buf, twr, hook);
+      if constexpr (kLoLds)
+        store_natural<L, E, Rl>(v, t, buf);
+      else
+        store_natural_rot(v, t, buf);
+    } else {
+      WaveStockham<L, E, DIR, 1, 0, Rs...>::run(v, t, buf, twr, hook);
+      store_natural<L, E, Rl>(v, t, buf);
+    }
     __syncthreads();  // tiles: every wave has read the ring for the previous iteration's stores
     auto emit = [&](int k, float2 X) {
       if (DIR > 0) X = c_scale(X, scale);

@@ -12,6 +12,9 @@ const StftEntry kLines[] = {
     FFTFREE_STFT_LN(7, 512, 8, 1, 0, 3, "8x8x8 small-batch lines k=1", 8, 8, 8),
     FFTFREE_STFT_LN(8, 512, 8, 8, 3, 7, "8x8x8 lines regs wg=3 k=8", 8, 8, 8),
     FFTFREE_STFT_LN(9, 512, 8, 8, 3, 1, "8x8x8 lines mixed wg=3 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(10, 512, 8, 8, 0, 11, "8x8x8 lines dpp wg=4 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(11, 512, 8, 8, 3, 15, "8x8x8 lines dpp regs wg=3 k=8", 8, 8, 8),
+    FFTFREE_STFT_LN(12, 512, 8, 8, 0, 27, "8x8x8 lines dpp-hi wg=4 k=8", 8, 8, 8),
 };
 }  // namespace
 

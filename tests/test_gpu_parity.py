@@ -37,6 +37,27 @@ def test_c2c_all_lds_sizes(ff, n, inverse, kernel):
     assert fp64.rel_l2(y, fp64.fft64(x, bool(inverse))) <= fp64.tolerance(n)
 
 
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_c2c_512_register_exchanges_same_bits(ff, inverse):
+    """The N=512 C2C variant whose exchanges between radix-8 passes are DPP / permlane
+    swaps in registers runs the LDS kernel's butterflies and twiddles on permuted lanes:
+    every output bit equal to the LDS kernel's, and within the FP64 bound."""
+    n = 512
+    p = ff.make_plan(n, inverse=inverse, kernel=2, transform=0, half_spectrum=False)
+    names = ff.variants(p)
+    lds = [i for i, m in enumerate(names) if m.startswith("8x8x8 fpw")]
+    regs = [i for i, m in enumerate(names) if m.startswith("8x8x8 dpp")]
+    assert len(lds) == 1 and regs == [0], names  # the register kernel is the default
+    x = complex_normal(1000, n, seed=71 + inverse)
+    ys = []
+    for vi in (lds[0], regs[0]):
+        assert ff.set_variant(p, vi)
+        ys.append(ff.c2c(p, x))
+    ff.set_variant(p, -1)
+    assert np.array_equal(ys[0].view(np.uint32), ys[1].view(np.uint32))
+    assert fp64.rel_l2(ys[1], fp64.fft64(x, bool(inverse))) <= fp64.tolerance(n)
+
+
 def test_config1_c2c_1024_stockham_and_metadata(ff):
     """Plain C2C forward, Stockham (kernel code 2), N=1024, radix 4, batch 64, complex64
     N(0,1): FP64-oracle error within 2e-7*10; LayoutMap and TwiddleIndexMap bit-exact."""
