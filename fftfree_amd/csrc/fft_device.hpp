@@ -230,6 +230,23 @@ __device__ __forceinline__ int xcd_tile(int i, int G) {
   return x * q + (x < rem ? x : rem) + s;
 }
 
+// Launch order in groups of G: the workgroups dispatched to one XCD in a round of 8*G
+// consecutive ids (ids i with i % 8 == x) take G consecutive tiles, so neighbouring tiles
+// run close together in time and on one XCD (one L2). Block 0 keeps tile 0; a last
+// partial round maps to itself, so the map is a bijection on [0, G_total) for any size.
+template <int G>
+__device__ __forceinline__ int xcd_group_tile(int i, int total) {
+  if constexpr (G == 1) {
+    return i;
+  } else {
+    constexpr int S = 8 * G;
+    const int base = i - i % S;
+    if (base + S > total) return i;
+    const int k = i - base;
+    return base + (k & 7) * G + (k >> 3);
+  }
+}
+
 // Output modes of the post-transform hook (eigen_fft.hpp:2535-2596) + output copy
 // (fft_cffi.cpp:811-825): which value lands in each of the three planes.
 enum OutMode : int {

@@ -497,7 +497,11 @@ __global__ void __launch_bounds__(256, 3) stft_r2c_mf_kernel(StftArgs a) {
 // taps, the split twiddles and the second pass's twiddles in registers instead of LDS:
 // about a quarter less LDS traffic per frame for more registers (at most 3 workgroups
 // per CU); it measured no faster. The default runs 4 workgroups per CU (87 us; 3 per
-// CU: 88.5 us, 2 per CU: 95 us), tiles of 32 frames (64-frame tiles: 92 us).
+// CU: 88.5 us, 2 per CU: 95 us), tiles of 32 frames (64-frame tiles: 92 us), dealt in
+// launch order in groups of 8 consecutive tiles per XCD with the even tiles walked last
+// frames first (bits 5..7): a tile shares 768 samples with each neighbour, and walked so,
+// both read them at about the same time through one L2, which takes the samples' reads
+// from 1.10x to 1.02x at the same time per launch.
 // Whole tiles of regular frames (every frame inside the PCM) run as above; they need the
 // window to span N, an even hop, 8-byte aligned samples and 16-byte aligned window and
 // planes. The frames after the last whole tile (a few dozen) go to workgroup 0,
@@ -557,8 +561,11 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   const int has_rest = a.frames > a.lead ? 1 : 0;
   // workgroup 0 takes the rest (when there is one); the others walk the tiles.
   // OPT bit 0: tiles in launch order (consecutive tiles on consecutive workgroups, so on
-  // different XCDs) instead of contiguous runs per XCD
-  const long tile = (long)((OPT & 1) ? vblock : xcd_tile(vblock, vgrid)) - has_rest;
+  // different XCDs) instead of contiguous runs per XCD; bits 5..6 = log2 G > 0: launch
+  // order in groups of G consecutive tiles per XCD, so that the samples two neighbouring
+  // tiles share are fetched once, through one L2
+  constexpr int kGroup = 1 << ((OPT >> 5) & 3);
+  const long tile = (long)((OPT & 1) ? xcd_group_tile<kGroup>(vblock, vgrid) : xcd_tile(vblock, vgrid)) - has_rest;
   const int hop = a.hop;
   const float* __restrict__ win = a.win;
   constexpr float scale = DIR > 0 ? 1.0f / (float)N : 1.0f;
@@ -759,8 +766,17 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
     return g;
   };
   // WHICH < 0: all three planes; else that plane only
+  // OPT bit 7: tiles of even index run their iterations last frames first, so that the
+  // samples they share with a neighbouring tile are read by both at about the same time
+  // (and once through L2) rather than a tile's duration apart. Iteration it of such a tile
+  // stores the mirror image of the forward range, [K*RUN - hi(it), K*RUN - lo(it)):
+  // whole lines, as K*RUN is.
+  constexpr bool kAlt = (OPT & 128) != 0;
+  static_assert(!kAlt || (K * RUN) % 32 == 0, "mirrored store ranges need whole-line tiles");
+  const bool rev = kAlt && tile >= 0 && (tile & 1) == 0;
   auto store_round = [&](const Regions& rg, int it, int c, int which) {
-    const int lo = lines_lo<RUN, K>(it), n4 = (lines_hi<RUN, K>(it) - lo) / 4;
+    const int lo = rev ? K * RUN - lines_hi<RUN, K>(it) : lines_lo<RUN, K>(it);
+    const int n4 = (lines_hi<RUN, K>(it) - lines_lo<RUN, K>(it)) / 4;
     const int j = tid + c * NT;
     const int g = lo + 4 * j;
     // indexed in 16-byte words, so that the compiler emits one 16-byte LDS read (its
@@ -800,15 +816,17 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
 
   const Regions rg = regions(tile);
   float2 cur[E];
-  load_frame(tile, 0, cur);
+  load_frame(tile, rev ? K - 1 : 0, cur);
 #pragma unroll
   for (int i = 0; i < K; ++i) {
-    // bin k of slot s lands in the ring at float (i*P + s)*BINS + k
-    const int rbase = (i * P + slot) * BINS;
+    // iteration i transforms the frames of position pos; bin k of slot s lands in the ring
+    // at float (pos*P + s)*BINS + k
+    const int pos = rev ? K - 1 - i : i;
+    const int rbase = (pos * P + slot) * BINS;
     spectrum(
         cur,
         [&] {
-          if (i + 1 < K) load_frame(tile, i + 1, cur);
+          if (i + 1 < K) load_frame(tile, rev ? K - 2 - i : i + 1, cur);
         },
         [&](int pass) {  // the previous iteration's store rounds, spread over the passes
           if (i > 0) stores_in_pass(rg, i - 1, pass);
