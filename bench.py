@@ -57,7 +57,9 @@ def parse():
                                                      "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
-    ap.add_argument("--n", type=int, default=1024, help="transform size of the c2c1024 workload (variant studies)")
+    ap.add_argument("--n", type=int, default=1024,
+                    help="transform size of the stft and c2c1024 workloads (variant studies; 1024 is the headline)")
+    ap.add_argument("--polar", action="store_true", help="stft workload: polar planes instead of magnitude")
     ap.add_argument("--window", type=int, default=0,
                     help="ragged workload: every request takes this window (hop W/4) instead of the mix")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -94,11 +96,12 @@ def host_info() -> dict:
     return {"cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count()}
 
 
-def stft_rel_l2_fp64(pcm, planes, n, W, H, frames, win, reduce_magnitude, chunk=4096):
+def stft_rel_l2_fp64(pcm, planes, n, W, H, frames, win, reduce_magnitude, chunk=4096, polar=False):
     """Relative L2 of the device planes against the FP64 oracle over EVERY frame: framed
     float32 input (pcm*win rounded to float32, as the reference multiplies), float64 DFT,
     chunked so host memory stays small. planes = (re, im, mag) torch tensors, frames x
-    bins. Returns (complex rel-L2 of re + i im, rel-L2 of mag)."""
+    bins; polar planes hold (|X|, arg X, |X|) and are compared as |X| e^(i arg X).
+    Returns (complex rel-L2 of re + i im, rel-L2 of mag)."""
     import numpy as np
     from numpy.lib.stride_tricks import sliding_window_view
     bins = n // 2 + 1
@@ -113,7 +116,8 @@ def stft_rel_l2_fp64(pcm, planes, n, W, H, frames, win, reduce_magnitude, chunk=
         m = np.abs(X)
         re, im, mg = (p[f0:f1].cpu().numpy().astype(np.float64) for p in planes)
         want = m + 0j if reduce_magnitude else X
-        acc += [np.sum(np.abs(re + 1j * im - want) ** 2), np.sum(np.abs(want) ** 2),
+        got = re * np.exp(1j * im) if polar else re + 1j * im
+        acc += [np.sum(np.abs(got - want) ** 2), np.sum(np.abs(want) ** 2),
                 np.sum((mg - m) ** 2), np.sum(m ** 2)]
     return float((acc[0] / acc[1]) ** 0.5), float((acc[2] / acc[3]) ** 0.5)
 
@@ -249,9 +253,11 @@ def main():
     torch.cuda.set_device(0)
     ff = CffiFft()
     if args.workload == "stft":
-        n, W, H, L = 1024, 1024, 256, 1 << 24
-        p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,
-                         half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=0)
+        # N = 1024 is the headline; --n and --polar are for variant studies at other sizes
+        n, L = args.n, 1 << 24
+        W, H = n, n // 4
+        p = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=not args.polar,
+                         store_polar=args.polar, half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=0)
         if args.variant >= 0:
             assert ff.set_variant(p, args.variant)
         pcm = noise_plus_sines(L, seed=2)
@@ -268,12 +274,15 @@ def main():
         per_launch = gpu_s / args.steps
         alg_bytes = frames * (4 * H + 12 * bins)
         achieved = alg_bytes / per_launch / 1e9
-        traffic = read_traffic("stft_r2c_n1024")
+        headline = n == 1024 and not args.polar
+        traffic = read_traffic("stft_r2c_n1024") if headline else None
         from oracle import fp64
-        err_c, err_m = stft_rel_l2_fp64(pcm, out, n, W, H, frames, fp64.build_window(1, W), True)
-        chosen = ff.default_variant(p, frames)
+        err_c, err_m = stft_rel_l2_fp64(pcm, out, n, W, H, frames, fp64.build_window(1, W), not args.polar,
+                                        polar=args.polar)
+        chosen = args.variant if args.variant >= 0 else ff.default_variant(p, frames)
         line = {
-            "metric": "STFT frames/s & %HBM peak at N=1024; ragged-batch and fwd+bwd vs plain, 1 GPU",
+            "metric": ("STFT frames/s & %HBM peak at N=1024; ragged-batch and fwd+bwd vs plain, 1 GPU" if headline else
+                       f"STFT frames/s at N={n}" + (" (polar planes)" if args.polar else "")),
             "value": frames * args.steps / gpu_s,
             "unit": "frames/s",
             "n_gpus": 1,
@@ -285,8 +294,9 @@ def main():
             "vs_baseline": None,
             "dtype": "f32",
             "data": "synthetic: 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
-            "config": {"workload": "batched R2C STFT, Hann 1024, hop 256, half spectrum + magnitude, 2^24 samples "
-                                   "-> 65533 frames x 513 bins, three planes",
+            "config": {"workload": f"batched R2C STFT, Hann {W}, hop {H}, half spectrum + "
+                                   f"{'polar' if args.polar else 'magnitude'}, 2^24 samples -> {frames} frames x "
+                                   f"{bins} bins, three planes",
                        "n": n, "window": W, "hop": H, "samples": L, "frames": frames, "variant_index": chosen,
                        "variant": ff.variants(p)[chosen],
                        "rel_l2_fp64": {"planes_re_im": err_c, "plane_mag": err_m, "frames_checked": frames,

@@ -623,7 +623,7 @@ struct RaggedPlan {
     items.clear();
     std::vector<double> cost;
     const int cfg = cfg_of_n.at(n);
-    const unsigned per = (unsigned)tiles_of_n.at(n);
+    const unsigned per = ragged_cfg_flat(cfg) ? 1u : (unsigned)tiles_of_n.at(n);
     for (size_t r : rs) {
       size_t lead = 0;
       const size_t blocks = blocks_of(r, cfg, &lead);
@@ -678,7 +678,15 @@ struct RaggedPlan {
       build_n(n, reqs_of_n.at(n), items, chunks);
       if (items.empty()) continue;
       const int cfg = cfg_of_n.at(n);
-      launches.push_back(Launch{cfg, all_chunks.size(), grid(cfg)});
+      int g = 0;
+      if (ragged_cfg_flat(cfg)) {  // one workgroup per item of the longest chunk, per XCD
+        unsigned longest = 0;
+        for (int x = 0; x < 8; ++x) longest = std::max(longest, chunks[x + 1] - chunks[x]);
+        g = 8 * (int)longest;
+      } else {
+        g = grid(cfg);
+      }
+      launches.push_back(Launch{cfg, all_chunks.size(), g});
       for (unsigned& c : chunks) c += (unsigned)all.size();
       all.insert(all.end(), items.begin(), items.end());
       all_chunks.insert(all_chunks.end(), chunks.begin(), chunks.end());
@@ -764,6 +772,7 @@ struct RaggedPlan {
       for (int cfg : engines)
         for (int per : {1, 2, 4, 8}) {
           if (cfg == best_cfg && per == best_per) continue;
+          if (ragged_cfg_flat(cfg) && per > 1) continue;  // one block per item
           cfg_of_n[n] = cfg;
           tiles_of_n[n] = per;
           stage({n});
@@ -2075,8 +2084,12 @@ int fft_ragged_describe(void* ragged, char* buf, size_t buflen) {
   std::string out = "launches=" + std::to_string(p->launches.size()) + " items=" + std::to_string(p->n_items);
   for (auto& kv : p->cfg_of_n) {
     const fftfree::VariantInfo* v = fftfree::ragged_cfg(kv.second);
+    const bool flat = fftfree::ragged_cfg_flat(kv.second);
+    int g = 0;  // the staged launch's grid
+    for (const auto& l : p->launches)
+      if (l.cfg == kv.second) g = l.grid;
     out += " N" + std::to_string(kv.first) + ":" + v->radices + "/fpb" + std::to_string(v->fpw) + "/x" +
-           std::to_string(p->tiles_of_n[kv.first]) + "/grid" + std::to_string(p->grid(kv.second));
+           std::to_string(flat ? 1 : p->tiles_of_n[kv.first]) + "/grid" + std::to_string(g);
   }
   std::snprintf(buf, buflen, "%s", out.c_str());
   return 1;

@@ -212,7 +212,10 @@ struct RaggedItem {
 };
 int ragged_cfg_count();
 const VariantInfo* ragged_cfg(int id);  // l = N/2; fpw = frames per block
-bool ragged_cfg_lines(int id);          // a line-aligned engine
+bool ragged_cfg_lines(int id);
+// Engines that run one block per workgroup (no item loop): items of one block each, grid
+// = 8 x the longest chunk.
+bool ragged_cfg_flat(int id);          // a line-aligned engine
 // Blocks of request `a` under line-aligned engine `cfg`: frames in whole tiles (`lead`,
 // to be set in the request's StftArgs), blocks, frames covered (`take`); false when the
 // request does not qualify (window != N, odd hop, misaligned buffers).

@@ -6,7 +6,7 @@
 // described by a StftArgs block in device memory. The graph holds one persistent launch
 // per window size N in the batch, each compiled for that N alone (its own register
 // allocation, no shape dispatch). A launch runs one engine over the work items of its N:
-//  * the line-aligned engine (N = 256, 512, 1024): the body of stft_r2c_lines_kernel,
+//  * the line-aligned engine (N = 256 .. 2048): the body of stft_r2c_lines_kernel,
 //    whose blocks are tiles of 32 or 64 frames whose plane regions are whole 128-byte
 //    lines, plus a block for the frames after the last tile;
 //  * the tile engine (every N): StftTile, FPW frames per tile with a radix pattern.
@@ -36,13 +36,16 @@ namespace {
   X(7, 1024, 16, 4, "16x16x4", 16, 16, 4)    \
   X(8, 2048, 8, 1, "8x8x8x4", 8, 8, 8, 4)    \
   X(9, 2048, 16, 2, "16x16x8", 16, 16, 8)
-#define FFTFREE_RAGGED_LINES(X)                          \
-  X(10, 128, 4, 64, "lines 4x4x4x2", 4, 4, 4, 2)        \
-  X(11, 256, 4, 32, "lines 4x4x4x4", 4, 4, 4, 4)        \
-  X(12, 512, 8, 32, "lines 8x8x8", 8, 8, 8)
+// Line-aligned rows also carry the body's OPT: 3 = tiles in launch order, one plane per
+// pass; 263 = the same with the twiddle tables in registers and the window taps in LDS
+// (N = 2048, whose LDS allows two workgroups per CU: registers are what is left).
+#define FFTFREE_RAGGED_LINES(X)                             \
+  X(10, 128, 4, 64, "lines 4x4x4x2", 3, 4, 4, 4, 2)        \
+  X(11, 256, 4, 32, "lines 4x4x4x4", 3, 4, 4, 4, 4)        \
+  X(12, 512, 8, 32, "lines 8x8x8", 3, 8, 8, 8)             \
+  X(13, 1024, 16, 32, "lines 16x16x4", 263, 16, 16, 4)
 constexpr int kLinesFirst = 10;
-constexpr int kLinesK = 8;    // iterations per line-aligned tile
-constexpr int kLinesOpt = 3;  // tiles in launch order, one plane per pass
+constexpr int kLinesK = 8;  // iterations per line-aligned tile
 
 #define FFTFREE_RAGGED_INFO(ID, L, E, FPW, NAME, ...) {L, E, FPW, NAME},
 const VariantInfo kRaggedCfgs[] = {FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_INFO) FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_INFO)};
@@ -71,11 +74,12 @@ __device__ __noinline__ void ragged_tile_block(const StftArgs* __restrict__ ap, 
   const StftArgs a = *ap;
   StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::run(a, tile, lds, nullptr);
 }
-template <int MODE, int L, int E, int... Rs>
+template <int MODE, int L, int E, int OPT, int... Rs>
 __device__ __noinline__ void ragged_lines_block(const StftArgs* __restrict__ ap, int block) {
   const StftArgs a = *ap;
-  stft_lines_body<L, E, kLinesK, kLinesOpt, MODE, -1, Rs...>(a, block, 0);
+  stft_lines_body<L, E, kLinesK, OPT, MODE, -1, Rs...>(a, block, 0);
 }
+
 
 // Tile engine, one shape per kernel.
 template <int MODE, int L, int E, int FPW, int... Rs>
@@ -88,11 +92,34 @@ __global__ void __launch_bounds__(256) stft_ragged_tile_kernel(const StftArgs* _
 
 // Line-aligned engine: block w.tile + k of request w.req (block 0 = the frames after the
 // last tile when the request has them; the host set each request's `lead`).
-template <int MODE, int L, int E, int... Rs>
+template <int MODE, int L, int E, int OPT, int... Rs>
 __global__ void __launch_bounds__(256) stft_ragged_lines_kernel(const StftArgs* __restrict__ reqs,
                                                                 const RaggedItem* __restrict__ items,
                                                                 const unsigned* __restrict__ chunks) {
-  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, Rs...>(reqs + w.req, (int)(w.tile + k))))
+  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, OPT, Rs...>(reqs + w.req, (int)(w.tile + k))))
+}
+// Line-aligned engine at N = 2048, one block per workgroup: the workgroups of XCD x take
+// the items of chunk x in order, one each (every item is one block; the host sizes the
+// grid to 8 times the longest chunk). A block of 16 values per lane needs the 256
+// registers of two waves per SIMD; as a call inside the item loop above, or inlined into
+// it, it needs ~290 plus scratch, so this kernel has no loop.
+template <int MODE, int L, int E, int OPT, int... Rs>
+__global__ void __launch_bounds__(256, 2) stft_ragged_lines_flat_kernel(const StftArgs* __restrict__ reqs,
+                                                                        const RaggedItem* __restrict__ items,
+                                                                        const unsigned* __restrict__ chunks) {
+  const int x = blockIdx.x & 7;
+  const unsigned i = chunks[x] + (blockIdx.x >> 3);
+  if (i >= chunks[x + 1]) return;  // uniform over the workgroup
+  const RaggedItem w = items[i];
+  const StftArgs a = reqs[w.req];
+  stft_lines_body<L, E, kLinesK, OPT, MODE, -1, Rs...>(a, (int)w.tile, 0);
+}
+template <int MODE, int L, int E, int OPT, int... Rs>
+const void* ragged_lines_fn() {
+  if constexpr (L >= 1024)
+    return reinterpret_cast<const void*>(&stft_ragged_lines_flat_kernel<MODE, L, E, OPT, Rs...>);
+  else
+    return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, OPT, Rs...>);
 }
 
 template <int MODE>
@@ -102,8 +129,8 @@ const void* ragged_kernel_mode(int cfg) {
   case ID: return reinterpret_cast<const void*>(&stft_ragged_tile_kernel<MODE, L, E, FPW, __VA_ARGS__>);
     FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_TILE_CASE)
 #undef FFTFREE_RAGGED_TILE_CASE
-#define FFTFREE_RAGGED_LINES_CASE(ID, L, E, FPW, NAME, ...) \
-  case ID: return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, __VA_ARGS__>);
+#define FFTFREE_RAGGED_LINES_CASE(ID, L, E, FPW, NAME, OPT, ...) \
+  case ID: return ragged_lines_fn<MODE, L, E, OPT, __VA_ARGS__>();
     FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LINES_CASE)
 #undef FFTFREE_RAGGED_LINES_CASE
     default: return nullptr;
@@ -124,6 +151,7 @@ const void* ragged_kernel(int mode, int cfg) {
 int ragged_cfg_count() { return kRaggedCfgCount; }
 const VariantInfo* ragged_cfg(int id) { return id >= 0 && id < kRaggedCfgCount ? &kRaggedCfgs[id] : nullptr; }
 bool ragged_cfg_lines(int id) { return id >= kLinesFirst && id < kRaggedCfgCount; }
+bool ragged_cfg_flat(int id) { return ragged_cfg_lines(id) && kRaggedCfgs[id].l >= 1024; }
 
 bool ragged_lines_layout(int cfg, const StftArgs& a, size_t* lead, size_t* blocks, size_t* take) {
   switch (cfg) {

@@ -524,8 +524,14 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   constexpr int BINS = L + 1;
   constexpr bool kB = mode_has_b<MODE>();
   constexpr int RUN = P * BINS;        // floats per plane per iteration
-  constexpr int RING = ring_floats(RUN + 32);
+  // LDS ring of the split's output: a power of two (index masks) unless that would double
+  // a large ring (N = 2048: 4100 + 32 floats per plane), which is then sized to fit and
+  // indexed with one conditional wrap
+  constexpr bool kPow2Ring = ring_floats(RUN + 32) < 2 * 4096;
+  constexpr int RING = kPow2Ring ? ring_floats(RUN + 32) : ((RUN + 32 + 3) / 4) * 4;
   constexpr bool kRegTabs = (OPT & 4) != 0;
+  // bit 8 (with bit 2): the window taps stay in LDS, only the twiddles go to registers
+  constexpr bool kRegWin = kRegTabs && (OPT & 256) == 0;
   // OPT bit 3: the exchanges between passes in registers (DPP / permlane swaps); with bit
   // 4 only the first one (lane bits 3..5), the second through LDS
   constexpr bool kRegX = (OPT & 8) != 0;
@@ -540,7 +546,7 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   __shared__ float2 lds[P][LP];
   __shared__ __attribute__((aligned(16))) float ring[kB ? 2 : 1][RING];
   // window taps x 1/2, as (x[2n], x[2n+1]) pairs
-  __shared__ __attribute__((aligned(16))) float2 wtab[kRegTabs ? 2 : L];
+  __shared__ __attribute__((aligned(16))) float2 wtab[kRegWin ? 2 : L];
   __shared__ float2 stw[kRegTabs ? 1 : L / 2 + 1];  // split twiddles, p = 0 .. L/2
   // second-pass twiddles: they depend on t mod R0 only, so (LDS tables) they are reloaded
   // before that pass of every frame instead of living in registers across the kernel
@@ -575,10 +581,10 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   StockhamPlan<L, E, DIR, Rs...>::load_twiddles(twr, tb2, a.tw, 2);
 
   // window taps x 1/2 and split twiddles in registers (bit 2), else in LDS tables
-  float2 wreg[kRegTabs ? E : 1];
+  float2 wreg[kRegWin ? E : 1];
   float2 wsp[kRegTabs ? JF : 1];
   float2 wmid = make_float2(0.0f, 0.0f);
-  if constexpr (kRegTabs) {
+  if constexpr (kRegWin) {
     const float2* __restrict__ w2 = reinterpret_cast<const float2*>(win ? win : reinterpret_cast<const float*>(a.tw));
 #pragma unroll
     for (int q = 0; q < E / R0; ++q)
@@ -587,6 +593,8 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
         const float2 x = w2[t + q * T + r * (L / R0)];
         wreg[q * R0 + r] = win ? make_float2(0.5f * x.x, 0.5f * x.y) : make_float2(0.5f, 0.5f);
       }
+  }
+  if constexpr (kRegTabs) {
 #pragma unroll
     for (int j = 0; j < JF; ++j) wsp[j] = load_tw<DIR>(a.tw, t + j * T);
     wmid = load_tw<DIR>(a.tw, L / 2);
@@ -598,6 +606,8 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
       const int m = tid / (R1 - 1), r = tid % (R1 - 1) + 1;
       tw1[m][r - 1] = load_tw<DIR>(a.tw, m * (L / R1 / NS1) * 2 * r);
     }
+  }
+  if constexpr (!kRegWin) {
     // window taps times 1/2 into LDS (1/2 when windows are off), 16-byte loads; held
     // there rather than in registers so that four workgroups fit per CU
     static_assert(L % 2 == 0, "whole 16-byte words of taps");
@@ -626,7 +636,7 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
       for (int r = 0; r < R0; ++r) {
         const int e = q * R0 + r;
         float2 w;
-        if constexpr (kRegTabs)
+        if constexpr (kRegWin)
           w = wreg[e];
         else
           w = wtab[t + q * T + r * (L / R0)];
@@ -781,7 +791,15 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
     const int g = lo + 4 * j;
     // indexed in 16-byte words, so that the compiler emits one 16-byte LDS read (its
     // lanes cover consecutive banks) instead of 4-byte pairs that conflict
-    const int w4 = (g >> 2) & (RING / 4 - 1);
+    int w4;
+    if constexpr (kPow2Ring) {
+      w4 = (g >> 2) & (RING / 4 - 1);
+    } else {  // lo's word mod RING/4 (uniform) + j; lanes past the range (their stores
+              // are dropped) read word w0, so every address stays inside the ring
+      const int w0 = (lo >> 2) % (RING / 4);
+      w4 = w0 + (j < n4 ? j : 0);  // n4 <= (RUN + 31) / 4 < RING / 4
+      if (w4 >= RING / 4) w4 -= RING / 4;
+    }
     const f32x4 A4 = reinterpret_cast<const f32x4*>(&ring[0][0])[w4];
     f32x4 B4 = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (kB) B4 = reinterpret_cast<const f32x4*>(&ring[kB ? 1 : 0][0])[w4];
@@ -822,7 +840,7 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
     // iteration i transforms the frames of position pos; bin k of slot s lands in the ring
     // at float (pos*P + s)*BINS + k
     const int pos = rev ? K - 1 - i : i;
-    const int rbase = (pos * P + slot) * BINS;
+    const int rbase = kPow2Ring ? (pos * P + slot) * BINS : ((pos * P + slot) * BINS) % RING;
     spectrum(
         cur,
         [&] {
@@ -834,7 +852,13 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
         [&](int k, float2 X) {
           float A, B;
           stage_vals<MODE>(X, A, B);
-          const int g = (rbase + k) & (RING - 1);
+          int g;
+          if constexpr (kPow2Ring) {
+            g = (rbase + k) & (RING - 1);
+          } else {  // rbase < RING, k <= L < RING
+            g = rbase + k;
+            if (g >= RING) g -= RING;
+          }
           ring[0][g] = A;
           if constexpr (kB) ring[kB ? 1 : 0][g] = B;
         });
@@ -846,8 +870,15 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   }
 }
 
+// Waves per SIMD the lines kernel is compiled for: 4 (3 with register tables); at N = 2048
+// two, which its LDS (frame buffers and ring, about 68 KB per workgroup) allows anyway.
+template <int L, int OPT>
+constexpr int lines_waves_per_eu() {
+  return L >= 1024 ? 2 : ((OPT & 4) ? 3 : 4);
+}
+
 template <int L, int E, int K, int OPT, int MODE, int DIR, int... Rs>
-__global__ void __launch_bounds__(256, (OPT & 4) ? 3 : 4) stft_r2c_lines_kernel(StftArgs a) {
+__global__ void __launch_bounds__(256, (lines_waves_per_eu<L, OPT>())) stft_r2c_lines_kernel(StftArgs a) {
   stft_lines_body<L, E, K, OPT, MODE, DIR, Rs...>(a, (int)blockIdx.x, (int)gridDim.x);
 }
 

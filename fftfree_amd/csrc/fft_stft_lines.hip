@@ -1,4 +1,4 @@
-// fft_stft_lines.hip -- instantiations of the line-aligned N = 1024 STFT kernel
+// fft_stft_lines.hip -- instantiations of the line-aligned N = 1024 and 2048 STFT kernel
 // (stft_r2c_lines_kernel, fft_stft_impl.hpp) and their rows of the variant table.
 #include "fft_stft_impl.hpp"
 
@@ -23,6 +23,12 @@ const StftEntry kLines[] = {
     FFTFREE_STFT_LN(16, 512, 8, 8, 0, 163, "8x8x8 lines pairs-alt wg=4 k=8", 8, 8, 8),
     FFTFREE_STFT_LN(17, 512, 8, 8, 0, 195, "8x8x8 lines quads-alt wg=4 k=8", 8, 8, 8),
     FFTFREE_STFT_LN(18, 512, 8, 8, 0, 3, "8x8x8 lines launch-order wg=4 k=8", 8, 8, 8),
+    // N = 2048 (the default there): one wave per frame (16 values per lane), twiddles in
+    // registers, window taps in LDS, 2 workgroups per CU (LDS: frame buffers and the
+    // ring); 122.5 us polar against 188.6 us for the multi-frame kernel at 32765 frames.
+    // Window taps in registers too: 130.2 us (256 VGPRs and a small spill).
+    FFTFREE_STFT_LN(0, 1024, 16, 8, 0, 487, "16x16x4 lines ldswin wg=2 k=8", 16, 16, 4),
+    FFTFREE_STFT_LN(4, 1024, 16, 8, 0, 231, "16x16x4 lines wg=2 k=8", 16, 16, 4),
 };
 }  // namespace
 

@@ -7,7 +7,7 @@ namespace {
 const StftEntry kMf[] = {
     FFTFREE_STFT_MF(0, 256, 16, 4, "16x16 multiframe k=4", 16, 16),
     FFTFREE_STFT_PL(3, 512, 8, 6, "8x8x8 tile-preload k=6", 8, 8, 8),
-    FFTFREE_STFT_MF(0, 1024, 16, 8, "16x16x4 multiframe k=8", 16, 16, 4),
+    FFTFREE_STFT_MF(1, 1024, 16, 8, "16x16x4 multiframe k=8", 16, 16, 4),
 };
 }  // namespace
 

@@ -281,11 +281,13 @@ int fft_ragged_autotune(void* ragged, int repeats);
 /* additive. Sets the engine of window size n (an index below fft_ragged_engine_count: a
  * tile shape, or a line-aligned engine) and the blocks per work item (>= 1), and rebuilds
  * the graph. Returns 0 when the engine does not run size n or, for a line-aligned engine,
- * some request of size n does not qualify (window != N, odd hop, buffers not aligned). */
+ * some request of size n does not qualify (window != N, odd hop, buffers not aligned). The
+ * line-aligned engine of N = 2048 runs one block per workgroup and ignores
+ * blocks_per_item. */
 int fft_ragged_set_engine(void* ragged, int n, int engine, int blocks_per_item);
 int fft_ragged_engine_count(void);
-/* additive. The whole batch as one graph launch (one persistent launch per window size)
- * on `stream`. */
+/* additive. The whole batch as one graph launch (one launch per window size: persistent
+ * workgroups walking work items, or at N = 2048 one workgroup per block) on `stream`. */
 int fft_ragged_execute(void* ragged, void* stream);
 /* additive. The same outputs with one plain launch per request (the unbatched baseline). */
 int fft_ragged_execute_plain(void* ragged, void* stream);

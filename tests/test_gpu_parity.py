@@ -312,14 +312,15 @@ def test_complex_batched_inverse_roundtrip(ff, kernel):
 
 
 @pytest.mark.parametrize("mode", [(1, 0, 1), (0, 0, 0), (0, 1, 1), (0, 0, 1)])
-def test_stft_every_variant_matches_oracle(ff, mode):
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_stft_every_variant_matches_oracle(ff, mode, n):
     """Every kernel variant the autotuner may pick (radix pattern, frames per
     workgroup, multi-frame) meets the oracle bound; variants with the same radix pattern
     agree to well inside it (different kernels may fuse multiply-adds differently, so
     bits can differ); misaligned output planes take the scalar path."""
     import torch
     reduce_mag, polar, half = mode
-    n, W, H = 1024, 1024, 256
+    W, H = n, n // 4
     pcm = noise_plus_sines(200 * H + W + 77, seed=21)
     p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=bool(reduce_mag),
                      store_polar=bool(polar), half_spectrum=bool(half), apply_windows=True, analysis_window=1)
@@ -371,6 +372,43 @@ def test_stft_multiframe_remainders(ff, frames, half):
     ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, fp64.build_window(1, W), False, 1, False, False, bool(half))
     assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= fp64.tolerance(n)
     assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("mode", [(1, 0), (0, 1), (0, 0)])
+def test_lines_2048_every_frame_vs_fp64(ff, mode):
+    """The N=2048 line-aligned variants (one wave per frame, 16 values per lane, the
+    split's output ring sized to fit rather than a power of two) on 2^22 samples (8189
+    frames: 255 whole tiles, some walked last frames first, and a rest block): every frame
+    of all three planes against the FP64 oracle, and the same bits in both variants
+    (window taps in registers or in LDS)."""
+    import torch
+    reduce_mag, polar = mode
+    n, W, H = 2048, 2048, 512
+    pcm = noise_plus_sines((1 << 22) + 333, seed=31)
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=bool(reduce_mag),
+                     store_polar=bool(polar), half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=1)
+    frames = ff.frame_count(p, pcm.size, 1)
+    bins = p.bins
+    d = torch.from_numpy(pcm).cuda()
+    names = ff.variants(p)
+    lines = [vi for vi, name in enumerate(names) if " lines" in name]
+    assert len(lines) == 2, names
+    ore, oim, omg = fp64.stft_planes(pcm, n, W, H, frames, fp64.build_window(1, W), False, 1, reduce_mag, polar, True)
+    ref = None
+    for vi in lines:
+        assert ff.set_variant(p, vi)
+        planes = [torch.full((frames, bins), float("nan"), device="cuda") for _ in range(3)]
+        assert ff.stft_device(p, d, *planes, pad_mode=1) == frames
+        torch.cuda.synchronize()
+        got = [x.cpu().numpy() for x in planes]
+        assert spectra_close(got[0], got[1], ore, oim, polar, fp64.tolerance(n)), names[vi]
+        assert fp64.rel_l2(got[2], omg) <= fp64.tolerance(n), names[vi]
+        if ref is None:
+            ref = got
+        else:
+            for i in range(3):
+                assert np.array_equal(got[i].view(np.uint32), ref[i].view(np.uint32)), (names[vi], i)
+    ff.set_variant(p, -1)
 
 
 @pytest.mark.parametrize("mode,samples", [((1, 0), 1 << 24), ((0, 0), 30000 * 256 + 768 + 100), ((0, 1), 26000 * 256 + 1000)])
