@@ -571,6 +571,10 @@ struct RaggedPlan {
   size_t n_items = 0;
   int device = 0;
   hipStream_t stream = nullptr;      // private stream for capture and tuning
+  // capture-only branches: the graph runs the launches of different N side by side
+  static constexpr int kBranches = 4;
+  hipStream_t branch[kBranches] = {};
+  std::vector<hipEvent_t> marks;     // fork and joins of the capture
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -578,6 +582,9 @@ struct RaggedPlan {
     (void)hipSetDevice(device);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
+    for (hipStream_t b : branch)
+      if (b) (void)hipStreamDestroy(b);
     if (stream) (void)hipStreamDestroy(stream);
     d_args.release();
     d_items.release();
@@ -720,6 +727,36 @@ struct RaggedPlan {
     return ns;
   }
 
+  // The staged launches as branches of the capture: launch i on branch i % kBranches,
+  // forked from and joined back into `stream`, so that the launches of different N run
+  // side by side and each one's tail (its last partial wave of workgroups) overlaps the
+  // others' work instead of leaving the GPU idle. Launches of one N share no outputs with
+  // another N's, so any order is correct.
+  hipError_t run_branched() {
+    if (launches.size() < 2) return run(stream);
+    if (!branch[0])
+      for (hipStream_t& b : branch) hip_check(hipStreamCreateWithFlags(&b, hipStreamNonBlocking), "hipStreamCreate");
+    while (marks.size() < launches.size() + 1) {
+      hipEvent_t e = nullptr;
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      marks.push_back(e);
+    }
+    hipError_t err = hipEventRecord(marks[0], stream);
+    for (size_t i = 0; i < launches.size() && err == hipSuccess; ++i) {
+      const Launch& l = launches[i];
+      hipStream_t b = branch[i % kBranches];
+      if (i < kBranches) err = hipStreamWaitEvent(b, marks[0], 0);
+      if (err == hipSuccess)
+        err = launch_stft_ragged(d_args.as<StftArgs>(), d_items.as<RaggedItem>(),
+                                 d_chunks.as<unsigned>() + l.chunk_off, mode, l.cfg, l.grid, b);
+      if (err == hipSuccess && i + kBranches >= launches.size()) {  // the branch's last launch
+        err = hipEventRecord(marks[1 + i], b);
+        if (err == hipSuccess) err = hipStreamWaitEvent(stream, marks[1 + i], 0);
+      }
+    }
+    return err;
+  }
+
   // Stage every size and capture the launches in a graph.
   void commit() {
     stage(sizes());
@@ -728,7 +765,7 @@ struct RaggedPlan {
     exec = nullptr;
     graph = nullptr;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    const hipError_t le = run(stream);
+    const hipError_t le = run_branched();
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(stream, &g);
     hip_check(le, "launch_stft_ragged (capture)");

@@ -355,6 +355,16 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Exchange barrier of a transform held by T threads: a wave fence when it is one wave,
+// else a workgroup barrier (every wave of the workgroup runs the same pass schedule).
+template <int T>
+__device__ __forceinline__ void pass_sync() {
+  if constexpr (T <= 64)
+    wave_lds_fence();
+  else
+    __syncthreads();
+}
+
 template <int L, int E, int DIR, int Ns, int P, int R, int... Rest>
 struct WaveStockham {
   static constexpr int T = L / E;
@@ -382,7 +392,7 @@ struct WaveStockham {
 #pragma unroll
         for (int r = 0; r < R; ++r) buf[pidx(o + r * Ns)] = v[q * R + r];
       }
-      wave_lds_fence();
+      pass_sync<T>();
       constexpr int Rn = FirstOf<Rest...>::value;
 #pragma unroll
       for (int q = 0; q < E / Rn; ++q) {
@@ -390,7 +400,7 @@ struct WaveStockham {
 #pragma unroll
         for (int r = 0; r < Rn; ++r) v[q * Rn + r] = buf[pidx(bb + r * (L / Rn))];
       }
-      wave_lds_fence();
+      pass_sync<T>();
       WaveStockham<L, E, DIR, Ns * R, P + 1, Rest...>::run(v, t, buf, twr, hook);
     }
   }

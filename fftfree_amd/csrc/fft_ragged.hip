@@ -23,8 +23,8 @@ using namespace dev;
 namespace {
 
 // Engines: ID, L (= N/2), E, frames per block, name, radices. IDs below kLinesFirst are
-// tile-engine shapes (FPW frames per tile); the others are line-aligned engines (K = 8
-// iterations of P = 256/(L/E) frames per tile).
+// tile-engine shapes (FPW frames per tile); the others are line-aligned engines (tiles of
+// FPW frames: K = FPW / P iterations of P = 256/(L/E) frames, a whole number of lines).
 #define FFTFREE_RAGGED_TILES(X)               \
   X(0, 128, 8, 16, "8x8x2", 8, 8, 2)         \
   X(1, 128, 16, 32, "16x8", 16, 8)           \
@@ -38,14 +38,21 @@ namespace {
   X(9, 2048, 16, 2, "16x16x8", 16, 16, 8)
 // Line-aligned rows also carry the body's OPT: 3 = tiles in launch order, one plane per
 // pass; 263 = the same with the twiddle tables in registers and the window taps in LDS
-// (N = 2048, whose LDS allows two workgroups per CU: registers are what is left).
+// (N = 2048, whose LDS allows two workgroups per CU: registers are what is left); 515 = the
+// window taps in registers, the twiddles in LDS (N = 4096, two waves per frame, whose LDS
+// has room for the twiddles but not for 16 KB of taps).
 #define FFTFREE_RAGGED_LINES(X)                             \
   X(10, 128, 4, 64, "lines 4x4x4x2", 3, 4, 4, 4, 2)        \
   X(11, 256, 4, 32, "lines 4x4x4x4", 3, 4, 4, 4, 4)        \
   X(12, 512, 8, 32, "lines 8x8x8", 3, 8, 8, 8)             \
-  X(13, 1024, 16, 32, "lines 16x16x4", 263, 16, 16, 4)
+  X(13, 1024, 16, 32, "lines 16x16x4", 263, 16, 16, 4)      \
+  X(14, 2048, 16, 32, "lines 16x16x8", 515, 16, 16, 8)
 constexpr int kLinesFirst = 10;
-constexpr int kLinesK = 8;  // iterations per line-aligned tile
+// iterations per line-aligned tile of FPW frames (P = 256 / (L / E) frames per iteration)
+template <int L, int E, int FPW>
+constexpr int lines_k() {
+  return FPW / (256 / (L / E));
+}
 
 #define FFTFREE_RAGGED_INFO(ID, L, E, FPW, NAME, ...) {L, E, FPW, NAME},
 const VariantInfo kRaggedCfgs[] = {FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_INFO) FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_INFO)};
@@ -74,10 +81,10 @@ __device__ __noinline__ void ragged_tile_block(const StftArgs* __restrict__ ap, 
   const StftArgs a = *ap;
   StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::run(a, tile, lds, nullptr);
 }
-template <int MODE, int L, int E, int OPT, int... Rs>
+template <int MODE, int L, int E, int K, int OPT, int... Rs>
 __device__ __noinline__ void ragged_lines_block(const StftArgs* __restrict__ ap, int block) {
   const StftArgs a = *ap;
-  stft_lines_body<L, E, kLinesK, OPT, MODE, -1, Rs...>(a, block, 0);
+  stft_lines_body<L, E, K, OPT, MODE, -1, Rs...>(a, block, 0);
 }
 
 
@@ -92,18 +99,18 @@ __global__ void __launch_bounds__(256) stft_ragged_tile_kernel(const StftArgs* _
 
 // Line-aligned engine: block w.tile + k of request w.req (block 0 = the frames after the
 // last tile when the request has them; the host set each request's `lead`).
-template <int MODE, int L, int E, int OPT, int... Rs>
+template <int MODE, int L, int E, int K, int OPT, int... Rs>
 __global__ void __launch_bounds__(256) stft_ragged_lines_kernel(const StftArgs* __restrict__ reqs,
                                                                 const RaggedItem* __restrict__ items,
                                                                 const unsigned* __restrict__ chunks) {
-  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, OPT, Rs...>(reqs + w.req, (int)(w.tile + k))))
+  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, K, OPT, Rs...>(reqs + w.req, (int)(w.tile + k))))
 }
 // Line-aligned engine at N = 2048, one block per workgroup: the workgroups of XCD x take
 // the items of chunk x in order, one each (every item is one block; the host sizes the
 // grid to 8 times the longest chunk). A block of 16 values per lane needs the 256
 // registers of two waves per SIMD; as a call inside the item loop above, or inlined into
 // it, it needs ~290 plus scratch, so this kernel has no loop.
-template <int MODE, int L, int E, int OPT, int... Rs>
+template <int MODE, int L, int E, int K, int OPT, int... Rs>
 __global__ void __launch_bounds__(256, 2) stft_ragged_lines_flat_kernel(const StftArgs* __restrict__ reqs,
                                                                         const RaggedItem* __restrict__ items,
                                                                         const unsigned* __restrict__ chunks) {
@@ -111,15 +118,17 @@ __global__ void __launch_bounds__(256, 2) stft_ragged_lines_flat_kernel(const St
   const unsigned i = chunks[x] + (blockIdx.x >> 3);
   if (i >= chunks[x + 1]) return;  // uniform over the workgroup
   const RaggedItem w = items[i];
-  const StftArgs a = reqs[w.req];
-  stft_lines_body<L, E, kLinesK, OPT, MODE, -1, Rs...>(a, (int)w.tile, 0);
+  // by reference: the body reads the request's fields from global memory (uniform loads);
+  // a local copy would have its address taken and live in scratch
+  const StftArgs& a = reqs[w.req];
+  stft_lines_body<L, E, K, OPT, MODE, -1, Rs...>(a, (int)w.tile, 0);
 }
-template <int MODE, int L, int E, int OPT, int... Rs>
+template <int MODE, int L, int E, int K, int OPT, int... Rs>
 const void* ragged_lines_fn() {
   if constexpr (L >= 1024)
-    return reinterpret_cast<const void*>(&stft_ragged_lines_flat_kernel<MODE, L, E, OPT, Rs...>);
+    return reinterpret_cast<const void*>(&stft_ragged_lines_flat_kernel<MODE, L, E, K, OPT, Rs...>);
   else
-    return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, OPT, Rs...>);
+    return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, K, OPT, Rs...>);
 }
 
 template <int MODE>
@@ -130,7 +139,7 @@ const void* ragged_kernel_mode(int cfg) {
     FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_TILE_CASE)
 #undef FFTFREE_RAGGED_TILE_CASE
 #define FFTFREE_RAGGED_LINES_CASE(ID, L, E, FPW, NAME, OPT, ...) \
-  case ID: return ragged_lines_fn<MODE, L, E, OPT, __VA_ARGS__>();
+  case ID: return ragged_lines_fn<MODE, L, E, lines_k<L, E, FPW>(), OPT, __VA_ARGS__>();
     FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LINES_CASE)
 #undef FFTFREE_RAGGED_LINES_CASE
     default: return nullptr;
@@ -156,7 +165,7 @@ bool ragged_cfg_flat(int id) { return ragged_cfg_lines(id) && kRaggedCfgs[id].l 
 bool ragged_lines_layout(int cfg, const StftArgs& a, size_t* lead, size_t* blocks, size_t* take) {
   switch (cfg) {
 #define FFTFREE_RAGGED_LAYOUT_CASE(ID, L, E, FPW, NAME, ...) \
-  case ID: return lines_layout<L, E, kLinesK>(a, lead, blocks, take);
+  case ID: return lines_layout<L, E, lines_k<L, E, FPW>()>(a, lead, blocks, take);
     FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LAYOUT_CASE)
 #undef FFTFREE_RAGGED_LAYOUT_CASE
     default: return false;

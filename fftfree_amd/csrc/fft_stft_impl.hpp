@@ -530,15 +530,17 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   constexpr bool kPow2Ring = ring_floats(RUN + 32) < 2 * 4096;
   constexpr int RING = kPow2Ring ? ring_floats(RUN + 32) : ((RUN + 32 + 3) / 4) * 4;
   constexpr bool kRegTabs = (OPT & 4) != 0;
-  // bit 8 (with bit 2): the window taps stay in LDS, only the twiddles go to registers
-  constexpr bool kRegWin = kRegTabs && (OPT & 256) == 0;
+  // bit 8 (with bit 2): the window taps stay in LDS, only the twiddles go to registers;
+  // bit 9 (without bit 2): the window taps go to registers, the twiddles stay in LDS
+  constexpr bool kRegWin = (kRegTabs && (OPT & 256) == 0) || (!kRegTabs && (OPT & 512) != 0);
   // OPT bit 3: the exchanges between passes in registers (DPP / permlane swaps); with bit
   // 4 only the first one (lane bits 3..5), the second through LDS
   constexpr bool kRegX = (OPT & 8) != 0;
   constexpr bool kLoLds = (OPT & 16) != 0;
-  static_assert(T <= 64 && (64 % T) == 0, "one transform per wave or less");
+  static_assert((T <= 64 && (64 % T) == 0) || T == 128, "a transform on one wave or less, or on two");
   static_assert(JF * T == L / 2, "T must divide L/2");
-  static_assert(RUN % 4 == 0, "iteration runs must be whole 16-byte words");
+  // store ranges end on line ends, except the tile's last, which ends at K * RUN
+  static_assert((K * RUN) % 4 == 0, "tiles must be whole 16-byte words");
   static_assert(product<Rs...>() == L, "radices must multiply to L");
   static_assert(!kRegX || (L == 512 && E == 8 && NP == 3 && R0 == 8 && Rl == 8 && SecondOf<Rs...>::value == 8),
                 "register exchanges: the 8x8x8 transform of one wave");
@@ -555,9 +557,9 @@ __device__ __forceinline__ void stft_lines_body(const StftArgs& a, int vblock, i
   __shared__ float2 tw1[kRegTabs ? 1 : NS1][R1 - 1];
 
   const int tid = threadIdx.x;
-  // with one transform per wave the slot is wave-uniform: a scalar, so that every
+  // with a transform on one wave or more the slot is wave-uniform: a scalar, so that every
   // slot-derived LDS and global address is scalar base + lane offset
-  const int slot = T == 64 ? __builtin_amdgcn_readfirstlane(tid / T) : tid / T;
+  const int slot = T >= 64 ? __builtin_amdgcn_readfirstlane(tid / T) : tid / T;
   const int t = tid & (T - 1);
   // the butterfly index of pass 1 (tb1) and pass 2 (tb2), which set their twiddles: t, or
   // with register exchanges the lane's digits rotated
@@ -1018,7 +1020,12 @@ size_t lines_slots() {
 template <int L, int E, int K, int WPC, int OPT, int DIR, int... Rs>
 hipError_t stft_lines_launch_impl(const StftArgs& a, hipStream_t s) {
   constexpr int P = 256 / (L / E);
-  if (a.bins != L + 1) return stft_mf_launch_impl<L, E, 6, 1, DIR, Rs...>(a, s);
+  if (a.bins != L + 1) {
+    if constexpr (L / E <= 64)
+      return stft_mf_launch_impl<L, E, 6, 1, DIR, Rs...>(a, s);
+    else
+      return stft_launch_impl<L, E, P, 1, DIR, Rs...>(a, s);
+  }
   // whole tiles, and a short remainder riding along in workgroup 0 of the same launch
   size_t nreg = 0, grid = 0, take = 0;
   lines_layout<L, E, K>(a, &nreg, &grid, &take);

@@ -312,7 +312,7 @@ def test_complex_batched_inverse_roundtrip(ff, kernel):
 
 
 @pytest.mark.parametrize("mode", [(1, 0, 1), (0, 0, 0), (0, 1, 1), (0, 0, 1)])
-@pytest.mark.parametrize("n", [1024, 2048])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
 def test_stft_every_variant_matches_oracle(ff, mode, n):
     """Every kernel variant the autotuner may pick (radix pattern, frames per
     workgroup, multi-frame) meets the oracle bound; variants with the same radix pattern
@@ -375,15 +375,17 @@ def test_stft_multiframe_remainders(ff, frames, half):
 
 
 @pytest.mark.parametrize("mode", [(1, 0), (0, 1), (0, 0)])
-def test_lines_2048_every_frame_vs_fp64(ff, mode):
-    """The N=2048 line-aligned variants (one wave per frame, 16 values per lane, the
-    split's output ring sized to fit rather than a power of two) on 2^22 samples (8189
-    frames: 255 whole tiles, some walked last frames first, and a rest block): every frame
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_lines_2048_every_frame_vs_fp64(ff, mode, n):
+    """The N=2048 and N=4096 line-aligned variants (16 values per lane: one wave per
+    frame at N=2048, two waves with workgroup barriers between the passes at N=4096; the
+    split's output ring sized to fit rather than a power of two) on 2^22 samples (8189 or
+    4093 frames: whole tiles, some walked last frames first, and a rest block): every frame
     of all three planes against the FP64 oracle, and the same bits in both variants
-    (window taps in registers or in LDS)."""
+    (window taps, or twiddles, in registers or in LDS)."""
     import torch
     reduce_mag, polar = mode
-    n, W, H = 2048, 2048, 512
+    W, H = n, n // 4
     pcm = noise_plus_sines((1 << 22) + 333, seed=31)
     p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, reduce_magnitude=bool(reduce_mag),
                      store_polar=bool(polar), half_spectrum=True, apply_windows=True, analysis_window=1, pad_mode=1)
