@@ -3,18 +3,20 @@
 //
 // Each request is an fft_execute_batched call of its own (frame rule fft_cffi.cpp:715-750,
 // framing 766-791, post hook eigen_fft.hpp:2535-2596, output copy fft_cffi.cpp:811-825),
-// described by a StftArgs block in device memory. The graph holds one persistent launch
-// per window size N in the batch, each compiled for that N alone (its own register
-// allocation, no shape dispatch). A launch runs one engine over the work items of its N:
-//  * the line-aligned engine (N = 256 .. 2048): the body of stft_r2c_lines_kernel,
-//    whose blocks are tiles of 32 or 64 frames whose plane regions are whole 128-byte
-//    lines, plus a block for the frames after the last tile;
-//  * the tile engine (every N): StftTile, FPW frames per tile with a radix pattern.
-// The host cuts every request into work items of `ntiles` blocks, sorts the items of each
-// N by cost and splits them into 8 contiguous, cost-balanced chunks, one per XCD: the
-// workgroups of XCD x (blockIdx % 8 == x, the dispatcher's round-robin order) stride
-// through chunk x, so the neighbouring tiles of one request, which share samples, are read
-// through the same L2.
+// described by a StftArgs block in device memory. The graph holds one launch per window
+// size N in the batch, each compiled for that N alone (its own register allocation, no
+// shape dispatch), the launches of different N on parallel branches. A launch runs one
+// engine over the work items of its N:
+//  * the line-aligned engine (N = 256 .. 4096): the body of stft_r2c_lines_kernel, whose
+//    blocks are tiles of 32 or 64 frames whose plane regions are whole 128-byte lines,
+//    plus a block for the frames after the last tile; one block per workgroup;
+//  * the tile engine (every N; requests the line-aligned one does not take): StftTile,
+//    FPW frames per tile with a radix pattern, persistent workgroups walking items of
+//    `ntiles` blocks.
+// The host cuts every request into work items, sorts the items of each N by cost and
+// splits them into 8 contiguous, cost-balanced chunks, one per XCD: the workgroups of XCD
+// x (blockIdx % 8 == x, the dispatcher's round-robin order) take chunk x in order, so the
+// neighbouring tiles of one request, which share samples, are read through the same L2.
 #include "fft_stft_impl.hpp"
 
 namespace fftfree {
@@ -81,11 +83,6 @@ __device__ __noinline__ void ragged_tile_block(const StftArgs* __restrict__ ap, 
   const StftArgs a = *ap;
   StftTile<L, E, FPW, 1, MODE, 1, -1, Rs...>::run(a, tile, lds, nullptr);
 }
-template <int MODE, int L, int E, int K, int OPT, int... Rs>
-__device__ __noinline__ void ragged_lines_block(const StftArgs* __restrict__ ap, int block) {
-  const StftArgs a = *ap;
-  stft_lines_body<L, E, K, OPT, MODE, -1, Rs...>(a, block, 0);
-}
 
 
 // Tile engine, one shape per kernel.
@@ -97,23 +94,18 @@ __global__ void __launch_bounds__(256) stft_ragged_tile_kernel(const StftArgs* _
   FFTFREE_RAGGED_ITEMS((ragged_tile_block<MODE, L, E, FPW, Rs...>(reqs + w.req, (size_t)w.tile + k, lds)))
 }
 
-// Line-aligned engine: block w.tile + k of request w.req (block 0 = the frames after the
-// last tile when the request has them; the host set each request's `lead`).
+// Line-aligned engine, one block per workgroup: the workgroups of XCD x take the items of
+// chunk x in order, one each (every item is one block, block 0 of a request being the
+// frames after its last tile when it has them; the host set each request's `lead` and
+// sizes the grid to 8 times the longest chunk). With no loop around the body its register
+// allocation is the standalone kernel's (117 registers at N = 1024; the 256 of two waves
+// per SIMD at N >= 2048), where a persistent loop over items needed 244 at N = 1024 and
+// spilled at N = 2048 (as a call or inlined). The graph runs the launches of different N
+// side by side, so each launch's last partial wave of workgroups overlaps other work.
 template <int MODE, int L, int E, int K, int OPT, int... Rs>
-__global__ void __launch_bounds__(256) stft_ragged_lines_kernel(const StftArgs* __restrict__ reqs,
-                                                                const RaggedItem* __restrict__ items,
-                                                                const unsigned* __restrict__ chunks) {
-  FFTFREE_RAGGED_ITEMS((ragged_lines_block<MODE, L, E, K, OPT, Rs...>(reqs + w.req, (int)(w.tile + k))))
-}
-// Line-aligned engine at N = 2048, one block per workgroup: the workgroups of XCD x take
-// the items of chunk x in order, one each (every item is one block; the host sizes the
-// grid to 8 times the longest chunk). A block of 16 values per lane needs the 256
-// registers of two waves per SIMD; as a call inside the item loop above, or inlined into
-// it, it needs ~290 plus scratch, so this kernel has no loop.
-template <int MODE, int L, int E, int K, int OPT, int... Rs>
-__global__ void __launch_bounds__(256, 2) stft_ragged_lines_flat_kernel(const StftArgs* __restrict__ reqs,
-                                                                        const RaggedItem* __restrict__ items,
-                                                                        const unsigned* __restrict__ chunks) {
+__global__ void __launch_bounds__(256, (lines_waves_per_eu<L, OPT>()))
+    stft_ragged_lines_kernel(const StftArgs* __restrict__ reqs, const RaggedItem* __restrict__ items,
+                             const unsigned* __restrict__ chunks) {
   const int x = blockIdx.x & 7;
   const unsigned i = chunks[x] + (blockIdx.x >> 3);
   if (i >= chunks[x + 1]) return;  // uniform over the workgroup
@@ -122,13 +114,6 @@ __global__ void __launch_bounds__(256, 2) stft_ragged_lines_flat_kernel(const St
   // a local copy would have its address taken and live in scratch
   const StftArgs& a = reqs[w.req];
   stft_lines_body<L, E, K, OPT, MODE, -1, Rs...>(a, (int)w.tile, 0);
-}
-template <int MODE, int L, int E, int K, int OPT, int... Rs>
-const void* ragged_lines_fn() {
-  if constexpr (L >= 1024)
-    return reinterpret_cast<const void*>(&stft_ragged_lines_flat_kernel<MODE, L, E, K, OPT, Rs...>);
-  else
-    return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, K, OPT, Rs...>);
 }
 
 template <int MODE>
@@ -139,7 +124,8 @@ const void* ragged_kernel_mode(int cfg) {
     FFTFREE_RAGGED_TILES(FFTFREE_RAGGED_TILE_CASE)
 #undef FFTFREE_RAGGED_TILE_CASE
 #define FFTFREE_RAGGED_LINES_CASE(ID, L, E, FPW, NAME, OPT, ...) \
-  case ID: return ragged_lines_fn<MODE, L, E, lines_k<L, E, FPW>(), OPT, __VA_ARGS__>();
+  case ID:                                                                  \
+    return reinterpret_cast<const void*>(&stft_ragged_lines_kernel<MODE, L, E, lines_k<L, E, FPW>(), OPT, __VA_ARGS__>);
     FFTFREE_RAGGED_LINES(FFTFREE_RAGGED_LINES_CASE)
 #undef FFTFREE_RAGGED_LINES_CASE
     default: return nullptr;
@@ -160,7 +146,7 @@ const void* ragged_kernel(int mode, int cfg) {
 int ragged_cfg_count() { return kRaggedCfgCount; }
 const VariantInfo* ragged_cfg(int id) { return id >= 0 && id < kRaggedCfgCount ? &kRaggedCfgs[id] : nullptr; }
 bool ragged_cfg_lines(int id) { return id >= kLinesFirst && id < kRaggedCfgCount; }
-bool ragged_cfg_flat(int id) { return ragged_cfg_lines(id) && kRaggedCfgs[id].l >= 1024; }
+bool ragged_cfg_flat(int id) { return ragged_cfg_lines(id); }
 
 bool ragged_lines_layout(int cfg, const StftArgs& a, size_t* lead, size_t* blocks, size_t* take) {
   switch (cfg) {

@@ -147,8 +147,8 @@ const StftEntry kLocal[] = {
     FFTFREE_STFT(5, 512, 16, 8, 1, "16x16x2 inplace", 16, 16, 2),
     FFTFREE_STFT(2, 1024, 16, 4, 1, "16x16x4 inplace", 16, 16, 4),
     FFTFREE_STFT(3, 1024, 8, 2, 1, "8x8x8x2 inplace", 8, 8, 8, 2),
-    FFTFREE_STFT(0, 2048, 16, 2, 1, "16x16x8 inplace", 16, 16, 8),
-    FFTFREE_STFT(1, 2048, 8, 1, 1, "8x8x8x4 inplace", 8, 8, 8, 4),
+    FFTFREE_STFT(1, 2048, 16, 2, 1, "16x16x8 inplace", 16, 16, 8),
+    FFTFREE_STFT(2, 2048, 8, 1, 1, "8x8x8x4 inplace", 8, 8, 8, 4),
     FFTFREE_STFT(0, 4096, 16, 1, 1, "16x16x16 inplace", 16, 16, 16),
 };
 

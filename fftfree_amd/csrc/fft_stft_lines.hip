@@ -29,12 +29,13 @@ const StftEntry kLines[] = {
     // Window taps in registers too: 130.2 us (256 VGPRs and a small spill).
     FFTFREE_STFT_LN(0, 1024, 16, 8, 0, 487, "16x16x4 lines ldswin wg=2 k=8", 16, 16, 4),
     FFTFREE_STFT_LN(4, 1024, 16, 8, 0, 231, "16x16x4 lines wg=2 k=8", 16, 16, 4),
-    // N = 4096: two waves per frame (16 values per lane, workgroup barriers between the
-    // passes), 2 frames per iteration, 16 iterations per 32-frame tile; window taps in
-    // registers, twiddles in LDS (beside the frame buffers and the ring), or both in
-    // registers
-    FFTFREE_STFT_LN(3, 2048, 16, 16, 0, 739, "16x16x8 lines regwin wg=2 k=16", 16, 16, 8),
-    FFTFREE_STFT_LN(4, 2048, 16, 16, 0, 231, "16x16x8 lines wg=2 k=16", 16, 16, 8),
+    // N = 4096 (the default there): two waves per frame (16 values per lane, workgroup
+    // barriers between the passes), 2 frames per iteration, 16 iterations per 32-frame
+    // tile; window taps in registers, twiddles in LDS (beside the frame buffers and the
+    // ring): 130.8 us polar at 16381 frames against 182.2 us for the tile kernel; both
+    // tables in registers: 132.4 us
+    FFTFREE_STFT_LN(0, 2048, 16, 16, 0, 739, "16x16x8 lines regwin wg=2 k=16", 16, 16, 8),
+    FFTFREE_STFT_LN(3, 2048, 16, 16, 0, 231, "16x16x8 lines wg=2 k=16", 16, 16, 8),
 };
 }  // namespace
 
