@@ -50,7 +50,15 @@ CASES = [(512, 128, 60, 1), (1024, 256, 50, 8), (1024, 256, 50, 1), (2048, 512, 
 @pytest.mark.parametrize("windows", [False, True])
 def test_gl_vs_oracle(ff, case, windows):
     """Fused path (inverse C2R half N in {512, 1024, 2048}, hop <= N) and general path
-    (N = 256), with and without Hann analysis/synthesis windows, against the oracle."""
+    (N = 256), with and without Hann analysis/synthesis windows, against the float
+    restatement of the reference and the FP64 Griffin-Lim from the same start angles.
+
+    Bound: 2 * tolerance(N) * sqrt(iterations), or twice the float restatement's own
+    distance from FP64 where that is larger: the phase normalisation X/|X| amplifies
+    rounding at bins of small |X|, so some inputs are ill-conditioned for any float32
+    implementation (N=2048, 4 iterations, Hann: the reference restatement is 1.6e-5 from
+    FP64 against a bound of 8.8e-6); the device result must then be no further from FP64
+    than twice the reference's own float error, and no further from the reference."""
     N, H, frames, iters = case
     print(f"griffin-lim N={N} H={H} frames={frames} iters={iters} windows={windows}", flush=True)
     w = np.concatenate([fp64.build_window(fp64.WINDOW_HANN, N)]) if windows else None
@@ -59,8 +67,14 @@ def test_gl_vs_oracle(ff, case, windows):
     re, im = ff.griffin_lim(fwd, inv, mag, hop=H, iterations=iters, seed=99)
     ok, rr, ri = ref.griffin_lim(mag, N, H, iterations=iters, seed=99, inv_synth=w, fwd_taps=w)
     assert ok == 1
+    X64 = fp64.griffin_lim64(mag, N, H, ref.gl_angles(99, mag.size).reshape(mag.shape), iters, synth=w, analysis=w)
+    ref_err = fp64.rel_l2(rr + 1j * ri, X64)
+    bound = max(2 * fp64.tolerance(N) * max(1.0, np.sqrt(iters)), 2 * ref_err)
     err = fp64.rel_l2(re + 1j * im, rr + 1j * ri)
-    assert err <= 2 * fp64.tolerance(N) * max(1.0, np.sqrt(iters)), err
+    err64 = fp64.rel_l2(re + 1j * im, X64)
+    print(f"  vs reference {err:.3e}, vs FP64 {err64:.3e}, reference vs FP64 {ref_err:.3e}, bound {bound:.3e}")
+    assert err <= bound, err
+    assert err64 <= bound, err64
 
 
 def test_gl_device_entry_matches_host_entry(ff):
