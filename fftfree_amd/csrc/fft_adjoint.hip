@@ -227,8 +227,9 @@ __global__ void __launch_bounds__(256) stft_signals_kernel(SignalArgs a) {
 }
 
 // Adjoint kernel. T = L/E threads transform one frame; a step transforms FPS = 256/T
-// consecutive frame positions, one per thread group, with workgroup barriers between
-// passes. Per step:
+// consecutive frame positions, one per thread group; the passes of a transform exchange
+// through LDS with wave fences when a frame is one wave (T = 64), else with workgroup
+// barriers. Per step:
 //  1. each group recomputes X for its frame, forms V, overwrites Z with conj(Y) in place
 //     (each thread owns the pair p, L-p), inverse-transforms and leaves r_f in its buffer;
 //  2. every sample the step's frames touch gets their windowed r_f added to an LDS
@@ -435,9 +436,15 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
       }
       const float gAm = ga ? ga[row + L / 2] : 0.0f;
       const float gBm = gb ? gb[row + L / 2] : 0.0f;
-      StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
-      store_natural<L, E, Rl>(v, t, buf);
-      __syncthreads();
+      if constexpr (T == 64) {  // the frame's own wave: wave-local exchanges
+        WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
+        store_natural<L, E, Rl>(v, t, buf);
+        wave_lds_fence();
+      } else {
+        StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+        store_natural<L, E, Rl>(v, t, buf);
+        __syncthreads();
+      }
 #pragma unroll
       for (int j = 0; j < JF; ++j) {
         const int p = t + j * T;
@@ -474,8 +481,13 @@ __global__ void __launch_bounds__(256) stft_adjoint_kernel(SignalArgs a) {
     for (int q = 0; q < E / R0; ++q)
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
-    __syncthreads();  // every value read before the first in-place scatter
-    StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+    if constexpr (T == 64) {  // the frame's own wave reads and scatters its buffer
+      wave_lds_fence();
+      WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
+    } else {
+      __syncthreads();  // every value read before the first in-place scatter
+      StockhamPlan<L, E, -1, Rs...>::template run<true>(v, t, buf, buf, twr, fb);
+    }
     // y = conj(result): r[2m] = v.x, r[2m+1] = -v.y
 #pragma unroll
     for (int q = 0; q < E / Rl; ++q)
@@ -685,7 +697,10 @@ hipError_t adjoint_launch(const SignalArgs& a, int span, int mode, hipStream_t s
 
 }  // namespace
 
-// Frames per adjoint step for each N (256 / threads per frame).
+// Frames per adjoint step for each N (256 / threads per frame). N = 2048 runs a frame on
+// two waves (8 values per lane, three workgroups per CU): one wave per frame (16 values
+// per lane, 253 registers, two workgroups per CU) took the saved-plane adjoint of 32 x 2^18
+// samples from 87-89 us to 94 us.
 int adjoint_fps(int n) { return n == 2048 ? 2 : 4; }
 
 namespace {
