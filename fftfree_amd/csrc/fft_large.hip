@@ -525,6 +525,10 @@ const LineEntry kLines[] = {
     FFTFREE_LINE(1024, 16, 8, "16x16x4", 16, 16, 4),
     FFTFREE_LINE(1024, 8, 8, "8x8x4x4", 8, 8, 4, 4),
     FFTFREE_LINE(1024, 4, 4, "4x4x4x4x4", 4, 4, 4, 4, 4),
+    // write-back row stores: WRITE_SIZE 1.00x the chunk instead of 1.30x (L2 merges the
+    // half lines of neighbouring workgroups), but 315.7 against 302.8 us per 2^20 x 16
+    // round trip
+    FFTFREE_LINE_T(1024, 16, 8, false, false, "16x16x4 lds-tw wb", 16, 16, 4),
     FFTFREE_LINE_T(2048, 16, 8, false, true, "16x16x8 lds-tw", 16, 16, 8),
     FFTFREE_LINE_T(2048, 16, 4, true, false, "16x16x8 lds-half-tw wb", 16, 16, 8),
     FFTFREE_LINE(2048, 16, 4, "16x16x8", 16, 16, 8),
