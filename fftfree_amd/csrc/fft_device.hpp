@@ -443,12 +443,12 @@ __device__ __forceinline__ void xlane_swap(float& a, float& b, int lane) {
   b = __builtin_bit_cast(float, nb);
 }
 
-// One stage of a lane-digit / register-index transpose: bit j of the register index
-// trades places with lane bit B.
-template <int B, int J>
+// One stage of a lane-digit / register-index transpose over v[0..E): bit j of the
+// register index trades places with lane bit B.
+template <int B, int J, int E = 8>
 __device__ __forceinline__ void xlane_stage8(float2* v, int lane) {
 #pragma unroll
-  for (int e = 0; e < 8; ++e)
+  for (int e = 0; e < E; ++e)
     if (!(e & (1 << J))) {
       xlane_swap<B>(v[e].x, v[e | (1 << J)].x, lane);
       xlane_swap<B>(v[e].y, v[e | (1 << J)].y, lane);
@@ -511,6 +511,59 @@ struct WaveRegs888 {
     hook(2);
   }
 };
+// The passes of a 1024-point transform held by one wave (E = 16 values per lane), radices
+// 16, 16, 4, with the exchanges in registers: the butterflies, twiddles and operation
+// order of WaveStockham<1024, 16, DIR, 1, 0, 16, 16, 4> on permuted lanes.
+//  * after pass 0 lane t register r holds element 16t + r; pass 1's butterfly b (elements
+//    b + 64r) runs on lane (b >> 4) | ((b & 15) << 2): transpose lane bits 2..5 with the
+//    register index (one permlane swap or two bank-masked DPP moves per pair);
+//  * after pass 1 lane l register r holds element (l >> 2) + 16r + 256(l & 3); pass 2's
+//    butterfly c (elements c + 256r, four per lane in registers 4q .. 4q+3) is
+//    c = (l >> 2) + 16(4q + (l & 3)): transpose lane bits 0..1 with register bits 0..1.
+// twr: pass 1's slots 16 + r - 1 for butterfly c1(lane), pass 2's 32 + 3q + r - 1 for
+// c2(lane, q) (regx1024_pass1 / regx1024_pass2). On return lane l register 4q + r holds
+// X[c2(l, q) + 256 r].
+__device__ __forceinline__ int regx1024_pass1(int lane) { return (lane >> 2) + 16 * (lane & 3); }
+__device__ __forceinline__ int regx1024_pass2(int lane, int q) { return (lane >> 2) + 16 * (4 * q + (lane & 3)); }
+template <int DIR>
+struct WaveRegs16x16x4 {
+  template <class Hook>
+  __device__ static __forceinline__ void run(float2* v, int lane, const float2* twr, Hook&& hook) {
+    reg_dft<16, DIR>(v);
+    hook(0);
+    xlane_stage8<2, 0, 16>(v, lane);
+    xlane_stage8<3, 1, 16>(v, lane);
+    xlane_stage8<4, 2, 16>(v, lane);
+    xlane_stage8<5, 3, 16>(v, lane);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = c_mul(v[r], twr[16 + r - 1]);
+    reg_dft<16, DIR>(v);
+    hook(1);
+    xlane_stage8<0, 0, 16>(v, lane);
+    xlane_stage8<1, 1, 16>(v, lane);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int r = 1; r < 4; ++r) v[4 * q + r] = c_mul(v[4 * q + r], twr[32 + 3 * q + r - 1]);
+      reg_dft<4, DIR>(v + 4 * q);
+    }
+    hook(2);
+  }
+  // the pass twiddles for this lane's butterflies; tw is exp(-2 pi i m / (1024 * stride))
+  __device__ static __forceinline__ void load_twiddles(float2* twr, int lane, const float2* __restrict__ tw,
+                                                       int stride) {
+    const int b1 = regx1024_pass1(lane) & 15;  // pass 1: span 16, twiddle step L/R/Ns = 4
+#pragma unroll
+    for (int r = 1; r < 16; ++r) twr[16 + r - 1] = load_tw<DIR>(tw, b1 * 4 * stride * r);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // pass 2: span 256, twiddle step 1
+      const int b2 = regx1024_pass2(lane, q);
+#pragma unroll
+      for (int r = 1; r < 4; ++r) twr[32 + 3 * q + r - 1] = load_tw<DIR>(tw, b2 * stride * r);
+    }
+  }
+};
+
 __device__ __forceinline__ void store_natural_rot(const float2* v, int lane, float2* dst) {
   const int c = rot_lane(lane);
 #pragma unroll

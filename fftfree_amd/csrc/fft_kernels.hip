@@ -121,6 +121,36 @@ __global__ void __launch_bounds__(FPW * 64) c2c_regx512_kernel(C2CArgs a) {
   }
 }
 
+// The 1024-point transform on one wave (16 values per lane, radices 16, 16, 4) with the
+// exchanges in registers (WaveRegs16x16x4): no LDS, no barriers, FPW transforms per
+// workgroup. Same butterflies and twiddles as c2c_kernel<1024, 16, ..., 16, 16, 4>.
+template <int FPW, int DIR>
+__global__ void __launch_bounds__(FPW * 64) c2c_regx1024_kernel(C2CArgs a) {
+  constexpr int L = 1024;
+  const int tr = threadIdx.x >> 6;  // wave-uniform: a wave is one transform
+  const int t = threadIdx.x & 63;
+  const size_t tile = (size_t)xcd_tile(blockIdx.x, gridDim.x);
+  const size_t f = tile * FPW + tr;
+  const bool active = f < a.batch;
+  const float2* __restrict__ col = a.in + (active ? f : 0) * (size_t)L;
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = active ? load_pre(col, t + 64 * r, L, a.pre_mode, a.pre_polar) : make_float2(0.0f, 0.0f);
+  float2 twr[48];
+  WaveRegs16x16x4<DIR>::load_twiddles(twr, t, a.tw, 1);
+  WaveRegs16x16x4<DIR>::run(v, t, twr, [](int) {});
+  if (active) {
+    float2* __restrict__ dst = a.out + f * (size_t)L;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int idx = regx1024_pass2(t, q) + 256 * r;
+        dst[idx] = post_op(v[4 * q + r], idx, L, a);
+      }
+  }
+}
+
 template <int DIR>
 __global__ void c2c_tiny_kernel(C2CArgs a) {
   const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -209,6 +239,14 @@ hipError_t c2c_regx512_launch(const C2CArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((c2c_regx512_kernel<FPW, DIR>), dim3((unsigned)grid), dim3(FPW * 64), 0, s, a);
   return hipGetLastError();
 }
+template <int FPW, int DIR>
+hipError_t c2c_regx1024_launch(const C2CArgs& a, hipStream_t s) {
+  const size_t grid = (a.batch + FPW - 1) / FPW;
+  if (grid == 0) return hipSuccess;
+  if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((c2c_regx1024_kernel<FPW, DIR>), dim3((unsigned)grid), dim3(FPW * 64), 0, s, a);
+  return hipGetLastError();
+}
 
 using C2CFn = hipError_t (*)(const C2CArgs&, hipStream_t);
 
@@ -234,6 +272,8 @@ const C2CEntry kC2C[] = {
     FFTFREE_C2C(512, 8, 4, "8x8x8", 8, 8, 8),
     FFTFREE_C2C(1024, 8, 2, "8x8x8x2", 8, 8, 8, 2),  // 13% faster at 65533 transforms, equal at 64
     FFTFREE_C2C(1024, 16, 4, "16x16x4", 16, 16, 4),
+    {1024, {1024, 16, 1, "16x16x4 dpp"}, &c2c_regx1024_launch<1, -1>, &c2c_regx1024_launch<1, 1>},
+    {1024, {1024, 16, 4, "16x16x4 dpp"}, &c2c_regx1024_launch<4, -1>, &c2c_regx1024_launch<4, 1>},
     FFTFREE_C2C(2048, 16, 2, "16x16x8", 16, 16, 8),
     FFTFREE_C2C(4096, 16, 1, "16x16x16", 16, 16, 16),
 };

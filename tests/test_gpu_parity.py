@@ -38,6 +38,34 @@ def test_c2c_all_lds_sizes(ff, n, inverse, kernel):
 
 
 @pytest.mark.parametrize("inverse", [0, 1])
+def test_c2c_1024_register_exchanges_same_bits(ff, inverse):
+    """The N=1024 C2C variants with one wave per transform and the exchanges between the
+    radix-16, 16, 4 passes in registers (1 and 4 transforms per workgroup) run the LDS
+    16x16x4 kernel's butterflies and twiddles on permuted lanes: the same bits, within
+    the FP64 bound, and with the magnitude post hook too."""
+    n = 1024
+    x = complex_normal(333, n, seed=91 + inverse)
+    for magnitude in (False, True):
+        p = ff.make_plan(n, inverse=inverse, kernel=2, transform=0, half_spectrum=False, reduce_magnitude=magnitude)
+        names = ff.variants(p)
+        lds = [i for i, m in enumerate(names) if m.startswith("16x16x4 fpw")]
+        regs = [i for i, m in enumerate(names) if m.startswith("16x16x4 dpp")]
+        assert len(lds) == 1 and len(regs) == 2, names
+        ys = []
+        for vi in lds + regs:
+            assert ff.set_variant(p, vi)
+            ys.append(ff.c2c(p, x))
+        ff.set_variant(p, -1)
+        for y in ys[1:]:
+            assert np.array_equal(y.view(np.uint32), ys[0].view(np.uint32))
+        want = fp64.fft64(x, bool(inverse))
+        if magnitude:
+            assert fp64.rel_l2(ys[-1].real, np.abs(want)) <= fp64.tolerance(n) and not ys[-1].imag.any()
+        else:
+            assert fp64.rel_l2(ys[-1], want) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("inverse", [0, 1])
 def test_c2c_512_register_exchanges_same_bits(ff, inverse):
     """The N=512 C2C variant whose exchanges between radix-8 passes are DPP / permlane
     swaps in registers runs the LDS kernel's butterflies and twiddles on permuted lanes:
