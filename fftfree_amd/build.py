@@ -17,8 +17,11 @@ LIB = PKG / "libfft_cffi.so"
 # bandwidth probes for bench.py's write_probe workload (measurement only, not the library)
 PROBE_SRC = "probe.hip"
 PROBE_LIB = PKG / "libfft_probe.so"
-SOURCES = ["fft_stft.hip", "fft_stft_lines.hip", "fft_stft_mf.hip", "fft_kernels.hip", "fft_large.hip", "fft_adjoint.hip", "fft_ragged.hip", "fft_istft.hip", "fft_phase.hip",
-           "ft_tools.hip", "plan.cpp", "fft_cffi.cpp", "ft_cffi.cpp"]
+# the slowest to compile first (the pool starts them first)
+SOURCES = ["fft_stft_lines.hip", "fft_stft_lines_exch.hip", "fft_stft_lines_order.hip",
+           "fft_stft_lines_2048.hip", "fft_stft_lines_4096.hip",
+           "fft_stft_mf.hip", "fft_stft.hip", "fft_ragged.hip", "fft_adjoint.hip", "fft_large.hip", "fft_kernels.hip",
+           "fft_istft.hip", "fft_phase.hip", "ft_tools.hip", "plan.cpp", "fft_cffi.cpp", "ft_cffi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]

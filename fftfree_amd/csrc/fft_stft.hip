@@ -162,11 +162,13 @@ const StftEntry kLocal[] = {
 const std::vector<StftEntry>& stft_table() {
   static const std::vector<StftEntry> tab = [] {
     std::vector<StftEntry> t(std::begin(kLocal), std::end(kLocal));
-    int c = 0;
-    const StftEntry* e = stft_lines_entries(&c);
-    t.insert(t.end(), e, e + c);
-    e = stft_mf_entries(&c);
-    t.insert(t.end(), e, e + c);
+    for (auto rows : {stft_lines_entries, stft_lines_exch_entries, stft_lines_order_entries, stft_lines_2048_entries,
+                      stft_lines_4096_entries,
+                      stft_mf_entries}) {
+      int c = 0;
+      const StftEntry* e = rows(&c);
+      t.insert(t.end(), e, e + c);
+    }
     std::stable_sort(t.begin(), t.end(), [](const StftEntry& x, const StftEntry& y) {
       return x.n != y.n ? x.n < y.n : x.order < y.order;
     });

@@ -51,6 +51,10 @@ size_t lds_cap_bytes(const void* k, int wpc);
 
 // The table rows each kernel translation unit contributes.
 const StftEntry* stft_lines_entries(int* count);
+const StftEntry* stft_lines_exch_entries(int* count);
+const StftEntry* stft_lines_order_entries(int* count);
+const StftEntry* stft_lines_2048_entries(int* count);
+const StftEntry* stft_lines_4096_entries(int* count);
 const StftEntry* stft_mf_entries(int* count);
 
 using namespace dev;
