@@ -452,10 +452,11 @@ def main():
             "metric": "phase inference frames/s (phase_infer_execute_device, linear phase), N=1024 hop 256",
             "value": frames * args.steps / gpu_s, "unit": "frames/s", "ms_per_step": per_launch * 1e3,
             "config": {"workload": "phase_infer_execute mode 0 (linear running phase, double), 65533 frames x "
-                                   "513 bins |X| -> (re, im); mode 1 (minimum phase: log, inverse C2C 1024, fold, "
-                                   "forward C2C, exp) timed alongside",
+                                   "513 bins |X| -> (re, im); mode 1 (minimum phase: log, inverse DFT 1024, fold, "
+                                   "forward DFT, exp, one launch, one wave per frame) timed alongside",
                        "n": n, "hop": H, "frames": frames, "mode1_ms_per_call": per1 * 1e3,
                        "mode1_frames_per_s": frames / per1,
+                       "mode1_frac_of_hbm": alg_bytes / per1 / 1e9 / HBM_PEAK_GBS,
                        "rel_l2_oracle": {"mode0": phase_err, "frames_checked": 4096,
                                          "oracle": "oracle/cpu_ref.cpp ref_phase_linear"}},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

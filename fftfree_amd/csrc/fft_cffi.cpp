@@ -1574,6 +1574,12 @@ size_t phase_execute(PhaseCtx* c, const float* d_mag, size_t frames, float* d_re
     return frames;
   }
   if (c->mode != 1 || !c->inv || !c->fwd) return 0;  // unknown mode, or N with no plan
+  if (phase_minimum_fused(N)) {  // every step in one launch, one wave per frame
+    ensure_gpu(c->fwd);
+    hip_check(launch_phase_minimum(d_mag, frames, bins, N, c->half, c->fwd->plan->W, d_re, d_im, s),
+              "launch_phase_minimum");
+    return frames;
+  }
   c->scratch.reserve(frames * (size_t)N * sizeof(float2));
   float2* cols = c->scratch.as<float2>();
   hip_check(launch_phase_log(d_mag, frames, bins, N, c->half, cols, s), "launch_phase_log");

@@ -138,6 +138,11 @@ hipError_t launch_gl_overlap(const float* frames_pcm, size_t frames, int n, int 
 // cepstrum fold, then (after the forward DFT) exp into the planes.
 hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n, int hop, float* o_re, float* o_im,
                                hipStream_t s);
+// Minimum phase in one launch, one wave per frame (N = 512, 1024: phase_minimum_fused);
+// tw is the N-point forward table.
+bool phase_minimum_fused(int n);
+hipError_t launch_phase_minimum(const float* mag, size_t frames, int bins, int n, int half, const float2* tw,
+                                float* o_re, float* o_im, hipStream_t s);
 hipError_t launch_phase_log(const float* mag, size_t frames, int bins, int n, int half, float2* cols, hipStream_t s);
 hipError_t launch_phase_fold(size_t frames, int n, float2* cols, hipStream_t s);
 hipError_t launch_phase_exp(const float2* cols, size_t frames, int bins, int n, float* o_re, float* o_im,

@@ -19,6 +19,7 @@
 #include "phase_sum.hpp"
 
 namespace fftfree {
+using namespace dev;
 
 namespace {
 
@@ -132,6 +133,80 @@ __global__ void __launch_bounds__(256) phase_exp_kernel(const float2* __restrict
   }
 }
 
+// Minimum phase, every step of a frame in one wave (N = 512 and 1024: one N-point
+// transform per wave, T = 64 lanes of E = N/64 values). The log spectrum x is real, so
+// its inverse DFT is conj(DFT(x)) / N exactly (conjugation commutes with every rounded
+// operation): both transforms are forward ones over one twiddle set. The first
+// transform leaves lane t holding indices t + 64j, j < E, in its last-pass order; the
+// second reads the same indices in its first-pass order, so the fold between them is a
+// register permutation. Samples per step as in phase_log_kernel, phase_fold_kernel and
+// phase_exp_kernel (fft_cffi.cpp:2095-2158), the transforms those of the C2C kernels.
+template <int L, int E, int... Rs>
+__global__ void __launch_bounds__(256) phase_minimum_kernel(const float* __restrict__ mag, size_t frames, int bins,
+                                                            int half, const float2* __restrict__ tw,
+                                                            float* __restrict__ o_re, float* __restrict__ o_im) {
+  constexpr int T = L / E;
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int NP = sizeof...(Rs);
+  static_assert(T == 64 && product<Rs...>() == L, "one transform per wave");
+  __shared__ float2 lds[4][padded_len<L>()];
+  const int wave = threadIdx.x >> 6;
+  const int t = threadIdx.x & 63;
+  const size_t f = (size_t)blockIdx.x * 4 + (size_t)wave;
+  if (f >= frames) return;  // a whole wave: the transforms below use wave-local exchanges only
+  float2* buf = &lds[wave][0];
+  const float* __restrict__ m = mag + f * (size_t)bins;
+  float2 twr[E * NP];
+  StockhamPlan<L, E, -1, Rs...>::load_twiddles(twr, t, tw, 1);
+  float2 v[E];
+#pragma unroll
+  for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int i = t + q * T + r * (L / R0);
+      const int k = half && i > L / 2 ? L - i : i;  // spec[N-k] = spec[k]
+      v[q * R0 + r] = make_float2(logf(fmaxf(1e-20f, m[k])), 0.0f);  // every index is set for even N
+    }
+  WaveStockham<L, E, -1, 1, 0, Rs...>::run(v, t, buf, twr, [](int) {});
+  // cepstrum c = conj(y) / N at i = t + 64 j, j = q + r * (E / Rl); causal fold; into the
+  // first-pass order of the next transform (index j = q0 + r0 * (E / R0))
+  float2 c[E];
+  constexpr float inv_n = 1.0f / (float)L;
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int j = q + r * (E / Rl);
+      const int i = t + 64 * j;
+      float2 z = make_float2(v[q * Rl + r].x * inv_n, -v[q * Rl + r].y * inv_n);
+      if (i >= 1 && i < (L + 1) / 2) {
+        z.x *= 2.0f;
+        z.y *= 2.0f;
+      } else if (i > L / 2) {
+        z = make_float2(0.0f, 0.0f);
+      }
+      c[(j % (E / R0)) * R0 + j / (E / R0)] = z;
+    }
+  WaveStockham<L, E, -1, 1, 0, Rs...>::run(c, t, buf, twr, [](int) {});
+  float* __restrict__ ore = o_re + f * (size_t)bins;
+  float* __restrict__ oim = o_im + f * (size_t)bins;
+#pragma unroll
+  for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+    for (int r = 0; r < Rl; ++r) {
+      const int k = t + q * T + r * (L / Rl);
+      if (k < bins) {
+        const float2 Lk = c[q * Rl + r];
+        const float a = expf(Lk.x);
+        float sn, cs;
+        sincosf(Lk.y, &sn, &cs);
+        ore[k] = a * cs;
+        oim[k] = a * sn;
+      }
+    }
+}
+
 unsigned grid_for(size_t work) {
   size_t g = (work + 255) / 256;
   return (unsigned)(g > 65536 ? 65536 : (g == 0 ? 1 : g));
@@ -147,6 +222,24 @@ hipError_t launch_phase_linear(const float* mag, size_t frames, int bins, int n,
   if ((work + 255) / 256 > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(phase_linear_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, mag, frames, bins, n,
                      hop, o_re, o_im);
+  return hipGetLastError();
+}
+
+bool phase_minimum_fused(int n) { return n == 512 || n == 1024; }
+
+hipError_t launch_phase_minimum(const float* mag, size_t frames, int bins, int n, int half, const float2* tw,
+                                float* o_re, float* o_im, hipStream_t s) {
+  if (frames == 0) return hipSuccess;
+  const size_t grid = (frames + 3) / 4;
+  if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+  if (n == 512)
+    hipLaunchKernelGGL((phase_minimum_kernel<512, 8, 8, 8, 8>), dim3((unsigned)grid), dim3(256), 0, s, mag, frames, bins,
+                       half, tw, o_re, o_im);
+  else if (n == 1024)
+    hipLaunchKernelGGL((phase_minimum_kernel<1024, 16, 16, 16, 4>), dim3((unsigned)grid), dim3(256), 0, s, mag, frames,
+                       bins, half, tw, o_re, o_im);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

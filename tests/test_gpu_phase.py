@@ -48,8 +48,12 @@ def test_phase_linear_vs_oracle(lib, case):
 
 
 @pytest.mark.parametrize("case", [(1024, True, 300), (256, True, 64), (512, False, 40), (8192, True, 12),
-                                  (1, True, 5), (2, True, 7), (1, False, 3)])
+                                  (1, True, 5), (2, True, 7), (1, False, 3), (1024, False, 33), (512, True, 61)])
 def test_phase_minimum_vs_oracle(lib, case):
+    """N = 512 and 1024 run the fused kernel (one wave per frame, both transforms and the
+    fold in registers and LDS; frame counts that leave the last workgroup partly empty),
+    the other N the log / C2C / fold / C2C / exp launches: both against the float
+    restatement of the reference."""
     N, half, F = case
     bins = N // 2 + 1 if half else N
     mag = (np.random.default_rng(N + 1).random((F, bins)).astype(np.float32) + 0.05) * 3
