@@ -377,14 +377,16 @@ def test_stft_every_variant_matches_oracle(ff, mode, n):
     ff.set_variant(p, -1)
 
 
-@pytest.mark.parametrize("frames", [1, 7, 8, 63, 64, 65, 127, 129, 200, 1000])
+@pytest.mark.parametrize("frames", [1, 7, 8, 31, 32, 33, 63, 64, 65, 127, 129, 200, 1000])
 @pytest.mark.parametrize("half", [1, 0])
-def test_stft_multiframe_remainders(ff, frames, half):
-    """Frame counts around the multi-frame kernel's tile size (the whole tiles and the
-    remainder share one launch) and a PCM shorter than one frame, through the device
-    entry with separately allocated (aligned) planes, against the FP64 oracle."""
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+def test_stft_multiframe_remainders(ff, frames, half, n):
+    """Frame counts around the default kernels' tile sizes (32-frame line-aligned tiles;
+    whole tiles and the remainder share one launch) and a PCM shorter than one frame,
+    through the device entry with separately allocated (aligned) planes, against the FP64
+    oracle."""
     import torch
-    n, W, H = 1024, 1024, 256
+    W, H = n, n // 4
     L = W + (frames - 1) * H + 17 if frames > 1 else W - 100
     pcm = noise_plus_sines(L, seed=frames)
     p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, half_spectrum=bool(half), apply_windows=True,
@@ -400,6 +402,35 @@ def test_stft_multiframe_remainders(ff, frames, half):
     ore, oim, omg = fp64.stft_planes(pcm, n, W, H, got, fp64.build_window(1, W), False, 1, False, False, bool(half))
     assert fp64.rel_l2(re + 1j * im, ore + 1j * oim) <= fp64.tolerance(n)
     assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n)
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+@pytest.mark.parametrize("case", ["short_window", "odd_hop", "pcm_offset", "plane_offset"])
+def test_stft_default_falls_back_where_lines_do_not_fit(ff, n, case):
+    """The line-aligned default needs W = N, an even hop, 8-byte aligned samples and
+    16-byte aligned planes; otherwise the launch hands its frames to the tile kernel.
+    Each case against the FP64 oracle, polar planes, 100+ frames."""
+    import torch
+    W = n - 300 if case == "short_window" else n
+    H = n // 4 + 1 if case == "odd_hop" else n // 4
+    L = W + 120 * H + 55
+    pcm = noise_plus_sines(L + 1, seed=n + len(case))
+    p = ff.make_plan(n, window=W, hop=H, kernel=2, transform=1, store_polar=True, half_spectrum=True,
+                     apply_windows=True, analysis_window=1, pad_mode=1)
+    frames = ff.frame_count(p, L, 1)
+    bins = n // 2 + 1
+    big = torch.from_numpy(pcm).cuda()
+    d = big[1:L + 1] if case == "pcm_offset" else big[:L]
+    off = 1 if case == "plane_offset" else 0
+    bufs = [torch.full((frames * bins + off,), float("nan"), device="cuda") for _ in range(3)]
+    planes = [b[off:].view(frames, bins) for b in bufs]
+    assert ff.stft_device(p, d, *planes, pad_mode=1) == frames
+    torch.cuda.synchronize()
+    re, im, mg = (x.cpu().numpy() for x in planes)
+    x = d.cpu().numpy()
+    ore, oim, omg = fp64.stft_planes(x, n, W, H, frames, fp64.build_window(1, W), False, 1, False, True, True)
+    assert spectra_close(re, im, ore, oim, True, fp64.tolerance(n)), case
+    assert fp64.rel_l2(mg, omg) <= fp64.tolerance(n), case
 
 
 @pytest.mark.parametrize("mode", [(1, 0), (0, 1), (0, 0)])
