@@ -1,10 +1,10 @@
 // fft_stft_impl.hpp -- the real-input STFT kernels and their launch templates, shared by
 // the translation units that instantiate them (fft_stft.hip: the variant table, the
-// generic and small-N kernels; fft_stft_lines.hip: the line-aligned N = 1024 kernels;
-// fft_stft_mf.hip: the multi-frame kernels). Kernels and launchers have internal
-// linkage: each translation unit owns the instantiations it names.
+// generic and small-N kernels; fft_stft_lines*.hip: the line-aligned kernels;
+// fft_stft_mf.hip: the multi-frame kernels; fft_ragged.hip: the ragged batch). Kernels and
+// launchers have internal linkage: each translation unit owns the instantiations it names.
 //
-// stft_r2c_lines_kernel  the default at N = 1024: fused framing + analysis window +
+// stft_r2c_lines_kernel  the default at N = 1024, 2048, 4096: fused framing + analysis window +
 //                        zero-pad (fft_cffi.cpp:766-791), the real-input DFT of size N
 //                        as a complex DFT of size N/2 plus a split step, the
 //                        post-transform hook (eigen_fft.hpp:2535-2596) and the

@@ -35,8 +35,10 @@ def run(lib, N, hop, half, mode, mag):
 
 
 @pytest.mark.parametrize("case", [(1024, 256, True, 3000), (512, 100, True, 777), (256, 256, False, 50),
-                                  (1000, 250, True, 20)])
+                                  (1000, 250, True, 20), (2048, 512, True, 100), (1024, 333, True, 64)])
 def test_phase_linear_vs_oracle(lib, case):
+    """Odd and even hops, block counts that leave the last 64-frame block partial, 257 to
+    1025 bins, against the reference's double running sum."""
     N, H, half, F = case
     bins = N // 2 + 1 if half else N
     mag = np.random.default_rng(N).random((F, bins)).astype(np.float32) * 10
