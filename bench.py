@@ -32,7 +32,9 @@ default workload's complex planes with the reference tool's default quantizer an
 contrast, no resampling: the one-pass kernel); copy
 (a plain 512 MiB device copy: the known-size kernel that calibrates the FETCH_SIZE and
 WRITE_SIZE counters); write_probe (the default workload's reads and plane writes with no
-transform, under several store forms and launch shapes: the floor memory sets).
+transform, under several store forms and launch shapes: the floor memory sets);
+large_probe (the same for the two passes of the 2^20-point C2C: their column and row
+access patterns with no arithmetic).
 """
 from __future__ import annotations
 
@@ -54,7 +56,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "phase", "ft_pipeline",
-                                                     "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe"])
+                                                     "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe",
+                                                     "large_probe"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
     ap.add_argument("--n", type=int, default=1024,
@@ -236,6 +239,63 @@ def write_probe(args, torch):
             "config": {"workload": "2047 tiles x (8960 samples read + 3 x 32 x 513 floats written)",
                        "best": best, "stft_algorithmic_bytes": alg,
                        "stft_floor_frac": alg / (best["us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, "probes": rows}}
+
+
+def large_probe(args, torch):
+    """The memory patterns of the 2^20-point two-pass transform with no arithmetic:
+    fft_probe_large moves 16 transforms (128 MiB in, 128 MiB out) as pass A (columns read
+    and written as runs of C values), pass B (rows read contiguously, written as runs of C)
+    or a plain copy, in launches of `per` transforms on one stream or alternating over two,
+    as the library runs its passes. The fastest pass-A and pass-B rows give the floor
+    memory alone sets for the round trip (two passes per transform, forward and inverse)."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "fftfree_amd", "libfft_probe.so"))
+    lib.fft_probe_large.restype = ctypes.c_int
+    lib.fft_probe_large.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    B, n = 16, 1 << 20
+    x = torch.zeros(B * n * 2, device="cuda")
+    y = torch.zeros(B * n * 2, device="cuda")
+    main_s = torch.cuda.current_stream()
+    aux = torch.cuda.Stream()
+    names = {0: "pass A (columns)", 1: "pass B (rows)", 2: "copy"}
+    cases = []
+    for mode in (0, 1, 2):
+        for c, nts in ((8, 0), (8, 1), (16, 0), (16, 1)):
+            for per, streams in ((4, 2), (4, 1), (16, 1)):
+                for wpc in (0, 2):
+                    cases.append((mode, c, nts, per, streams, wpc))
+    rows = []
+    for mode, c, nts, per, streams, wpc in cases:
+        def step():
+            aux.wait_stream(main_s)
+            for i in range(B // per):
+                s = aux if (streams == 2 and i % 2) else main_s
+                off = i * per * n * 8
+                rc = lib.fft_probe_large(x.data_ptr() + off, y.data_ptr() + off, per, mode, c, nts, wpc,
+                                         s.cuda_stream)
+                assert rc == 0, rc
+            main_s.wait_stream(aux)
+
+        gpu_s, _ = timed(step, args.steps, args.warmup, torch)
+        us = gpu_s / args.steps * 1e6
+        gbs = 2 * B * n * 8 / (us * 1e-6) / 1e9
+        rows.append({"pattern": names[mode], "lines": c, "streaming_stores": bool(nts), "transforms_per_launch": per,
+                     "streams": streams, "wpc": wpc, "us": us, "GBps": gbs})
+        print(f"large probe {names[mode]} C={c} nts={nts} per={per} streams={streams} wpc={wpc}: {us:.1f} us "
+              f"{gbs:.0f} GB/s", flush=True)
+    best = {m: max((r for r in rows if r["pattern"] == names[m]), key=lambda r: r["GBps"]) for m in (0, 1, 2)}
+    pass_bytes = 2 * B * n * 8
+    floor_us = 2 * (pass_bytes / (best[0]["GBps"] * 1e9) + pass_bytes / (best[1]["GBps"] * 1e9)) * 1e6
+    alg = 2 * pass_bytes
+    return {"metric": "HBM floor of the 2^20 x 16 two-pass C2C's memory patterns (probe, no arithmetic)",
+            "value": best[0]["GBps"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best[0]["us"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "zeros",
+            "config": {"workload": "16 transforms of 2^20 complex64 (1024 x 1024), one pass per step",
+                       "best_pass_a": best[0], "best_pass_b": best[1], "best_copy": best[2],
+                       "round_trip_floor_us": floor_us,
+                       "round_trip_floor_frac": alg / (floor_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "probes": rows}}
 
 
 def main():
@@ -762,6 +822,8 @@ def main():
         rb.close()
     elif args.workload == "write_probe":
         line = write_probe(args, torch)
+    elif args.workload == "large_probe":
+        line = large_probe(args, torch)
     elif args.workload == "copy":
         nbytes = 512 << 20
         x = torch.ones(nbytes // 4, device="cuda")

@@ -22,10 +22,13 @@
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 
+#include <algorithm>
 #include <cstdio>
 
 namespace fftfree {
 using namespace dev;
+
+int device_cus();
 
 namespace {
 
@@ -469,6 +472,199 @@ __global__ void __launch_bounds__(C*(L / E)) row_line_kernel(LargeArgs a) {
   }
 }
 
+// Persistent, software-pipelined passes (one workgroup per CU, one wave per line): the
+// workgroups of XCD x walk the contiguous run of tiles [lo_x, hi_x) with stride = the
+// XCD's workgroup count, so the tiles in flight on one XCD are neighbours (sharing HBM
+// lines through one L2). While the waves transform tile i, the global loads of the tile
+// after it are in flight into registers; the twiddle tables are filled once per
+// workgroup. Every global load the transform needs is part of that prefetch (the
+// inter-pass twiddle factors too), so no wait inside the transform waits for the next
+// tile's data. Measured on the 2^20 x 16 round trip (variant "pipelined"): 309-312 us
+// with one pass pipelined, 330 us with both, against 297 us for the one-tile
+// workgroups at two per CU, whose second workgroup (and the other stream's launch)
+// overlaps memory with arithmetic as well; the prefetch's registers leave room for
+// one 8-wave workgroup per CU only.
+__device__ __forceinline__ void xcd_range(int ntiles, int* lo, int* hi, int* step) {
+  const int x = blockIdx.x & 7;
+  const int q8 = ntiles >> 3, r8 = ntiles & 7;
+  *lo = x * q8 + (x < r8 ? x : r8);
+  *hi = *lo + q8 + (x < r8 ? 1 : 0);
+  *step = ((int)gridDim.x - x + 7) >> 3;  // workgroups dealt to XCD x
+  *lo += (int)(blockIdx.x >> 3);
+}
+
+// Workgroup barrier that orders LDS only: the pipelined passes exchange nothing through
+// global memory inside a launch, so the barrier need not wait for the stores of the last
+// tile or the loads of the next one (a plain __syncthreads() waits for both).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// s_waitcnt vmcnt(0) alone (gfx9 encoding: expcnt and lgkmcnt at their maxima). The
+// pipelined passes place it after the transform, before the tile's stores are issued:
+// the next tile's loads (issued before the transform) and the last tile's stores are
+// then complete, and nothing later waits on a counter that mixes loads with the stores
+// just issued (such a wait would be for all of them).
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ float2 readlane2(float2 v, int j) {
+  return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), j)),
+                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), j)));
+}
+
+// Pass A, pipelined: tile = C adjacent columns (length L, stride N2) of one transform.
+template <int L, int E, int C, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E), 1) col_pipe_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int LP = staged_stride<L>();
+  constexpr int TWN = line_tab_size<L, false, Rs...>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  static_assert(T == 64 && E <= 64, "one wave per line; the line factors of E rows fit a wave");
+  __shared__ float2 lds[C][LP];
+  __shared__ float2 tab[TWN > 0 ? TWN : 1];
+
+  const int tid = threadIdx.x;
+  const int groups = a.n2 / C;
+  int tile, hi, step;
+  xcd_range(groups * (int)a.batch, &tile, &hi, &step);
+  const size_t N2 = (size_t)a.n2;
+  const int col = tid % C, row0 = tid / C;
+  const int line = __builtin_amdgcn_readfirstlane(tid / T);
+  const int t = tid - line * T;
+  float2* buf = &lds[line][0];
+  const size_t mask = (size_t(1) << a.s) - 1;
+
+  float2 nx[E];
+  float2 th, tl, uh, ul;  // W_N^{n2 t} = th * tl; W_N^{n2 T j} = uh * ul for j = t mod E
+  auto fetch = [&](int tl_) {
+    const size_t b = (size_t)(tl_ / groups);
+    const size_t c0 = (size_t)(tl_ - (int)b * groups) * C;
+    const float2* __restrict__ src = a.in + b * a.n + c0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) nx[k] = src[(size_t)(row0 + k * (NT / C)) * N2 + col];
+    const size_t n2 = c0 + (size_t)line;
+    const size_t m1 = n2 * (size_t)t, m2 = n2 * (size_t)(T * (t & (E - 1)));
+    th = load_tw<DIR>(a.tw, (int)(m1 & ~mask));
+    tl = load_tw<DIR>(a.tw, (int)(m1 & mask));
+    uh = load_tw<DIR>(a.tw, (int)(m2 & ~mask));
+    ul = load_tw<DIR>(a.tw, (int)(m2 & mask));
+  };
+  LineTw<L, DIR, 1, 0, Rs...>::fill(tab, a.tw, (int)(a.n / L), tid, NT);
+  if (tile < hi) fetch(tile);
+  wait_vm();  // the loop's first pass then starts, as every later one, with no loads pending
+  for (; tile < hi; tile += step) {
+    const int b = tile / groups;
+    const size_t c0 = (size_t)(tile - b * groups) * C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) lds[col][pidx(row0 + k * (NT / C))] = pre1(nx[k], a.hooks.pre_mode);
+    const float2 wt = c_mul(th, tl);
+    const float2 wu = c_mul(uh, ul);
+    lds_barrier();  // the tile is in LDS (and, on the first pass, the twiddle tables)
+    if (tile + step < hi) fetch(tile + step);
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[q * R0 + r] = buf[pidx(t + q * T + r * (L / R0))];
+    wave_lds_fence();
+    LineStockham<L, E, DIR, false, 1, 0, Rs...>::run(v, t, buf, tab, wave_lds_fence);
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+      for (int r = 0; r < Rl; ++r) {
+        const int j = q + r * (E / Rl);  // k1 = t + T*j
+        buf[pidx(t + T * j)] = c_mul(v[q * Rl + r], c_mul(wt, readlane2(wu, j)));
+      }
+    wait_vm();
+    lds_barrier();
+    float2* __restrict__ dst = a.out + (size_t)b * a.n + c0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = row0 + k * (NT / C);
+      dst[(size_t)j * N2 + col] = lds[col][pidx(j)];
+    }
+    lds_barrier();  // every read of the tile is done before the next one is written
+  }
+}
+
+// Pass B, pipelined: tile = C adjacent rows (length L, contiguous) of one transform; the
+// output (row k1, k2) goes to k1 + N1*k2 through LDS as runs of C values, streaming stores.
+template <int L, int E, int C, int DIR, int... Rs>
+__global__ void __launch_bounds__(C*(L / E), 1) row_pipe_kernel(LargeArgs a) {
+  constexpr int T = L / E;
+  constexpr int NT = C * T;
+  constexpr int Rl = LastOf<Rs...>::value;
+  constexpr int R0 = FirstOf<Rs...>::value;
+  constexpr int LP = staged_stride<L>();
+  constexpr int TWN = line_tab_size<L, false, Rs...>();
+  static_assert(product<Rs...>() == L, "radices must multiply to L");
+  static_assert(T == 64, "one wave per line");
+  __shared__ float2 lds[C][LP];
+  __shared__ float2 tab[TWN > 0 ? TWN : 1];
+
+  const int tid = threadIdx.x;
+  const int groups = a.n1 / C;
+  int tile, hi, step;
+  xcd_range(groups * (int)a.batch, &tile, &hi, &step);
+  const size_t N1 = (size_t)a.n1;
+  const int line = __builtin_amdgcn_readfirstlane(tid / T);
+  const int t = tid - line * T;
+  float2* buf = &lds[line][0];
+
+  float2 nx[E];
+  auto fetch = [&](int tl_) {
+    const size_t b = (size_t)(tl_ / groups);
+    const size_t r0 = (size_t)(tl_ - (int)b * groups) * C;
+    const float2* __restrict__ row = a.in + b * a.n + (r0 + (size_t)line) * (size_t)L;
+#pragma unroll
+    for (int q = 0; q < E / R0; ++q)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) nx[q * R0 + r] = row[t + q * T + r * (L / R0)];
+  };
+  LineTw<L, DIR, 1, 0, Rs...>::fill(tab, a.tw, (int)(a.n / L), tid, NT);
+  if (tile < hi) fetch(tile);
+  wait_vm();
+  lds_barrier();  // twiddle tables
+  for (; tile < hi; tile += step) {
+    const int b = tile / groups;
+    const size_t r0 = (size_t)(tile - b * groups) * C;
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = nx[k];
+    if (tile + step < hi) fetch(tile + step);
+    LineStockham<L, E, DIR, false, 1, 0, Rs...>::run(v, t, buf, tab, wave_lds_fence);
+    const size_t k1 = r0 + (size_t)line;
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < E / Rl; ++q)
+#pragma unroll
+      for (int r = 0; r < Rl; ++r) {
+        const int k2 = t + q * T + r * (L / Rl);
+        buf[pidx(k2)] = post_large(v[q * Rl + r], k1 + N1 * (size_t)k2, a.n, a.hooks);
+      }
+    wait_vm();
+    lds_barrier();
+    float2* __restrict__ dst = a.out + (size_t)b * a.n + r0;
+    const int col = tid % C;
+    const int row0 = tid / C;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int k2 = row0 + k * (NT / C);
+      const float2 z = lds[col][pidx(k2)];
+      float* o = reinterpret_cast<float*>(dst + (size_t)k2 * N1 + col);
+      __builtin_nontemporal_store(z.x, o);
+      __builtin_nontemporal_store(z.y, o + 1);
+    }
+    lds_barrier();  // every read of the tile is done before the next one's exchanges
+  }
+}
+
 using LineFn = hipError_t (*)(const LargeArgs&, int lines, hipStream_t);
 
 template <int L, int E, int C, int DIR, int... Rs>
@@ -501,6 +697,24 @@ hipError_t row_line_launch(const LargeArgs& a, int lines, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Persistent launches: one workgroup per CU (at most one per tile).
+template <int L, int E, int C, int DIR, int... Rs>
+hipError_t col_pipe_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  const size_t tiles = (size_t)(lines / C) * a.batch;
+  const unsigned grid = (unsigned)std::min<size_t>(tiles, (size_t)device_cus());
+  hipLaunchKernelGGL((col_pipe_kernel<L, E, C, DIR, Rs...>), dim3(grid), dim3(C * (L / E)), 0, s, a);
+  return hipGetLastError();
+}
+template <int L, int E, int C, int DIR, int... Rs>
+hipError_t row_pipe_launch(const LargeArgs& a, int lines, hipStream_t s) {
+  if (lines % C || (size_t)(lines / C) * a.batch > 0x7fffffffull) return hipErrorInvalidValue;
+  const size_t tiles = (size_t)(lines / C) * a.batch;
+  const unsigned grid = (unsigned)std::min<size_t>(tiles, (size_t)device_cus());
+  hipLaunchKernelGGL((row_pipe_kernel<L, E, C, DIR, Rs...>), dim3(grid), dim3(C * (L / E)), 0, s, a);
+  return hipGetLastError();
+}
+
 struct LineEntry {
   int l, e, c;
   const char* radices;
@@ -514,6 +728,9 @@ struct LineEntry {
 #define FFTFREE_LINE_T(L, E, C, HT, NTS, NAME, ...)                                                  \
   {L, E, C, NAME, &col_line_launch<L, E, C, HT, -1, __VA_ARGS__>, &col_line_launch<L, E, C, HT, 1, __VA_ARGS__>, \
    &row_line_launch<L, E, C, HT, NTS, -1, __VA_ARGS__>, &row_line_launch<L, E, C, HT, NTS, 1, __VA_ARGS__>}
+#define FFTFREE_LINE_P(L, E, C, NAME, ...)                                                          \
+  {L, E, C, NAME, &col_pipe_launch<L, E, C, -1, __VA_ARGS__>, &col_pipe_launch<L, E, C, 1, __VA_ARGS__>,    \
+   &row_pipe_launch<L, E, C, -1, __VA_ARGS__>, &row_pipe_launch<L, E, C, 1, __VA_ARGS__>}
 const LineEntry kLines[] = {
     FFTFREE_LINE(64, 8, 32, "8x8", 8, 8),
     FFTFREE_LINE(128, 16, 32, "16x8", 16, 8),
@@ -529,6 +746,7 @@ const LineEntry kLines[] = {
     // half lines of neighbouring workgroups), but 315.7 against 302.8 us per 2^20 x 16
     // round trip
     FFTFREE_LINE_T(1024, 16, 8, false, false, "16x16x4 lds-tw wb", 16, 16, 4),
+    FFTFREE_LINE_P(1024, 16, 8, "16x16x4 pipelined", 16, 16, 4),
     FFTFREE_LINE_T(2048, 16, 8, false, true, "16x16x8 lds-tw", 16, 16, 8),
     FFTFREE_LINE_T(2048, 16, 4, true, false, "16x16x8 lds-half-tw wb", 16, 16, 8),
     FFTFREE_LINE(2048, 16, 4, "16x16x8", 16, 16, 8),
@@ -537,6 +755,7 @@ const LineEntry kLines[] = {
 };
 #undef FFTFREE_LINE
 #undef FFTFREE_LINE_T
+#undef FFTFREE_LINE_P
 
 int line_count(int l) {
   int c = 0;

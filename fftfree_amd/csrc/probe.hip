@@ -2,8 +2,9 @@
 // library proper does not use them). fft_probe_planes moves exactly the bytes the
 // line-aligned STFT kernel moves -- each tile's span of samples read, each tile's region
 // of the three planes written as whole 16-byte words -- with no transform in between, so
-// its time is the floor that memory alone sets for that pattern. Built into its own
-// library, libfft_probe.so, loaded by bench.py's write_probe workload.
+// its time is the floor that memory alone sets for that pattern; fft_probe_large does the
+// same for the two passes of the 2^20-point transform. Built into its own library,
+// libfft_probe.so, loaded by bench.py's write_probe and large_probe workloads.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
@@ -105,9 +106,96 @@ __global__ void __launch_bounds__(256) probe_planes_kernel(ProbeArgs a) {
   (void)pad;
 }
 
+// The access patterns of the two-pass large transform's passes at N = 1024 x 1024, with no
+// transform: a workgroup of 512 threads moves a tile of C lines x 1024 values (float2).
+//   MODE 0 (pass A): the C adjacent columns read (runs of C values, stride 1024) and
+//                    written back in place in `out` (the same runs);
+//   MODE 1 (pass B): C rows read contiguously, written as runs of C values, stride 1024;
+//   MODE 2:          the tile read and written contiguously (a copy).
+// Tile t covers transform t / (1024 / C) of the launch; every index stays inside the
+// launch's `batch` transforms of 2^20 values.
+constexpr int kLargeSide = 1024;
+constexpr size_t kLargeN = (size_t)kLargeSide * kLargeSide;
+
+template <int C, int MODE, bool NTS>
+__global__ void __launch_bounds__(512) probe_large_kernel(const float2* __restrict__ in, float2* __restrict__ out) {
+  extern __shared__ float pad[];  // only there to cap the workgroups per CU
+  constexpr int NT = 512;
+  constexpr int E = kLargeSide * C / NT;
+  constexpr int groups = kLargeSide / C;
+  const int tid = threadIdx.x;
+  const size_t b = blockIdx.x / groups;
+  const size_t c0 = (size_t)(blockIdx.x % groups) * C;
+  const int col = tid % C, row0 = tid / C;
+  float2 x[E];
+  if (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = in[b * kLargeN + (size_t)(row0 + k * (NT / C)) * kLargeSide + c0 + col];
+  } else {
+    const float2* __restrict__ src = in + b * kLargeN + c0 * kLargeSide;
+#pragma unroll
+    for (int k = 0; k < E; ++k) x[k] = src[tid + k * NT];
+  }
+  float2* __restrict__ dst = out + b * kLargeN;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const size_t idx = MODE == 2 ? c0 * kLargeSide + tid + k * NT : (size_t)(row0 + k * (NT / C)) * kLargeSide + c0 + col;
+    if (NTS) {
+      float* o = reinterpret_cast<float*>(dst + idx);
+      __builtin_nontemporal_store(x[k].x, o);
+      __builtin_nontemporal_store(x[k].y, o + 1);
+    } else {
+      dst[idx] = x[k];
+    }
+  }
+  (void)pad;
+}
+
+template <int C, int MODE, bool NTS>
+int launch_large_probe(const float2* in, float2* out, int batch, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((probe_large_kernel<C, MODE, NTS>), dim3((unsigned)(batch * (kLargeSide / C))), dim3(512), lds, s,
+                     in, out);
+  return (int)hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" {
+
+// One launch of the large-transform pattern probe over `batch` transforms of 2^20 float2
+// (in and out each hold batch * 2^20 values): mode 0 (pass A), 1 (pass B) or 2 (copy),
+// c = 8 or 16 lines per workgroup, nts = streaming stores, wpc = at most that many
+// workgroups per CU (0: no cap). Returns the launch status.
+int fft_probe_large(const void* in, void* out, int batch, int mode, int c, int nts, int wpc, void* stream) {
+  if (batch <= 0 || mode < 0 || mode > 2 || (c != 8 && c != 16)) return (int)hipErrorInvalidValue;
+  size_t lds = 0;
+  if (wpc > 0) {
+    int dev = 0, cu_lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+        cu_lds > 0)
+      lds = (size_t)cu_lds / (size_t)(wpc + 1) + 512;
+  }
+  const float2* i = static_cast<const float2*>(in);
+  float2* o = static_cast<float2*>(out);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int key = mode * 4 + (c == 16 ? 2 : 0) + (nts ? 1 : 0);
+  switch (key) {
+    case 0: return launch_large_probe<8, 0, false>(i, o, batch, lds, s);
+    case 1: return launch_large_probe<8, 0, true>(i, o, batch, lds, s);
+    case 2: return launch_large_probe<16, 0, false>(i, o, batch, lds, s);
+    case 3: return launch_large_probe<16, 0, true>(i, o, batch, lds, s);
+    case 4: return launch_large_probe<8, 1, false>(i, o, batch, lds, s);
+    case 5: return launch_large_probe<8, 1, true>(i, o, batch, lds, s);
+    case 6: return launch_large_probe<16, 1, false>(i, o, batch, lds, s);
+    case 7: return launch_large_probe<16, 1, true>(i, o, batch, lds, s);
+    case 8: return launch_large_probe<8, 2, false>(i, o, batch, lds, s);
+    case 9: return launch_large_probe<8, 2, true>(i, o, batch, lds, s);
+    case 10: return launch_large_probe<16, 2, false>(i, o, batch, lds, s);
+    default: return launch_large_probe<16, 2, true>(i, o, batch, lds, s);
+  }
+}
+
 
 // Runs the probe over `tiles` tiles on `grid` workgroups of 256 threads (tile t on
 // workgroup t mod grid), at most `wpc` workgroups per CU (0: no cap). The caller

@@ -124,6 +124,39 @@ def test_c2c_large_c2r_half_rebuild(ff, polar):
     assert fp64.rel_l2(y, sig) <= 2 * fp64.tolerance(n)
 
 
+@pytest.mark.parametrize("lg", [20, 21])
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_c2c_large_every_variant(ff, lg, inverse):
+    """Every (column pass, row pass) kernel pair of the two-pass transform meets the FP64
+    bound; the pipelined persistent passes do the arithmetic of the one-tile-per-workgroup
+    LDS-table passes in the same order, so their pairs give the same bits as those."""
+    n = 1 << lg
+    batch = 5  # tiles of 8 lines: 5 * 128 or 5 * 256 of them, not a multiple of the CUs
+    p = ff.make_plan(n, inverse=inverse, kernel=2, transform=0, half_spectrum=False)
+    names = ff.variants(p)
+    x = complex_normal(batch, n, seed=lg + 7 * inverse)
+    want = fp64.fft64(x, bool(inverse))
+    outs = {}
+    for v, name in enumerate(names):
+        print(f"large variant n={n} inverse={inverse} {v}: {name}", flush=True)
+        assert ff.set_variant(p, v)
+        outs[name] = ff.c2c(p, x)
+        assert fp64.rel_l2(outs[name], want) <= fp64.tolerance(n), name
+    cols = sorted({nm.split(" | ")[0] for nm in names})
+    rows = sorted({nm.split(" | ")[1] for nm in names})
+    pcol = [c for c in cols if "pipelined" in c]
+    prow = [r for r in rows if "pipelined" in r]
+    assert pcol or prow
+    plain_col = [c for c in cols if c.endswith("lds-tw e=16 lines=8") and "pipelined" not in c]
+    plain_row = [r for r in rows if r.endswith("lds-tw e=16 lines=8") and "pipelined" not in r]
+    for c in pcol:
+        for r in plain_row:
+            assert np.array_equal(outs[f"{c} | {r}"], outs[f"{plain_col[0]} | {r}"])
+    for r in prow:
+        for c in plain_col:
+            assert np.array_equal(outs[f"{c} | {r}"], outs[f"{c} | {plain_row[0]}"])
+
+
 @pytest.mark.parametrize("lg", [20, 22])
 def test_large_c2c_batch16_roundtrip(ff, lg):
     """2^20 and 2^22 points, batch 16, complex normal input: every forward transform meets
