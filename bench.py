@@ -535,7 +535,8 @@ def main():
                     break
             line["cpu_baseline"] = dict({"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
                                          "sample": f"{done} frames (4096-frame slices of the same |X|) through the "
-                                                   f"restated mode-0 loop, 1 thread, {el:.1f} s"}, **host_info())
+                                                   f"restated mode-0 loop, 1 thread (the reference's phase_infer.hpp loop runs on "
+                                                   f"one thread: no worker pool), {el:.1f} s"}, **host_info())
         for h in handles.values():
             lib.phase_free(h)
     elif args.workload == "ft_pipeline":
@@ -587,7 +588,8 @@ def main():
                     break
             line["cpu_baseline"] = dict({"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
                                          "sample": f"{done} frames (2048 x 513 slices of the same grid) through the "
-                                                   f"restated pipeline, 1 thread, {el:.1f} s"}, **host_info())
+                                                   f"restated pipeline, 1 thread (the reference's ft_tools.hpp pipeline runs on one "
+                                                   f"thread: no worker pool), {el:.1f} s"}, **host_info())
     elif args.workload == "c2c_large":
         n, B = 1 << args.lg, 16
         fwd = ff.make_plan(n, kernel=2, radix=4, transform=0, half_spectrum=False)
