@@ -34,7 +34,8 @@ contrast, no resampling: the one-pass kernel); copy
 WRITE_SIZE counters); write_probe (the default workload's reads and plane writes with no
 transform, under several store forms and launch shapes: the floor memory sets);
 large_probe (the same for the two passes of the 2^20-point C2C: their column and row
-access patterns with no arithmetic).
+access patterns with no arithmetic); stream (fft_stream_push_pcm / push_spectrum latency
+per push and throughput, host buffers).
 """
 from __future__ import annotations
 
@@ -57,7 +58,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="stft", choices=["stft", "stft_host", "istft", "griffin_lim", "phase", "ft_pipeline",
                                                      "c2c1024", "c2c_large", "stft_grad", "ragged", "copy", "write_probe",
-                                                     "large_probe"])
+                                                     "large_probe", "stream"])
     ap.add_argument("--lg", type=int, default=22, help="log2 N of the c2c_large workload")
     ap.add_argument("--batch", type=int, default=64, help="transforms per launch of the c2c1024 workload")
     ap.add_argument("--n", type=int, default=1024,
@@ -296,6 +297,75 @@ def large_probe(args, torch):
                        "best_pass_a": best[0], "best_pass_b": best[1], "best_copy": best[2],
                        "round_trip_floor_us": floor_us,
                        "round_trip_floor_frac": alg / (floor_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "probes": rows}}
+
+
+def stream_bench(args, ff, np):
+    """The streaming entries on the headline geometry (N = 1024, Hann 1024, hop 256): a
+    forward context's fft_stream_push_pcm fed chunks of C samples (each push returns the
+    C / 256 frames it completes: |X| plus re and im planes, host buffers), and an inverse
+    context's fft_stream_push_spectrum fed F frames per push (unit-weight COLA overlap-add,
+    256 samples out per frame). Every push is a synchronous host call (upload, kernel,
+    download), so the wall clock per push is the latency a streaming caller sees."""
+    import ctypes
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    n, W, H = 1024, 1024, 256
+    bins = n // 2 + 1
+    lib = ff.lib
+    fwd = ff.make_plan(n, window=W, hop=H, kernel=2, radix=4, transform=1, reduce_magnitude=True,
+                       half_spectrum=True, apply_windows=True, analysis_window=1)
+    inv = ff.make_plan(n, window=W, hop=H, inverse=1, kernel=2, transform=2, half_spectrum=True, apply_windows=True,
+                       analysis_window=1, synthesis_window=1, cola=1)
+    rng = np.random.default_rng(5)
+    rows = []
+    for chunk in (256, 4096, 65536, 1 << 20):
+        lib.fft_stream_reset(fwd.handle)
+        x = (0.1 * rng.standard_normal(chunk)).astype(np.float32)
+        cap = chunk // H + 4
+        planes = [np.empty(cap * bins, np.float32) for _ in range(3)]
+        pushes = max(8, min(2000, (1 << 23) // chunk))
+        for _ in range(4 + W // chunk):  # fill the buffer past the first frame
+            lib.fft_stream_push_pcm(fwd.handle, fp(x), chunk, fp(planes[0]), fp(planes[1]), fp(planes[2]), 0, 0)
+        frames = 0
+        t0 = time.perf_counter()
+        for _ in range(pushes):
+            frames += lib.fft_stream_push_pcm(fwd.handle, fp(x), chunk, fp(planes[0]), fp(planes[1]), fp(planes[2]),
+                                              0, 0)
+        el = time.perf_counter() - t0
+        moved = pushes * chunk * 4 + frames * 3 * bins * 4
+        rows.append({"entry": "fft_stream_push_pcm", "samples_per_push": chunk, "frames_per_push": frames / pushes,
+                     "us_per_push": el / pushes * 1e6, "frames_per_s": frames / el, "host_GBps": moved / el / 1e9})
+        print(f"stream pcm chunk={chunk}: {el / pushes * 1e6:.1f} us/push {frames / el:.3e} frames/s", flush=True)
+    for fpp in (1, 16, 256, 4096):
+        lib.fft_stream_reset(inv.handle)
+        re = (rng.standard_normal(fpp * bins)).astype(np.float32)
+        im = (rng.standard_normal(fpp * bins)).astype(np.float32)
+        out = np.empty(fpp * H + n, np.float32)
+        pushes = max(8, min(2000, 32768 // fpp))
+        for _ in range(4):
+            lib.fft_stream_push_spectrum(inv.handle, fp(re), fp(im), fpp, fp(out), 0, 0)
+        samples = 0
+        t0 = time.perf_counter()
+        for _ in range(pushes):
+            samples += lib.fft_stream_push_spectrum(inv.handle, fp(re), fp(im), fpp, fp(out), 0, 0)
+        el = time.perf_counter() - t0
+        moved = pushes * fpp * 2 * bins * 4 + samples * 4
+        rows.append({"entry": "fft_stream_push_spectrum", "frames_per_push": fpp, "samples_per_push": samples / pushes,
+                     "us_per_push": el / pushes * 1e6, "frames_per_s": pushes * fpp / el, "host_GBps": moved / el / 1e9})
+        print(f"stream spectrum frames={fpp}: {el / pushes * 1e6:.1f} us/push {pushes * fpp / el:.3e} frames/s",
+              flush=True)
+    best = max((r for r in rows if r["entry"] == "fft_stream_push_pcm"), key=lambda r: r["frames_per_s"])
+    one = next(r for r in rows if r["entry"] == "fft_stream_push_pcm" and r["samples_per_push"] == H)
+    return {"metric": "streaming STFT push latency and throughput (host buffers), N=1024 hop 256",
+            "value": best["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": None, "warmup": None,
+            "ms_per_step": one["us_per_push"] / 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "0.1 * N(0,1) samples; N(0,1) spectra, seed 5",
+            "config": {"workload": "fft_stream_push_pcm chunks of 256 .. 2^20 samples (magnitude + re/im planes); "
+                                   "fft_stream_push_spectrum 1 .. 4096 frames (COLA overlap-add)",
+                       "ms_per_step_is": "one push of one hop (256 samples -> 1 frame)", "pushes": rows},
+            "roofline": {"bound": "pcie / launch latency", "achieved": None, "peak": None, "unit": None,
+                         "frac": None, "traffic": None,
+                         "note": "every push copies its samples in and its planes out over PCIe and waits for them; "
+                                 "small pushes are bound by that round trip, large ones by the copies"}}
 
 
 def main():
@@ -826,6 +896,8 @@ def main():
         line = write_probe(args, torch)
     elif args.workload == "large_probe":
         line = large_probe(args, torch)
+    elif args.workload == "stream":
+        line = stream_bench(args, ff, np)
     elif args.workload == "copy":
         nbytes = 512 << 20
         x = torch.ones(nbytes // 4, device="cuda")

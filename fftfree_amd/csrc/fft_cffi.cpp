@@ -414,6 +414,34 @@ size_t stft_host_pipelined(FftContext* c, const float* pcm, size_t pcm_len, size
   const size_t W = (size_t)(c->window ? c->window : c->N);
   const size_t H = (size_t)(c->hop ? c->hop : (int)W);
   const size_t plane_bytes = bins * sizeof(float);
+  // Small calls (at most 2 MiB of planes: a streaming push of a few hops, say): one
+  // upload, the launch over all frames (the pipeline's single chunk) and one download on
+  // the context's stream, the host copies on this thread. No copy threads and no events
+  // between streams, which cost more than such a call's copies.
+  if (frames * 3 * plane_bytes <= (size_t(2) << 20)) {
+    const size_t need = std::min(pcm_len, (frames - 1) * H + W);
+    const size_t stride = (frames * bins + 63) / 64 * 64;
+    c->d_pcm.reserve(pcm_len * sizeof(float));
+    c->d_planes.reserve(3 * stride * sizeof(float));
+    c->h_in.reserve(need * sizeof(float));
+    c->h_out.reserve(3 * stride * sizeof(float));
+    std::memcpy(c->h_in.as<float>(), pcm, need * sizeof(float));
+    hip_check(hipMemcpyAsync(c->d_pcm.as<float>(), c->h_in.as<float>(), need * sizeof(float), hipMemcpyHostToDevice,
+                             c->stream),
+              "H2D pcm");
+    float* d = c->d_planes.as<float>();
+    run_stft_frames(c, stft_args(c, c->d_pcm.as<float>(), pcm_len), 0, frames, d, d + stride, d + 2 * stride,
+                    c->stream);
+    hip_check(hipMemcpyAsync(c->h_out.as<float>(), d, 3 * stride * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+              "D2H planes");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    const float* h = c->h_out.as<float>();
+    std::memcpy(o_re, h, frames * plane_bytes);
+    std::memcpy(o_im, h + stride, frames * plane_bytes);
+    std::memcpy(o_mag, h + 2 * stride, frames * plane_bytes);
+    *produced = frames;
+    return frames;
+  }
   // A multiple of every kernel variant's tile (4 to 32 frames), so chunk edges fall on
   // tile edges of the single launch over all frames and the planes are bit-identical.
   constexpr size_t kTileMultiple = 192;
@@ -1662,8 +1690,10 @@ size_t phase_infer_execute(void* handle, const float* in_mag, size_t frames, flo
 void phase_free(void* handle) { delete static_cast<fftfree::PhaseCtx*>(handle); }
 
 // ---- streaming (fft_cffi.cpp:1342-1742) -----------------------------------------------
-// The sliding buffers live on the host, as the reference keeps them; every transform
-// runs through the batched entry points above (one launch per call).
+// The sliding buffers and the queue live on the host, as the reference keeps them. A PCM
+// push runs through fft_execute_batched (one launch; small pushes take its single-stream
+// path); a spectrum push runs the inverse transform and the overlap-add onto the carried
+// accumulator on the device and brings back only the overlap-added samples.
 namespace fftfree {
 namespace {
 
@@ -1802,33 +1832,74 @@ size_t fft_stream_push_spectrum(void* handle, const float* in_real, const float*
       c->stream_norm.clear();
     }
     if (frames > 0) {
-      if (!in_real) return fftfree::stream_emit(c, out_pcm, max_samples);
-      std::vector<float> fr(frames * n, 0.0f);
-      size_t produced = in_imag ? fft_execute_complex_batched(handle, in_real, in_imag, frames, fr.data(), 2, 0, frames)
-                                : 0;  // the reference's complex entry refuses a NULL imag plane
-      if (produced == 0) return fftfree::stream_emit(c, out_pcm, max_samples);
-      if (produced > frames) produced = frames;
-      const size_t hop = (size_t)std::max(fftfree::stream_hop(c), 1);
-      for (size_t f = 0; f < produced; ++f) {
-        const float* x = fr.data() + f * n;
-        for (size_t i = 0; i < n; ++i) {
-          c->stream_accum[i] += x[i];
-          if (use_norm) c->stream_norm[i] += fftfree::stream_cola_weight(c, i);
-        }
-        size_t emit = (flush_mode == FFT_STREAM_FLUSH_FINAL && f + 1 == produced) ? n : hop;
-        if (emit > n) emit = n;
-        for (size_t i = 0; i < emit; ++i) {
-          float v = c->stream_accum[i];
-          if (use_norm && std::fabs(c->stream_norm[i]) > 1e-12f) v /= c->stream_norm[i];
-          c->stream_queue.push_back(v);
-        }
-        // shift out the emitted samples (all of them when emit == n)
-        std::move(c->stream_accum.begin() + (std::ptrdiff_t)emit, c->stream_accum.end(), c->stream_accum.begin());
-        std::fill(c->stream_accum.end() - (std::ptrdiff_t)emit, c->stream_accum.end(), 0.0f);
-        if (use_norm) {
-          std::move(c->stream_norm.begin() + (std::ptrdiff_t)emit, c->stream_norm.end(), c->stream_norm.begin());
-          std::fill(c->stream_norm.end() - (std::ptrdiff_t)emit, c->stream_norm.end(), 0.0f);
-        }
+      // the reference's complex entry refuses a NULL plane: nothing is added
+      if (!in_real || !in_imag) return fftfree::stream_emit(c, out_pcm, max_samples);
+      // Per frame the reference adds the frame's N samples to the accumulator, queues the
+      // first e = min(hop, N) of them (all N after the last frame of a final flush) and
+      // shifts the accumulator by that much. On the device: the frames' inverse transform,
+      // then the overlap-add of frame f at offset f*e onto the carried accumulator (the
+      // same additions in the same order); the host divides by the COLA sums, queues the
+      // first (F-1)*e + (e or N) samples and carries the rest.
+      fftfree::ensure_gpu(c);
+      const size_t bins = fftfree::bins_of(c);
+      const size_t e = std::min((size_t)std::max(fftfree::stream_hop(c), 1), n);
+      const size_t len = (frames - 1) * e + n;
+      auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+      const size_t in_bytes = frames * bins * sizeof(float), n_bytes = n * sizeof(float);
+      // device staging: what goes up (planes, carried accumulator and norm, weights), the
+      // frames, then what comes down (overlap-add and norm sums), each run contiguous
+      const size_t o_im = up(in_bytes), o_carry = o_im + up(in_bytes), o_cnorm = o_carry + up(n_bytes),
+                   o_w = o_cnorm + up(n_bytes), o_fr = o_w + up(n_bytes),
+                   o_ext = o_fr + up(frames * n * sizeof(float)), o_norm = o_ext + up(len * sizeof(float)),
+                   o_end = use_norm ? o_norm + up(len * sizeof(float)) : o_norm;
+      c->d_planes.reserve(o_end);  // (complex_batched's own scratch is d_cplx)
+      char* base = c->d_planes.as<char>();
+      auto at = [&](size_t off) { return reinterpret_cast<float*>(base + off); };
+      hipStream_t st = c->stream;
+      // small pushes go up in one copy from pinned memory (a copy's fixed cost is most of
+      // their time); large ones copy the planes straight from the caller
+      const bool small = o_fr <= (size_t(1) << 20);
+      const size_t h0 = small ? 0 : o_carry;
+      c->h_in.reserve(o_fr - h0);
+      auto hin = [&](size_t off) { return c->h_in.as<char>() + (off - h0); };  // by device offset
+      if (small) {
+        std::memcpy(hin(0), in_real, in_bytes);
+        std::memcpy(hin(o_im), in_imag, in_bytes);
+      }
+      std::memcpy(hin(o_carry), c->stream_accum.data(), n_bytes);
+      if (use_norm) {
+        std::memcpy(hin(o_cnorm), c->stream_norm.data(), n_bytes);
+        float* w = reinterpret_cast<float*>(hin(o_w));
+        for (size_t i = 0; i < n; ++i) w[i] = fftfree::stream_cola_weight(c, i);
+      }
+      fftfree::hip_check(hipMemcpyAsync(base + h0, hin(h0), o_fr - h0, hipMemcpyHostToDevice, st), "H2D stream");
+      if (!small) {
+        fftfree::hip_check(hipMemcpyAsync(at(0), in_real, in_bytes, hipMemcpyHostToDevice, st), "H2D re");
+        fftfree::hip_check(hipMemcpyAsync(at(o_im), in_imag, in_bytes, hipMemcpyHostToDevice, st), "H2D im");
+      }
+      fftfree::complex_batched(c, at(0), at(o_im), frames, at(o_fr), st);
+      fftfree::hip_check(fftfree::launch_stream_ola(at(o_fr), frames, (int)n, (int)e, at(o_carry), at(o_cnorm),
+                                                    use_norm ? at(o_w) : nullptr, at(o_ext), at(o_norm), len, st),
+                         "launch_stream_ola");
+      c->h_out.reserve(o_end - o_ext);
+      fftfree::hip_check(hipMemcpyAsync(c->h_out.ptr, base + o_ext, o_end - o_ext, hipMemcpyDeviceToHost, st),
+                         "D2H overlap-add");
+      fftfree::hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      const float* ext = c->h_out.as<float>();
+      const float* en = use_norm ? c->h_out.as<float>() + (o_norm - o_ext) / sizeof(float) : nullptr;
+      const size_t out_n = (frames - 1) * e + (flush_mode == FFT_STREAM_FLUSH_FINAL ? n : e);
+      const size_t q0 = c->stream_queue.size();
+      c->stream_queue.resize(q0 + out_n);
+      float* q = c->stream_queue.data() + q0;
+      for (size_t t = 0; t < out_n; ++t) {
+        float v = ext[t];
+        if (use_norm && std::fabs(en[t]) > 1e-12f) v /= en[t];
+        q[t] = v;
+      }
+      for (size_t i = 0; i < n; ++i) {  // what is left: len - out_n <= n - e samples
+        const size_t t = out_n + i;
+        c->stream_accum[i] = t < len ? ext[t] : 0.0f;
+        if (use_norm) c->stream_norm[i] = t < len ? en[t] : 0.0f;
       }
     }
     if (flush_mode == FFT_STREAM_FLUSH_FINAL && frames == 0 && !c->stream_accum.empty()) {

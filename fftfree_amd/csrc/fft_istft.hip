@@ -538,6 +538,33 @@ __global__ void __launch_bounds__(256) gl_overlap_kernel(const float* __restrict
   }
 }
 
+// The streaming inverse's overlap-add (fft_cffi.cpp:1602-1742). The reference adds each
+// frame into an N-sample accumulator and then shifts it by e samples (e = min(hop, N));
+// laid out flat, frame f lands at offset f*e of a buffer that starts with the accumulator
+// carried in, so sample t is carry[t] (t < n) plus the frames covering it in ascending
+// order, the same additions in the same order. With weights (COLA normalisation) the
+// norm buffer gets the carried norm plus w[t - f*e] the same way.
+__global__ void __launch_bounds__(256) stream_ola_kernel(const float* __restrict__ fr, size_t frames, int n, int e,
+                                                         const float* __restrict__ carry,
+                                                         const float* __restrict__ carry_norm,
+                                                         const float* __restrict__ w, float* __restrict__ out,
+                                                         float* __restrict__ out_norm, size_t len) {
+  const size_t N = (size_t)n, E = (size_t)e;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < len; t += (size_t)gridDim.x * 256) {
+    const size_t flo = t >= N ? (t - N) / E + 1 : 0;  // first frame with f*e + n > t
+    size_t fhi = t / E;                               // last frame with f*e <= t
+    if (fhi > frames - 1) fhi = frames - 1;
+    float v = t < N ? carry[t] : 0.0f;
+    float m = (w && t < N) ? carry_norm[t] : 0.0f;
+    for (size_t f = flo; f <= fhi; ++f) {
+      v += fr[f * N + (t - f * E)];
+      if (w) m += w[t - f * E];
+    }
+    out[t] = v;
+    if (w) out_norm[t] = m;
+  }
+}
+
 template <int L, int E, int MODE, int ACCM, bool GLM, int... Rs>
 hipError_t istft_launch_mode(IstftArgs a, hipStream_t s) {
   const void* kern = reinterpret_cast<const void*>(&istft_kernel<L, E, MODE, ACCM, GLM, Rs...>);
@@ -645,6 +672,18 @@ hipError_t launch_gl_update(const float* mag, const float* t_re, const float* t_
   if (grid > 65536) grid = 65536;
   hipLaunchKernelGGL(gl_update_kernel, dim3((unsigned)grid), dim3(256), 0, s, mag, t_re, t_im, count, bins, half,
                      n_even, o_re, o_im);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_ola(const float* frames_pcm, size_t frames, int n, int e, const float* carry,
+                             const float* carry_norm, const float* w, float* out, float* out_norm, size_t len,
+                             hipStream_t s) {
+  if (len == 0 || frames == 0) return hipSuccess;
+  if (n <= 0 || e <= 0 || e > n || len != (frames - 1) * (size_t)e + (size_t)n) return hipErrorInvalidValue;
+  size_t grid = (len + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(stream_ola_kernel, dim3((unsigned)grid), dim3(256), 0, s, frames_pcm, frames, n, e, carry,
+                     carry_norm, w, out, out_norm, len);
   return hipGetLastError();
 }
 

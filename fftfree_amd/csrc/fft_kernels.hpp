@@ -129,6 +129,12 @@ hipError_t launch_gl_update(const float* mag, const float* t_re, const float* t_
                             int half, int n_even, float* o_re, float* o_im, hipStream_t s);
 // Overlap-add of frames*N frame-major samples with hop H into out[0, ola_len), divided by
 // the number of frames per sample.
+// The streaming inverse's overlap-add of `frames` frames of n samples at offsets f*e
+// (e <= n) onto the n carried samples (and, when w is given, the COLA weight sums onto
+// the carried norm): len = (frames - 1)*e + n samples out.
+hipError_t launch_stream_ola(const float* frames_pcm, size_t frames, int n, int e, const float* carry,
+                             const float* carry_norm, const float* w, float* out, float* out_norm, size_t len,
+                             hipStream_t s);
 hipError_t launch_gl_overlap(const float* frames_pcm, size_t frames, int n, int hop, float* out, size_t ola_len,
                              hipStream_t s);
 

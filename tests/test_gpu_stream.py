@@ -144,3 +144,41 @@ def test_stream_inverse_matches_batched_overlap_add(ff):
     m = norm > 1e-3 * norm.max()
     assert got.size == L + H and not got[L:].any()
     assert fp64.rel_l2(got[:L][m], batch[:L][m]) <= 2 * fp64.tolerance(N)
+
+
+@pytest.mark.parametrize("hop", [256, 384, 1024])
+def test_stream_spectrum_reference_additions(ff, hop):
+    """The spectrum push's overlap-add is the reference's accumulator loop
+    (fft_cffi.cpp:1602-1742) bit for bit: per frame, add the frame's N samples to the
+    accumulator, queue the first min(hop, N) (all N after the last frame of a final
+    flush), shift. Checked on the library's own frames (fft_execute_complex_batched of
+    the same context), with the loop run in float32 here; pushes of 0 .. 16 frames, the
+    accumulator carried across them."""
+    N = 1024
+    p = ff.make_plan(N, window=N, hop=hop, inverse=1, kernel=2, transform=C2R, half_spectrum=True, apply_windows=True,
+                     analysis_window=fp64.WINDOW_HANN, synthesis_window=fp64.WINDOW_HANN, cola=0)
+    lib = ff.lib
+    lib.fft_stream_reset(p.handle)
+    bins = N // 2 + 1
+    e = min(hop, N)
+    rng = np.random.default_rng(hop)
+    acc = np.zeros(N, np.float32)
+    for call, k in enumerate([1, 3, 0, 16, 5, 2, 7]):
+        final = call == 6
+        re = rng.standard_normal((k, bins)).astype(np.float32)
+        im = rng.standard_normal((k, bins)).astype(np.float32)
+        want = []
+        if k:
+            fr = np.empty(k * N, np.float32)
+            assert lib.fft_execute_complex_batched(p.handle, fp(re), fp(im), k, fp(fr), 2, 0, k) == k
+            for f in range(k):
+                acc += fr[f * N:(f + 1) * N]
+                emit = N if (final and f == k - 1) else e
+                want.append(acc[:emit].copy())
+                acc = np.concatenate([acc[emit:], np.zeros(emit, np.float32)])
+        want = np.concatenate(want) if want else np.zeros(0, np.float32)
+        out = np.full(k * N + N, np.nan, np.float32)
+        got = lib.fft_stream_push_spectrum(p.handle, fp(re) if k else None, fp(im) if k else None, k, fp(out), 0,
+                                           FLUSH_FINAL if final else FLUSH_NONE)
+        assert got == want.size, call
+        assert np.array_equal(out[:got], want), call
