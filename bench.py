@@ -530,7 +530,9 @@ def main():
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                 "data": "synthetic: |STFT| of 0.1*U[-1,1] noise + 8 sinusoids, seed 2",
                 "config": {"workload": "Griffin-Lim, inverse C2R + forward R2C half spectrum, no windows, pad 1, "
-                                       f"{iters} iterations per call (ms_per_step), start included",
+                                       f"{iters} iterations per call (ms_per_step), start included (the mt19937 start "
+                                       "words are drawn on the host by a context's first call for a seed and "
+                                       "frame count, about 0.7 ms, and reused from the device after that)",
                            "n": n, "hop": H, "frames": frames, "iterations": iters,
                            "ms_per_iteration": per_iter * 1e3, "ms_one_iteration_call": t_one / steps * 1e3,
                            "rel_l2_fp64": None,

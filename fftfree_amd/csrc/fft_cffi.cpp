@@ -106,6 +106,9 @@ struct FftContext {
   DeviceBuffer d_halo;  // adjoint run-boundary partial sums
   DeviceBuffer d_gl;    // Griffin-Lim working set (on the inverse context)
   uint32_t* h_words = nullptr;       // Griffin-Lim start words (pinned host memory)
+  // the start words now in d_gl: their seed and count (0: none); they depend on nothing else
+  unsigned gl_words_seed = 0;
+  size_t gl_words_count = 0;
   size_t h_words_cap = 0;
   hipEvent_t words_copied = nullptr;  // the last copy out of h_words
   bool windows_dirty = true;
@@ -1386,7 +1389,9 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t o_words = 0, o_ola = up(total * 4), o_tre = o_ola + up(ola_len * 4), o_tim = o_tre + up(total * 4),
                o_fr = o_tim + up(total * 4), bytes = o_fr + (fused ? 0 : up(frames * (size_t)N * 4));
+  const size_t had = iv->d_gl.bytes;
   iv->d_gl.reserve(bytes);
+  if (iv->d_gl.bytes != had) iv->gl_words_count = 0;  // a new allocation holds no words
   char* base = iv->d_gl.as<char>();
   uint32_t* d_words = reinterpret_cast<uint32_t*>(base + o_words);
   float* d_ola = reinterpret_cast<float*>(base + o_ola);
@@ -1395,8 +1400,12 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
   float* d_fr = reinterpret_cast<float*>(base + o_fr);
 
   // the start: mt19937 is sequential, so its words come from the host (phase_infer.hpp:124-134),
-  // into a pinned buffer of the context, copied in chunks while the next chunk is drawn
-  {
+  // into a pinned buffer of the context, copied in chunks while the next chunk is drawn. The
+  // words depend only on the seed and their count: a call that repeats both on this context
+  // finds them on the device already.
+  const unsigned seed_used = seed ? seed : 0x47d7bfadu;
+  if (iv->gl_words_count != total || iv->gl_words_seed != seed_used) {
+    iv->gl_words_count = 0;
     if (iv->words_copied) hip_check(hipEventSynchronize(iv->words_copied), "wait for the last start copy");
     else hip_check(hipEventCreateWithFlags(&iv->words_copied, hipEventDisableTiming), "hipEventCreate");
     if (iv->h_words_cap < total) {
@@ -1407,7 +1416,7 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
                 "hipHostMalloc");
       iv->h_words_cap = total;
     }
-    Mt19937Block rng(seed ? seed : 0x47d7bfadu);  // the values of std::mt19937
+    Mt19937Block rng(seed_used);  // the values of std::mt19937
     constexpr size_t kChunk = size_t(1) << 18;
     for (size_t b = 0; b < total; b += kChunk) {
       const size_t e = std::min(total, b + kChunk);
@@ -1415,8 +1424,13 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
       hip_check(hipMemcpyAsync(d_words + b, iv->h_words + b, (e - b) * 4, hipMemcpyHostToDevice, s), "H2D words");
     }
     hip_check(hipEventRecord(iv->words_copied, s), "hipEventRecord");
-    hip_check(launch_gl_start(d_mag, d_words, total, d_re, d_im, s), "launch_gl_start");
+    iv->gl_words_seed = seed_used;
+    iv->gl_words_count = total;
+  } else {
+    // the copy that put them there may have been enqueued on another stream
+    hip_check(hipStreamWaitEvent(s, iv->words_copied, 0), "hipStreamWaitEvent(start words)");
   }
+  hip_check(launch_gl_start(d_mag, d_words, total, d_re, d_im, s), "launch_gl_start");
   const StftArgs fa = stft_args(fw, d_ola, ola_len);
   for (int it = 0; it < iters; ++it) {
     if (fused) {

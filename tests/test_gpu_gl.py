@@ -89,6 +89,27 @@ def test_gl_device_entry_matches_host_entry(ff):
     assert np.array_equal(dr.cpu().numpy(), re) and np.array_equal(di.cpu().numpy(), im)
 
 
+def test_gl_repeated_calls_reuse_the_start_words(ff):
+    """The start words depend only on the seed and their count, so a context keeps them on
+    the device between calls: calls that repeat a seed give the same bits as the first,
+    whatever ran in between (another seed, another frame count, the host entry)."""
+    N, H = 1024, 256
+    mag = spectrogram(N, H, 60 * H + N, seed=6)
+    short = np.ascontiguousarray(mag[:37])
+    fwd, inv = plans(ff, N, H)
+    first = ff.griffin_lim(fwd, inv, mag, hop=H, iterations=3, seed=11)
+    again = ff.griffin_lim(fwd, inv, mag, hop=H, iterations=3, seed=11)
+    other = ff.griffin_lim(fwd, inv, mag, hop=H, iterations=3, seed=12)
+    cut = ff.griffin_lim(fwd, inv, short, hop=H, iterations=3, seed=11)
+    third = ff.griffin_lim(fwd, inv, mag, hop=H, iterations=3, seed=11)
+    fresh_f, fresh_i = plans(ff, N, H)
+    fresh = ff.griffin_lim(fresh_f, fresh_i, short, hop=H, iterations=3, seed=11)
+    for x in (again, third):
+        assert np.array_equal(x[0], first[0]) and np.array_equal(x[1], first[1])
+    assert not np.array_equal(other[0], first[0])
+    assert np.array_equal(cut[0], fresh[0]) and np.array_equal(cut[1], fresh[1])
+
+
 def test_gl_refusals(ff):
     """The reference's argument checks (fft_cffi.cpp:2187-2212): NULL or aliased buffers,
     a half_spectrum flag that differs from the contexts; plus contexts of different N."""
