@@ -63,7 +63,10 @@ __device__ __forceinline__ float2 gl_dir(float m, float x, float y) {
     if (!__builtin_signbit(x)) return make_float2(m, __builtin_copysignf(0.0f, y) * m);
     return make_float2(-m, (__builtin_signbit(y) ? 8.742278e-8f : -8.742278e-8f) * m);
   }
-  const float inv = 1.0f / hypotf(x, y);
+  // 1/|(x, y)|: the hardware reciprocal square root where x^2 + y^2 stays a normal float
+  // (about 1 ulp, as 1/hypotf), else the scaled hypotf
+  const float s2 = __builtin_fmaf(x, x, y * y);
+  const float inv = (s2 > 1e-30f && s2 < 1e30f) ? __builtin_amdgcn_rsqf(s2) : 1.0f / hypotf(x, y);
   return make_float2(m * (x * inv), m * (y * inv));
 }
 
