@@ -344,8 +344,9 @@ StftArgs stft_args(FftContext* c, const float* d_pcm, size_t pcm_len) {
 }
 
 // Frames [f0, f0+nf) of the STFT into planes (already offset to frame f0).
+// `variant`: the kernel variant when the context has none set (-1: the default choice).
 void run_stft_frames(FftContext* c, const StftArgs& base, size_t f0, size_t nf, float* re, float* im, float* mag,
-                     hipStream_t s) {
+                     hipStream_t s, int variant = -1) {
   StftArgs a = base;
   a.frames = nf;
   a.frame_offset = f0;
@@ -353,7 +354,8 @@ void run_stft_frames(FftContext* c, const StftArgs& base, size_t f0, size_t nf, 
   a.o_im = im;
   a.o_mag = mag;
   if (fused_ok(c)) {
-    hip_check(launch_stft(a, c->inverse ? 1 : 0, c->plan->stft_variant, s), "launch_stft");
+    hip_check(launch_stft(a, c->inverse ? 1 : 0, c->plan->stft_variant >= 0 ? c->plan->stft_variant : variant, s),
+              "launch_stft");
     return;
   }
   // generic path: frame -> plan kernel (hooks) -> planes, in chunks of <= 256 MiB
@@ -1432,6 +1434,10 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
   }
   hip_check(launch_gl_start(d_mag, d_words, total, d_re, d_im, s), "launch_gl_start");
   const StftArgs fa = stft_args(fw, d_ola, ola_len);
+  // the forward planes are read back by the next iteration's inverse straight away
+  StftArgs fq = fa;
+  fq.frames = uf;
+  const int fwd_variant = readback_stft_variant(fq);
   for (int it = 0; it < iters; ++it) {
     if (fused) {
       IstftArgs a;
@@ -1460,7 +1466,7 @@ bool griffin_lim(FftContext* fw, FftContext* iv, const float* d_mag, size_t fram
       complex_batched(iv, d_re, d_im, frames, d_fr, s);
       hip_check(launch_gl_overlap(d_fr, frames, N, hop, d_ola, ola_len, s), "launch_gl_overlap");
     }
-    run_stft_frames(fw, fa, 0, uf, d_tre, d_tim, nullptr, s);
+    run_stft_frames(fw, fa, 0, uf, d_tre, d_tim, nullptr, s, fwd_variant);
   }
   hip_check(launch_gl_update(d_mag, d_tre, d_tim, uf, (int)bins, half, (N & 1) == 0, d_re, d_im, s),
             "launch_gl_update");

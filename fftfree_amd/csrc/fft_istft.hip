@@ -218,7 +218,12 @@ __global__ void __launch_bounds__(256) istft_kernel(IstftArgs a) {
         f32x4* r = reinterpret_cast<f32x4*>(&acc[(prb + u) & (ACC - 1)]);
         const f32x4 v = *r;
         *r = zero4;
-        if (pwrite) __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(o + u));
+        // Griffin-Lim's forward STFT reads the signal straight back: ordinary stores leave
+        // it in the caches; otherwise streaming stores
+        if (pwrite) {
+          if constexpr (GLM) *reinterpret_cast<f32x4*>(o + u) = v;
+          else __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(o + u));
+        }
       }
       if (tid < pfin - n4) {
         const int u = n4 + tid;

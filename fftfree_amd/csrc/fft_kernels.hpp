@@ -169,6 +169,8 @@ enum MetaKind : int {
 hipError_t launch_stft(const StftArgs& a, int inverse, int variant, hipStream_t s);
 // The variant launch_stft runs for `variant` < 0 (by N and frame count, on the current device).
 int default_stft_variant(const StftArgs& a);
+// The variant for planes the next launch reads straight back (ordinary stores), or -1.
+int readback_stft_variant(const StftArgs& a);
 hipError_t launch_c2c(const C2CArgs& a, int inverse, int variant, hipStream_t s);
 // Generic real-input path: frames as complex columns (N per frame), and the output
 // copy of transformed columns into planes (mag plane = re when polar).

@@ -282,6 +282,26 @@ int default_stft_variant(const StftArgs& a) {
   return v;
 }
 
+// For planes that the next launch reads straight back (Griffin-Lim's forward planes, read
+// by its next inverse): the in-place tile entries of N, whose ordinary stores leave the
+// planes in the caches where the default's streaming stores send them to HBM (at N = 1024,
+// 4093 frames: 28.4 against 33.9 us per Griffin-Lim iteration). The one with the most
+// frames per workgroup that still gives every CU a workgroup, else the fewest; -1 when N
+// has none.
+int readback_stft_variant(const StftArgs& a) {
+  const int count = count_variants(a.n);
+  const size_t cus = (size_t)device_cus();
+  int best = -1, small = -1;
+  for (int i = 0; i < count; ++i) {
+    const StftEntry* e = find_variant(a.n, i);
+    if (!std::strstr(e->info.radices, "inplace")) continue;
+    const int fpw = e->info.fpw;
+    if (small < 0 || fpw < find_variant(a.n, small)->info.fpw) small = i;
+    if ((a.frames + fpw - 1) / (size_t)fpw >= cus && (best < 0 || fpw > find_variant(a.n, best)->info.fpw)) best = i;
+  }
+  return best >= 0 ? best : small;
+}
+
 hipError_t launch_stft(const StftArgs& a, int inverse, int variant, hipStream_t s) {
   if (a.frames == 0) return hipSuccess;
   if (a.n < 16) {
